@@ -1,0 +1,67 @@
+// Shared device helpers for the flash-ph MI355X (gfx950) kernels.
+#pragma once
+
+#include <cstring>  // rocprim's texture_cache_iterator uses memset without it
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/flashph.h"
+
+#define FPH_WAVE 64
+
+#define FPH_CHECK_HIP(expr)                                  \
+    do {                                                     \
+        hipError_t err_ = (expr);                            \
+        if (err_ != hipSuccess) return FPH_ERR_HIP;          \
+    } while (0)
+
+#define FPH_LAUNCH_CHECK()                                   \
+    do {                                                     \
+        hipError_t err_ = hipGetLastError();                 \
+        if (err_ != hipSuccess) return FPH_ERR_HIP;          \
+    } while (0)
+
+static inline unsigned fph_blocks(int64_t n, int threads) {
+    int64_t b = (n + threads - 1) / threads;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+// First index in sorted x[0..n) with x[idx] >= v.
+template <typename T>
+__device__ __forceinline__ int fph_lower_bound(const T* x, int n, T v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (x[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Rank of undirected edge (s, t) from the vertex CSR, -1 if absent.
+__device__ __forceinline__ int fph_edge_rank(const int* adj_ptr, const int* adj_idx,
+                                             const int* adj_rank, int s, int t) {
+    int lo = adj_ptr[s], hi = adj_ptr[s + 1];
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        int x = adj_idx[mid];
+        if (x < t) lo = mid + 1;
+        else if (x > t) hi = mid;
+        else return adj_rank[mid];
+    }
+    return -1;
+}
+
+__device__ __forceinline__ int fph_lane() { return (int)(threadIdx.x & (FPH_WAVE - 1)); }
+
+__device__ __forceinline__ unsigned long long fph_lanemask_lt() {
+    int l = fph_lane();
+    return l == 0 ? 0ull : ((~0ull) >> (64 - l));
+}
+
+// Correctly rounded FP32 square root: sqrt in FP64 then round to FP32.  FP64
+// carries 53 >= 2*24 + 2 bits, so the double rounding is exact for sqrt.
+__device__ __forceinline__ float fph_sqrt_rn(float x) {
+    return (float)__dsqrt_rn((double)x);
+}

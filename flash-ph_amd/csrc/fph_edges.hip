@@ -1,0 +1,334 @@
+// Stage (a): Vietoris-Rips edge enumeration, filtration sort and vertex CSR.
+//
+// Replaces kernels/edge_kernel.py:23-224 (count + emit over B x B tiles),
+// rips.py:30-50,188-202 (packed key + stable argsort) and rips.py:53-95
+// (per-vertex CSR) of the reference.
+#include "fph_common.h"
+
+#include <rocprim/rocprim.hpp>
+
+namespace {
+
+constexpr int kTile = 64;     // points per tile side
+constexpr int kChunk = 32;    // dimensions staged in LDS per pass
+constexpr int kThreads = 256; // 16 x 16 threads, 4 x 4 pairs each
+
+// Tile index -> (I, J) with I <= J, tiles ordered by J then I.
+__device__ __forceinline__ void tile_coords(long long t, int* I, int* J) {
+    long long j = (long long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((j + 1) * (j + 2) / 2 <= t) j++;
+    while (j * (j + 1) / 2 > t) j--;
+    *J = (int)j;
+    *I = (int)(t - j * (j + 1) / 2);
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// One workgroup per upper-triangular 64 x 64 tile.  Each thread owns a 4 x 4
+// block of pairs (rows ty*4+r, columns tx*4+c) and accumulates
+//     acc = fma(x_i[k] - x_j[k], x_i[k] - x_j[k], acc),   k = 0 .. d-1
+// in the reference's order (edge_kernel.py:112-123 with fused multiply-add).
+// Pairs with acc < thr_sq (strict, FP32) are appended, unordered, as
+//     key = float_bits(acc) << 32 | (j(j-1)/2 + i) mod 2^32     (rips.py:41-49)
+//     val = j << 32 | i
+// through one atomic per wavefront.  The count keeps growing past `cap` so the
+// caller can size a retry.
+__global__ __launch_bounds__(kThreads) void k_edge_screen(
+    const float* __restrict__ pts, int n, int d, float thr_sq, long long tile0,
+    unsigned long long* __restrict__ keys, unsigned long long* __restrict__ vals,
+    long long cap, unsigned long long* __restrict__ count) {
+    __shared__ float4 sa[kChunk][kTile / 4];
+    __shared__ float4 sb[kChunk][kTile / 4];
+    int I, J;
+    tile_coords(tile0 + (long long)blockIdx.x, &I, &J);
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int i0 = I * kTile, j0 = J * kTile;
+
+    float acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[r][c] = 0.0f;
+
+    float* fa = reinterpret_cast<float*>(sa);
+    float* fb = reinterpret_cast<float*>(sb);
+    for (int k0 = 0; k0 < d; k0 += kChunk) {
+        const int kc = min(kChunk, d - k0);
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < kTile * kc; idx += kThreads) {
+            const int p = idx / kc, k = idx - p * kc;
+            const int gi = i0 + p, gj = j0 + p;
+            fa[k * kTile + p] = gi < n ? pts[(long long)gi * d + k0 + k] : 0.0f;
+            fb[k * kTile + p] = gj < n ? pts[(long long)gj * d + k0 + k] : 0.0f;
+        }
+        __syncthreads();
+        for (int k = 0; k < kc; k++) {
+            const float4 a = sa[k][ty];
+            const float4 b = sb[k][tx];
+            const float av[4] = {a.x, a.y, a.z, a.w};
+            const float bv[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const float diff = av[r] - bv[c];
+                    acc[r][c] = __builtin_fmaf(diff, diff, acc[r][c]);
+                }
+        }
+    }
+
+    // Selection and wave-level compaction.
+    unsigned long long bal[16];
+    int total = 0;
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int i = i0 + ty * 4 + r, j = j0 + tx * 4 + c;
+            const bool ok = (acc[r][c] < thr_sq) && i < n && j < n && (I < J || i < j);
+            bal[r * 4 + c] = __ballot(ok);
+            total += __popcll(bal[r * 4 + c]);
+        }
+    if (total == 0) return;
+    unsigned long long base = 0;
+    if (fph_lane() == 0) base = atomicAdd(count, (unsigned long long)total);
+    base = __shfl(base, 0);
+    const unsigned long long lt = fph_lanemask_lt();
+    unsigned long long pos = base;
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const unsigned long long b = bal[r * 4 + c];
+            if ((b >> fph_lane()) & 1ull) {
+                const unsigned long long at = pos + __popcll(b & lt);
+                if (at < (unsigned long long)cap) {
+                    const unsigned long long i = (unsigned long long)(i0 + ty * 4 + r);
+                    const unsigned long long j = (unsigned long long)(j0 + tx * 4 + c);
+                    unsigned bits;
+                    memcpy(&bits, &acc[r][c], 4);
+                    const unsigned long long colex = j * (j - 1) / 2 + i;
+                    keys[at] = ((unsigned long long)bits << 32) | (colex & 0xFFFFFFFFull);
+                    vals[at] = (j << 32) | i;
+                }
+            }
+            pos += __popcll(b);
+        }
+}
+
+// Reference emission order of edge (i, j): tile (i/32, j/32) row-major over
+// the 32-point block grid, then (i mod 32, j mod 32) row-major
+// (edge_kernel.py:133-149, 207-210).  Only consulted when the packed key can
+// tie (n > 92682, where j(j-1)/2 + i no longer fits in 32 bits).
+__global__ void k_emit_order(const unsigned long long* __restrict__ vals, long long E,
+                             long long nb32, unsigned long long* __restrict__ ord,
+                             unsigned* __restrict__ idx) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const unsigned long long v = vals[e];
+    const unsigned long long i = v & 0xFFFFFFFFull, j = v >> 32;
+    ord[e] = (((i >> 5) * (unsigned long long)nb32 + (j >> 5)) << 10) | ((i & 31) << 5) | (j & 31);
+    idx[e] = (unsigned)e;
+}
+
+__global__ void k_gather_kv(const unsigned long long* __restrict__ keys,
+                            const unsigned long long* __restrict__ vals,
+                            const unsigned* __restrict__ idx, long long E,
+                            unsigned long long* __restrict__ ko,
+                            unsigned long long* __restrict__ vo) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const unsigned s = idx[e];
+    ko[e] = keys[s];
+    vo[e] = vals[s];
+}
+
+__global__ void k_edge_unpack(const unsigned long long* __restrict__ keys,
+                              const unsigned long long* __restrict__ vals, long long E,
+                              int* __restrict__ ei, int* __restrict__ ej,
+                              float* __restrict__ dsq, float* __restrict__ filt) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const unsigned long long v = vals[e];
+    const unsigned bits = (unsigned)(keys[e] >> 32);
+    float f;
+    memcpy(&f, &bits, 4);
+    ei[e] = (int)(v & 0xFFFFFFFFull);
+    ej[e] = (int)(v >> 32);
+    dsq[e] = f;
+    if (filt) filt[e] = fph_sqrt_rn(f);
+}
+
+__global__ void k_half_edges(const int* __restrict__ ei, const int* __restrict__ ej, long long E,
+                             unsigned long long* __restrict__ hk, unsigned* __restrict__ hv) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const unsigned long long i = (unsigned)ei[e], j = (unsigned)ej[e];
+    hk[e] = (i << 32) | j;
+    hv[e] = (unsigned)e;
+    hk[E + e] = (j << 32) | i;
+    hv[E + e] = (unsigned)e;
+}
+
+__global__ void k_csr_unpack(const unsigned long long* __restrict__ hk,
+                             const unsigned* __restrict__ hv, long long H,
+                             const float* __restrict__ edge_dsq, int* __restrict__ adj_idx,
+                             int* __restrict__ adj_rank, float* __restrict__ adj_dsq) {
+    long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H) return;
+    const unsigned r = hv[h];
+    adj_idx[h] = (int)(hk[h] & 0xFFFFFFFFull);
+    adj_rank[h] = (int)r;
+    adj_dsq[h] = edge_dsq[r];
+}
+
+__global__ void k_csr_ptr(const unsigned long long* __restrict__ hk, long long H, long long n,
+                          int* __restrict__ ptr) {
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v > n) return;
+    const unsigned long long key = (unsigned long long)v << 32;
+    long long lo = 0, hi = H;
+    while (lo < hi) {
+        long long mid = (lo + hi) >> 1;
+        if (hk[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    ptr[v] = (int)lo;
+}
+
+inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+int bit_width_u64(unsigned long long x) {
+    int b = 0;
+    while (x) { b++; x >>= 1; }
+    return b;
+}
+
+}  // namespace
+
+extern "C" int fph_edge_screen(const float* pts, int64_t n, int d, float thr_sq,
+                               uint64_t* keys, uint64_t* vals, int64_t cap,
+                               uint64_t* count_dev, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (n < 0 || d < 1 || cap < 0 || !count_dev) return FPH_ERR_ARG;
+    if (n > 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
+    k_zero_u64<<<1, 64, 0, s>>>((unsigned long long*)count_dev, 1);
+    FPH_LAUNCH_CHECK();
+    if (n < 2) return FPH_OK;
+    const long long nb = (n + kTile - 1) / kTile;
+    const long long ntiles = nb * (nb + 1) / 2;
+    const long long kMaxGrid = 1LL << 30;
+    for (long long t0 = 0; t0 < ntiles; t0 += kMaxGrid) {
+        const long long g = std::min(kMaxGrid, ntiles - t0);
+        k_edge_screen<<<(unsigned)g, kThreads, 0, s>>>(
+            pts, (int)n, d, thr_sq, t0, (unsigned long long*)keys, (unsigned long long*)vals,
+            (long long)cap, (unsigned long long*)count_dev);
+        FPH_LAUNCH_CHECK();
+    }
+    return FPH_OK;
+}
+
+extern "C" int fph_edge_sort(const uint64_t* keys_in, const uint64_t* vals_in, int64_t E,
+                             int64_t n, uint64_t* keys_tmp, uint64_t* vals_tmp,
+                             int32_t* edge_i, int32_t* edge_j, float* edge_dsq,
+                             float* edge_filt, void* ws, size_t* ws_bytes, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (E < 0 || !ws_bytes) return FPH_ERR_ARG;
+    if (E >= 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
+    const bool tie_order = n > 92682;
+    const unsigned N = (unsigned)E;
+    // workspace layout: [sort temp | ord | ord_sorted | idx | idx_sorted]
+    size_t tmp_main = 0, tmp_ord = 0;
+    FPH_CHECK_HIP(rocprim::radix_sort_pairs(nullptr, tmp_main, (const unsigned long long*)keys_in,
+                                            (unsigned long long*)keys_tmp,
+                                            (const unsigned long long*)vals_in,
+                                            (unsigned long long*)vals_tmp, N, 0, 64, s));
+    if (tie_order) {
+        FPH_CHECK_HIP(rocprim::radix_sort_pairs(nullptr, tmp_ord, (const unsigned long long*)nullptr,
+                                                (unsigned long long*)nullptr,
+                                                (const unsigned*)nullptr, (unsigned*)nullptr, N,
+                                                0, 64, s));
+    }
+    size_t tmp = align_up(std::max(tmp_main, tmp_ord));
+    size_t need = tmp;
+    if (tie_order) need += 2 * align_up(8 * (size_t)E) + 2 * align_up(4 * (size_t)E);
+    if (!ws) { *ws_bytes = need; return FPH_OK; }
+    if (*ws_bytes < need) return FPH_ERR_ARG;
+    if (E == 0) return FPH_OK;
+    char* w = (char*)ws;
+    const unsigned long long* k_src = (const unsigned long long*)keys_in;
+    const unsigned long long* v_src = (const unsigned long long*)vals_in;
+    unsigned long long* kk = (unsigned long long*)keys_tmp;
+    unsigned long long* vv = (unsigned long long*)vals_tmp;
+    if (tie_order) {
+        unsigned long long* ord = (unsigned long long*)(w + tmp);
+        unsigned long long* ord2 = (unsigned long long*)(w + tmp + align_up(8 * (size_t)E));
+        unsigned* idx = (unsigned*)(w + tmp + 2 * align_up(8 * (size_t)E));
+        unsigned* idx2 = (unsigned*)(w + tmp + 2 * align_up(8 * (size_t)E) + align_up(4 * (size_t)E));
+        const long long nb32 = (n + 31) / 32;
+        k_emit_order<<<fph_blocks(E, 256), 256, 0, s>>>(v_src, E, nb32, ord, idx);
+        FPH_LAUNCH_CHECK();
+        size_t t2 = tmp;
+        FPH_CHECK_HIP(rocprim::radix_sort_pairs(ws, t2, ord, ord2, idx, idx2, N, 0, 64, s));
+        // keys/vals in emission order, written to the caller's tmp buffers and
+        // sorted back into ord/idx space reinterpreted as u64 pairs below.
+        k_gather_kv<<<fph_blocks(E, 256), 256, 0, s>>>(k_src, v_src, idx2, E, kk, vv);
+        FPH_LAUNCH_CHECK();
+        // Final stable sort on the packed key: emission order breaks ties.
+        unsigned long long* kout = ord;   // reuse (E u64 each)
+        unsigned long long* vout = ord2;
+        size_t t3 = tmp;
+        FPH_CHECK_HIP(rocprim::radix_sort_pairs(ws, t3, kk, kout, vv, vout, N, 0, 64, s));
+        k_edge_unpack<<<fph_blocks(E, 256), 256, 0, s>>>(kout, vout, E, edge_i, edge_j, edge_dsq,
+                                                         edge_filt);
+        FPH_LAUNCH_CHECK();
+        return FPH_OK;
+    }
+    size_t t1 = tmp;
+    FPH_CHECK_HIP(rocprim::radix_sort_pairs(ws, t1, k_src, kk, v_src, vv, N, 0, 64, s));
+    k_edge_unpack<<<fph_blocks(E, 256), 256, 0, s>>>(kk, vv, E, edge_i, edge_j, edge_dsq,
+                                                     edge_filt);
+    FPH_LAUNCH_CHECK();
+    return FPH_OK;
+}
+
+extern "C" int fph_vertex_csr(const int32_t* edge_i, const int32_t* edge_j,
+                              const float* edge_dsq, int64_t E, int64_t n, int32_t* adj_ptr,
+                              int32_t* adj_idx, int32_t* adj_rank, float* adj_dsq, void* ws,
+                              size_t* ws_bytes, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (E < 0 || n < 0 || !ws_bytes) return FPH_ERR_ARG;
+    if (E >= 0x3FFFFFFFLL) return FPH_ERR_LIMIT;
+    const size_t H = 2 * (size_t)E;
+    size_t tmp = 0;
+    const int end_bit = 32 + std::max(1, bit_width_u64((unsigned long long)std::max<int64_t>(n, 1)));
+    FPH_CHECK_HIP(rocprim::radix_sort_pairs(nullptr, tmp, (const unsigned long long*)nullptr,
+                                            (unsigned long long*)nullptr, (const unsigned*)nullptr,
+                                            (unsigned*)nullptr, (unsigned)H, 0, end_bit, s));
+    tmp = align_up(tmp);
+    const size_t need = tmp + 2 * align_up(8 * H) + 2 * align_up(4 * H);
+    if (!ws) { *ws_bytes = need; return FPH_OK; }
+    if (*ws_bytes < need) return FPH_ERR_ARG;
+    char* w = (char*)ws;
+    unsigned long long* hk = (unsigned long long*)(w + tmp);
+    unsigned long long* hk2 = (unsigned long long*)(w + tmp + align_up(8 * H));
+    unsigned* hv = (unsigned*)(w + tmp + 2 * align_up(8 * H));
+    unsigned* hv2 = (unsigned*)(w + tmp + 2 * align_up(8 * H) + align_up(4 * H));
+    if (E > 0) {
+        k_half_edges<<<fph_blocks(E, 256), 256, 0, s>>>(edge_i, edge_j, E, hk, hv);
+        FPH_LAUNCH_CHECK();
+        size_t t1 = tmp;
+        FPH_CHECK_HIP(rocprim::radix_sort_pairs(ws, t1, hk, hk2, hv, hv2, (unsigned)H, 0, end_bit, s));
+        k_csr_unpack<<<fph_blocks((int64_t)H, 256), 256, 0, s>>>(hk2, hv2, (long long)H, edge_dsq,
+                                                                adj_idx, adj_rank, adj_dsq);
+        FPH_LAUNCH_CHECK();
+    }
+    k_csr_ptr<<<fph_blocks(n + 1, 256), 256, 0, s>>>(hk2, (long long)H, n, adj_ptr);
+    FPH_LAUNCH_CHECK();
+    return FPH_OK;
+}
+
+extern "C" const char* fph_version(void) { return "flashph-mi355x 0.1.0 (gfx950)"; }

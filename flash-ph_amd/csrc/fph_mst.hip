@@ -1,0 +1,215 @@
+// Stage (b): Boruvka minimum spanning forest and the H0 diagram.
+//
+// Replaces boruvka.py:17-113 and the H0 assembly of persistence.py:169-192.
+// Edge weights are the filtration ranks 0..E-1 (distinct), so the forest is
+// unique and equals Kruskal by rank.  The round loop is issued without host
+// synchronisation: each kernel reads a device flag and returns at once after
+// convergence.
+#include "fph_common.h"
+
+#include <rocprim/rocprim.hpp>
+
+namespace {
+
+constexpr unsigned kNone = 0xFFFFFFFFu;
+
+struct BvkState {
+    int* comp;        // current root of each vertex's component
+    int* nxt;         // hook pointer of roots during a round
+    unsigned* min_e;  // lightest crossing edge per root
+    int* flags;       // [0] = done, [1 + r] = crossing edge seen in round r
+};
+
+__global__ void k_bvk_init(BvkState st, int n, unsigned char* __restrict__ mst, long long E,
+                           int rounds) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        st.comp[i] = (int)i;
+        st.nxt[i] = (int)i;
+        st.min_e[i] = kNone;
+    }
+    if (i < E) mst[i] = 0;
+    if (i < rounds + 1) st.flags[i] = 0;
+}
+
+// Every crossing edge offers itself to both endpoint components (atomicMin on
+// rank); the hook pointers of the previous round are reset here.
+__global__ void k_bvk_find(BvkState st, const int* __restrict__ ei, const int* __restrict__ ej,
+                           long long E, int n, int round) {
+    if (st.flags[0]) return;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) st.nxt[i] = (int)i;
+    if (i >= E) return;
+    const int cu = st.comp[ei[i]], cv = st.comp[ej[i]];
+    if (cu == cv) return;
+    atomicMin(&st.min_e[cu], (unsigned)i);
+    atomicMin(&st.min_e[cv], (unsigned)i);
+    st.flags[1 + round] = 1;
+}
+
+// Each root adds its lightest crossing edge to the forest and points at the
+// component across it.  A mutual pair (both roots chose the same edge) keeps
+// the smaller root, so the hook graph is a forest.
+__global__ void k_bvk_hook(BvkState st, const int* __restrict__ ei, const int* __restrict__ ej,
+                           int n, int round, unsigned char* __restrict__ mst) {
+    if (st.flags[0]) return;
+    if (!st.flags[1 + round]) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) st.flags[0] = 1;
+        return;
+    }
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n || st.comp[c] != c) return;
+    const unsigned e = st.min_e[c];
+    if (e == kNone) return;
+    mst[e] = 1;
+    const int cu = st.comp[ei[e]], cv = st.comp[ej[e]];
+    const int other = (cu == c) ? cv : cu;
+    if (st.min_e[other] == e && other > c) return;  // mutual: c stays root
+    st.nxt[c] = other;
+}
+
+// Pointer jumping with path halving: every vertex walks its root's hook chain
+// to the new root.  Concurrent halving writes always store an ancestor, so
+// racing readers still see a valid chain.
+__global__ void k_bvk_jump(BvkState st, int n) {
+    if (st.flags[0]) return;
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    int r = st.comp[v];
+    // The hook graph is a forest (distinct ranks), so every walk ends at a
+    // root within n steps; the bound only guards against corrupted input.
+    for (int it = 0; it < n; it++) {
+        const int p = st.nxt[r];
+        if (p == r) break;
+        const int gp = st.nxt[p];
+        if (gp != p) st.nxt[r] = gp;
+        r = p;
+    }
+    st.comp[v] = r;
+    st.min_e[v] = kNone;
+}
+
+__global__ void k_bvk_minlabel_init(unsigned* __restrict__ lab, int n) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) lab[v] = kNone;
+}
+
+__global__ void k_bvk_minlabel(const int* __restrict__ comp, unsigned* __restrict__ lab, int n) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) atomicMin(&lab[comp[v]], (unsigned)v);
+}
+
+// Component label = smallest vertex of the component (boruvka.py:91-99 hooks
+// the larger root under the smaller one, so its roots are component minima).
+__global__ void k_bvk_label(const int* __restrict__ comp, const unsigned* __restrict__ lab,
+                            long long* __restrict__ out, int n, const int* __restrict__ flags,
+                            int rounds, int* __restrict__ converged) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n && out) out[v] = (long long)lab[comp[v]];
+    if (v == 0 && converged) converged[0] = flags[0] ? 1 : (flags[rounds] ? 0 : 1);
+}
+
+__global__ void k_u8_to_i32(const unsigned char* __restrict__ m, long long E, int* __restrict__ o) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < E) o[e] = m[e];
+}
+
+// H0 rows: (0, sqrt(dsq)) for forest edges in rank order, then (0, inf) for
+// each component (persistence.py:172-188).  Exactly n rows.
+__global__ void k_h0_write(const unsigned char* __restrict__ mst, const int* __restrict__ pos,
+                           const float* __restrict__ dsq, long long E, long long n,
+                           float* __restrict__ h0, long long* __restrict__ mst_idx,
+                           long long* __restrict__ n_mst) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nm = E > 0 ? (long long)pos[E - 1] + mst[E - 1] : 0;
+    if (i < E && mst[i] && pos[i] < n) {
+        const long long p = pos[i];
+        h0[2 * p] = 0.0f;
+        h0[2 * p + 1] = fph_sqrt_rn(dsq[i]);
+        if (mst_idx) mst_idx[p] = i;
+    }
+    if (i < n && i >= nm) {
+        h0[2 * i] = 0.0f;
+        h0[2 * i + 1] = __builtin_inff();
+    }
+    if (i == 0 && n_mst) n_mst[0] = nm;
+}
+
+inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace
+
+extern "C" int fph_boruvka_rounds(int64_t n) {
+    int64_t m = n < 2 ? 2 : n;
+    int r = 0;
+    while ((1LL << r) < m) r++;
+    return r + 2;
+}
+
+extern "C" size_t fph_boruvka_scratch_bytes(int64_t n) {
+    const int R = fph_boruvka_rounds(n);
+    return 4 * align_up(4 * (size_t)std::max<int64_t>(n, 1)) + align_up(4 * (size_t)(R + 2));
+}
+
+extern "C" int fph_boruvka_mst(const int32_t* edge_i, const int32_t* edge_j, int64_t E,
+                               int64_t n, uint8_t* mst_mask, int64_t* comp_label,
+                               int32_t* converged, void* scratch, size_t scratch_bytes,
+                               void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (E < 0 || n < 1 || !scratch) return FPH_ERR_ARG;
+    if (n > 0x7FFFFFFFLL || E >= 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
+    if (scratch_bytes < fph_boruvka_scratch_bytes(n)) return FPH_ERR_ARG;
+    const int R = fph_boruvka_rounds(n);
+    char* w = (char*)scratch;
+    const size_t vb = align_up(4 * (size_t)n);
+    BvkState st;
+    st.comp = (int*)w;
+    st.nxt = (int*)(w + vb);
+    st.min_e = (unsigned*)(w + 2 * vb);
+    unsigned* lab = (unsigned*)(w + 3 * vb);
+    st.flags = (int*)(w + 4 * vb);
+    const long long span = std::max<long long>(std::max<long long>(E, n), R + 1);
+    k_bvk_init<<<fph_blocks(span, 256), 256, 0, s>>>(st, (int)n, mst_mask, E, R);
+    FPH_LAUNCH_CHECK();
+    const long long fspan = std::max<long long>(E, n);
+    for (int r = 0; r < R; r++) {
+        k_bvk_find<<<fph_blocks(fspan, 256), 256, 0, s>>>(st, edge_i, edge_j, E, (int)n, r);
+        k_bvk_hook<<<fph_blocks(n, 256), 256, 0, s>>>(st, edge_i, edge_j, (int)n, r, mst_mask);
+        k_bvk_jump<<<fph_blocks(n, 256), 256, 0, s>>>(st, (int)n);
+    }
+    FPH_LAUNCH_CHECK();
+    k_bvk_minlabel_init<<<fph_blocks(n, 256), 256, 0, s>>>(lab, (int)n);
+    k_bvk_minlabel<<<fph_blocks(n, 256), 256, 0, s>>>(st.comp, lab, (int)n);
+    k_bvk_label<<<fph_blocks(n, 256), 256, 0, s>>>(st.comp, lab, (long long*)comp_label, (int)n,
+                                                  st.flags, R, converged);
+    FPH_LAUNCH_CHECK();
+    return FPH_OK;
+}
+
+extern "C" int fph_h0_diagram(const uint8_t* mst_mask, const float* edge_dsq, int64_t E,
+                              int64_t n, float* h0, int64_t* mst_idx, int64_t* n_mst,
+                              void* ws, size_t* ws_bytes, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (E < 0 || n < 1 || !ws_bytes) return FPH_ERR_ARG;
+    size_t tmp = 0;
+    FPH_CHECK_HIP(rocprim::exclusive_scan(nullptr, tmp, (const int*)nullptr, (int*)nullptr, 0,
+                                          (size_t)std::max<int64_t>(E, 1), rocprim::plus<int>(), s));
+    tmp = align_up(tmp);
+    const size_t need = tmp + 2 * align_up(4 * (size_t)std::max<int64_t>(E, 1));
+    if (!ws) { *ws_bytes = need; return FPH_OK; }
+    if (*ws_bytes < need) return FPH_ERR_ARG;
+    char* w = (char*)ws;
+    int* m32 = (int*)(w + tmp);
+    int* pos = (int*)(w + tmp + align_up(4 * (size_t)std::max<int64_t>(E, 1)));
+    if (E > 0) {
+        k_u8_to_i32<<<fph_blocks(E, 256), 256, 0, s>>>(mst_mask, E, m32);
+        FPH_LAUNCH_CHECK();
+        size_t t1 = tmp;
+        FPH_CHECK_HIP(rocprim::exclusive_scan(ws, t1, m32, pos, 0, (size_t)E, rocprim::plus<int>(), s));
+    }
+    const long long span = std::max<long long>(E, n);
+    k_h0_write<<<fph_blocks(span, 256), 256, 0, s>>>(mst_mask, pos, edge_dsq, E, n, h0,
+                                                    (long long*)mst_idx, (long long*)n_mst);
+    FPH_LAUNCH_CHECK();
+    return FPH_OK;
+}

@@ -1,0 +1,583 @@
+// Stages (c) and (d): triangles, tetrahedra, cofacet columns and apparent
+// pairs.
+//
+// Triangles and tetrahedra are enumerated per *maximal edge* in filtration
+// order.  For a triangle whose largest-rank edge is e = (u, v), the colex
+// order C(c,3)+C(b,2)+a of its sorted vertices equals the ascending order of
+// its third vertex w; for a tetrahedron whose largest edge is e, the colex4
+// order equals the (y, x) order of its other two vertices x < y.  Hence the
+// reference's stable (max_rank, colex) ranking (reduce_h1_gpu.py:509-521,
+// apparent_pair_h2_kernel.py:240-249) is produced directly by a count / scan
+// / emit over edges, with no global sort:
+//     rank(triangle) = E     + tri_off[e] + (index of w in e's lower link)
+//     rank(tetra)    = E + T + tet_off[e] + (index of (y, x) in e's list)
+// Vertex lists per edge are stored sorted (tri_w: w; tet_key: y << 16 | x),
+// so any simplex's rank is found with one binary search.
+#include "fph_common.h"
+
+#include <rocprim/rocprim.hpp>
+
+namespace {
+
+inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+// ---------------------------------------------------------------------------
+// Triangles (replaces triangle_kernel.py:28-144 and reduce_h1_gpu.py:65-100,
+// 509-521).
+// ---------------------------------------------------------------------------
+
+// own[e]: triangles whose largest edge is e (both other edges older);
+// cof[e]: all triangles containing e (0 for forest edges, whose columns are
+// cleared by H0).
+__global__ void k_tri_count(const int* __restrict__ ei, const int* __restrict__ ej, long long E,
+                            const int* __restrict__ aptr, const int* __restrict__ aidx,
+                            const int* __restrict__ arank, const unsigned char* __restrict__ mst,
+                            int* __restrict__ own_cnt, int* __restrict__ cof_cnt) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int u = ei[e], v = ej[e];
+    int iu = aptr[u], eu = aptr[u + 1], iv = aptr[v], ev = aptr[v + 1];
+    int own = 0, cof = 0;
+    while (iu < eu && iv < ev) {
+        const int a = aidx[iu], b = aidx[iv];
+        if (a == b) {
+            cof++;
+            if (arank[iu] < e && arank[iv] < e) own++;
+            iu++;
+            iv++;
+        } else if (a < b) {
+            iu++;
+        } else {
+            iv++;
+        }
+    }
+    own_cnt[e] = own;
+    cof_cnt[e] = mst[e] ? 0 : cof;
+}
+
+// Emit e's own triangles in ascending third vertex w: w, the owning edge, the
+// ranks of (u, w) and (v, w), and cmf = the largest rank among the
+// triangle's non-forest faces (-1 if none) -- the quantity the reference
+// computes by scatter-max (reduce_h1_gpu.py:186-197).
+__global__ void k_tri_emit(const int* __restrict__ ei, const int* __restrict__ ej, long long E,
+                           const int* __restrict__ aptr, const int* __restrict__ aidx,
+                           const int* __restrict__ arank, const unsigned char* __restrict__ mst,
+                           const int* __restrict__ tri_off, int* __restrict__ tri_w,
+                           int* __restrict__ tri_e, int* __restrict__ tri_ru,
+                           int* __restrict__ tri_rv, int* __restrict__ tri_cmf) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int u = ei[e], v = ej[e];
+    int iu = aptr[u], eu = aptr[u + 1], iv = aptr[v], ev = aptr[v + 1];
+    int t = tri_off[e];
+    const bool me = mst[e] != 0;
+    while (iu < eu && iv < ev) {
+        const int a = aidx[iu], b = aidx[iv];
+        if (a == b) {
+            const int r1 = arank[iu], r2 = arank[iv];
+            if (r1 < e && r2 < e) {
+                tri_w[t] = a;
+                tri_e[t] = (int)e;
+                tri_ru[t] = r1;
+                tri_rv[t] = r2;
+                int cmf = -1;
+                if (!me) cmf = (int)e;
+                else {
+                    if (!mst[r1]) cmf = r1;
+                    if (!mst[r2] && r2 > cmf) cmf = r2;
+                }
+                tri_cmf[t] = cmf;
+                t++;
+            }
+            iu++;
+            iv++;
+        } else if (a < b) {
+            iu++;
+        } else {
+            iv++;
+        }
+    }
+}
+
+// Cofacet column of every non-forest edge: the global ranks of all triangles
+// containing it (unsorted here; sorted per row afterwards).  Replaces the
+// face lookup and two-level argsort of reduce_h1_gpu.py:65-152.
+__global__ void k_cof1_emit(const int* __restrict__ ei, const int* __restrict__ ej, long long E,
+                            const int* __restrict__ aptr, const int* __restrict__ aidx,
+                            const int* __restrict__ arank, const unsigned char* __restrict__ mst,
+                            const int* __restrict__ tri_off, const int* __restrict__ tri_w,
+                            const int* __restrict__ cof_off, unsigned* __restrict__ cof_raw) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E || mst[e]) return;
+    const int u = ei[e], v = ej[e];
+    int iu = aptr[u], eu = aptr[u + 1], iv = aptr[v], ev = aptr[v + 1];
+    int k = cof_off[e], kown = 0;
+    const long long base = E;
+    while (iu < eu && iv < ev) {
+        const int a = aidx[iu], b = aidx[iv];
+        if (a == b) {
+            const int r1 = arank[iu], r2 = arank[iv];
+            long long rank;
+            if (r1 < e && r2 < e) {
+                rank = base + tri_off[e] + kown;
+                kown++;
+            } else {
+                const int m = r1 > r2 ? r1 : r2;
+                const int third = r1 > r2 ? v : u;
+                const int lo = tri_off[m], len = tri_off[m + 1] - lo;
+                rank = base + lo + fph_lower_bound(tri_w + lo, len, third);
+            }
+            cof_raw[k++] = (unsigned)rank;
+            iu++;
+            iv++;
+        } else if (a < b) {
+            iu++;
+        } else {
+            iv++;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Apparent pairs (replaces reduce_h1_gpu.py:159-223; stage (c)).
+// kind: 0 cleared, 1 no cofacet (essential), 2 apparent, 3 residual.
+// ---------------------------------------------------------------------------
+
+__global__ void k_fill_i32(int* __restrict__ p, long long n, int v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u8(unsigned char* __restrict__ p, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void k_apparent(long long K, long long col_base, const int* __restrict__ off,
+                           const unsigned* __restrict__ data, const unsigned char* __restrict__ skip,
+                           const int* __restrict__ row_cmf, long long row_base,
+                           unsigned char* __restrict__ kind, int* __restrict__ own,
+                           int* __restrict__ pivot, int* __restrict__ res_list,
+                           int* __restrict__ res_count) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
+    int p = -1;
+    unsigned char kd;
+    if (skip && skip[c]) {
+        kd = 0;
+    } else if (off[c + 1] == off[c]) {
+        kd = 1;
+    } else {
+        const unsigned y = data[off[c]];
+        if ((long long)row_cmf[y - row_base] == col_base + c) {
+            kd = 2;
+            own[y - row_base] = (int)c;
+            p = (int)y;
+        } else {
+            kd = 3;
+            res_list[atomicAdd(res_count, 1)] = (int)c;
+        }
+    }
+    kind[c] = kd;
+    pivot[c] = p;
+}
+
+// H1 -> H2 clearing: a triangle that is the pivot of an H1 column (apparent
+// or reduced, zero persistence included) is not an H2 column
+// (cohomology_general.py:171-173).
+__global__ void k_clear_pivots(long long K, const unsigned char* __restrict__ kind,
+                               const int* __restrict__ pivot, long long row_base,
+                               unsigned char* __restrict__ cleared) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
+    const unsigned char kd = kind[c];
+    if ((kd == 2 || kd == 3) && pivot[c] >= 0) cleared[pivot[c] - row_base] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// Tetrahedra per maximal edge (replaces triangle_kernel.py:152-270 and the
+// colex4 ranking of apparent_pair_h2_kernel.py:52-58).
+// ---------------------------------------------------------------------------
+
+__global__ void k_tet_count(long long E, const int* __restrict__ aptr, const int* __restrict__ aidx,
+                            const int* __restrict__ arank, const int* __restrict__ tri_off,
+                            const int* __restrict__ tri_w, int* __restrict__ cnt) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int lo = tri_off[e], len = tri_off[e + 1] - lo;
+    int c = 0;
+    for (int b = 1; b < len; b++) {
+        const int y = tri_w[lo + b];
+        for (int a = 0; a < b; a++) {
+            const int r = fph_edge_rank(aptr, aidx, arank, y, tri_w[lo + a]);
+            if (r >= 0 && r < e) c++;
+        }
+    }
+    cnt[e] = c;
+}
+
+__device__ __forceinline__ int tri_rank_local(const int* tri_off, const int* tri_w, int m,
+                                              int third) {
+    const int lo = tri_off[m];
+    return lo + fph_lower_bound(tri_w + lo, tri_off[m + 1] - lo, third);
+}
+
+// Local rank of the triangle {s, x, y} given the ranks of its three edges
+// (s,x), (s,y), (x,y): find the largest edge and look up the third vertex.
+__device__ __forceinline__ int tri_face_local(const int* tri_off, const int* tri_w, int s, int x,
+                                              int y, int rsx, int rsy, int rxy) {
+    if (rsx > rsy && rsx > rxy) return tri_rank_local(tri_off, tri_w, rsx, y);
+    if (rsy > rxy) return tri_rank_local(tri_off, tri_w, rsy, x);
+    return tri_rank_local(tri_off, tri_w, rxy, s);
+}
+
+__global__ void k_tet_emit(long long E, const int* __restrict__ ei, const int* __restrict__ ej,
+                           const int* __restrict__ aptr, const int* __restrict__ aidx,
+                           const int* __restrict__ arank, const int* __restrict__ tri_off,
+                           const int* __restrict__ tri_w, const int* __restrict__ tri_ru,
+                           const int* __restrict__ tri_rv, const unsigned char* __restrict__ cleared,
+                           const int* __restrict__ tet_off, unsigned* __restrict__ tet_key,
+                           int* __restrict__ tet_e, int* __restrict__ tet_cmf) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int u = ei[e], v = ej[e];
+    const int lo = tri_off[e], len = tri_off[e + 1] - lo;
+    int q = tet_off[e];
+    for (int b = 1; b < len; b++) {
+        const int y = tri_w[lo + b];
+        for (int a = 0; a < b; a++) {
+            const int x = tri_w[lo + a];
+            const int rxy = fph_edge_rank(aptr, aidx, arank, y, x);
+            if (rxy < 0 || rxy >= e) continue;
+            // faces as local triangle ranks, largest first
+            const int f_uvy = lo + b, f_uvx = lo + a;
+            const int f_uxy = tri_face_local(tri_off, tri_w, u, x, y, tri_ru[lo + a],
+                                             tri_ru[lo + b], rxy);
+            const int f_vxy = tri_face_local(tri_off, tri_w, v, x, y, tri_rv[lo + a],
+                                             tri_rv[lo + b], rxy);
+            long long cmf = -1;
+            if (!cleared[f_uvy]) cmf = f_uvy;
+            else if (!cleared[f_uvx]) cmf = f_uvx;
+            else {
+                if (!cleared[f_uxy]) cmf = f_uxy;
+                if (!cleared[f_vxy] && f_vxy > cmf) cmf = f_vxy;
+            }
+            tet_key[q] = ((unsigned)y << 16) | (unsigned)x;
+            tet_e[q] = (int)e;
+            tet_cmf[q] = cmf < 0 ? -1 : (int)(E + cmf);
+            q++;
+        }
+    }
+}
+
+// Cofacet column of every uncleared triangle {u, v, w} (u, v the ends of its
+// largest edge): tetrahedra {u, v, w, d}, d a common neighbour of all three.
+__global__ void k_cof2_count(long long T, const int* __restrict__ ei, const int* __restrict__ ej,
+                             const int* __restrict__ aptr, const int* __restrict__ aidx,
+                             const int* __restrict__ tri_w, const int* __restrict__ tri_e,
+                             const unsigned char* __restrict__ cleared, int* __restrict__ cnt) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    if (cleared[t]) { cnt[t] = 0; return; }
+    const int e = tri_e[t];
+    const int u = ei[e], v = ej[e], w = tri_w[t];
+    int ia = aptr[u], ea = aptr[u + 1], ib = aptr[v], eb = aptr[v + 1], ic = aptr[w],
+        ec = aptr[w + 1];
+    int c = 0;
+    while (ia < ea && ib < eb && ic < ec) {
+        const int na = aidx[ia], nb = aidx[ib], nc = aidx[ic];
+        const int mx = max(na, max(nb, nc));
+        if (na == mx && nb == mx && nc == mx) {
+            c++;
+            ia++; ib++; ic++;
+        } else {
+            if (na < mx) ia++;
+            if (nb < mx) ib++;
+            if (nc < mx) ic++;
+        }
+    }
+    cnt[t] = c;
+}
+
+__device__ __forceinline__ long long tet_rank_in(const int* tet_off, const unsigned* tet_key,
+                                                 int m, int p, int q) {
+    const unsigned key = p > q ? (((unsigned)p << 16) | (unsigned)q)
+                               : (((unsigned)q << 16) | (unsigned)p);
+    const int lo = tet_off[m];
+    return lo + fph_lower_bound(tet_key + lo, tet_off[m + 1] - lo, key);
+}
+
+__global__ void k_cof2_emit(long long T, long long E, const int* __restrict__ ei,
+                            const int* __restrict__ ej, const int* __restrict__ aptr,
+                            const int* __restrict__ aidx, const int* __restrict__ arank,
+                            const int* __restrict__ tri_w, const int* __restrict__ tri_e,
+                            const unsigned char* __restrict__ cleared,
+                            const int* __restrict__ tet_off, const unsigned* __restrict__ tet_key,
+                            const int* __restrict__ cof_off, unsigned* __restrict__ cof_raw) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T || cleared[t]) return;
+    const int e = tri_e[t];
+    const int u = ei[e], v = ej[e], w = tri_w[t];
+    int ia = aptr[u], ea = aptr[u + 1], ib = aptr[v], eb = aptr[v + 1], ic = aptr[w],
+        ec = aptr[w + 1];
+    int k = cof_off[t];
+    const long long base = E + T;
+    while (ia < ea && ib < eb && ic < ec) {
+        const int na = aidx[ia], nb = aidx[ib], nc = aidx[ic];
+        const int mx = max(na, max(nb, nc));
+        if (na == mx && nb == mx && nc == mx) {
+            const int d = na;
+            const int rud = arank[ia], rvd = arank[ib], rwd = arank[ic];
+            long long local;
+            if (rud < e && rvd < e && rwd < e) local = tet_rank_in(tet_off, tet_key, e, w, d);
+            else if (rud > rvd && rud > rwd) local = tet_rank_in(tet_off, tet_key, rud, v, w);
+            else if (rvd > rwd) local = tet_rank_in(tet_off, tet_key, rvd, u, w);
+            else local = tet_rank_in(tet_off, tet_key, rwd, u, v);
+            cof_raw[k++] = (unsigned)(base + local);
+            ia++; ib++; ic++;
+        } else {
+            if (na < mx) ia++;
+            if (nb < mx) ib++;
+            if (nc < mx) ic++;
+        }
+    }
+}
+
+// counts[0..K) -> off[0..K], total into *total_dev (int64).
+int scan_counts(const int* counts, int* off, long long K, long long* total_dev, void* tmp,
+                size_t tmp_bytes, hipStream_t s);
+
+__global__ void k_scan_finish(const int* __restrict__ counts, int* __restrict__ off, long long K,
+                              long long* __restrict__ total) {
+    // off[K] = off[K-1] + counts[K-1]
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const long long tot = K > 0 ? (long long)off[K - 1] + counts[K - 1] : 0;
+        off[K] = (int)tot;
+        if (total) total[0] = tot;
+    }
+}
+
+size_t scan_bytes(long long K) {
+    size_t tmp = 0;
+    rocprim::exclusive_scan(nullptr, tmp, (const int*)nullptr, (int*)nullptr, 0,
+                            (size_t)std::max<long long>(K, 1), rocprim::plus<int>(), (hipStream_t)0);
+    return align_up(tmp);
+}
+
+int scan_counts(const int* counts, int* off, long long K, long long* total_dev, void* tmp,
+                size_t tmp_bytes, hipStream_t s) {
+    if (K > 0) {
+        size_t t = tmp_bytes;
+        FPH_CHECK_HIP(rocprim::exclusive_scan(tmp, t, counts, off, 0, (size_t)K,
+                                              rocprim::plus<int>(), s));
+    }
+    k_scan_finish<<<1, 64, 0, s>>>(counts, off, K, (long long*)total_dev);
+    FPH_LAUNCH_CHECK();
+    return FPH_OK;
+}
+
+size_t segsort_bytes(long long C, long long K) {
+    size_t tmp = 0;
+    rocprim::segmented_radix_sort_keys(nullptr, tmp, (const unsigned*)nullptr, (unsigned*)nullptr,
+                                       (unsigned)std::max<long long>(C, 1),
+                                       (unsigned)std::max<long long>(K, 1), (const int*)nullptr,
+                                       (const int*)nullptr, 0, 32, (hipStream_t)0);
+    return align_up(tmp);
+}
+
+int bit_width(unsigned long long x) {
+    int b = 0;
+    while (x) { b++; x >>= 1; }
+    return b;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// C-ABI
+// ----------------------------------------------------------------------------
+
+extern "C" int fph_triangle_count(const int32_t* edge_i, const int32_t* edge_j, int64_t E,
+                                  const int32_t* adj_ptr, const int32_t* adj_idx,
+                                  const int32_t* adj_rank, const uint8_t* mst_mask,
+                                  int32_t* tri_off, int32_t* cof_off, int64_t* totals,
+                                  void* ws, size_t* ws_bytes, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (E < 0 || !ws_bytes) return FPH_ERR_ARG;
+    const size_t tmp = scan_bytes(E);
+    const size_t need = tmp + 2 * align_up(4 * (size_t)std::max<int64_t>(E, 1));
+    if (!ws) { *ws_bytes = need; return FPH_OK; }
+    if (*ws_bytes < need) return FPH_ERR_ARG;
+    char* w = (char*)ws;
+    int* own = (int*)(w + tmp);
+    int* cof = (int*)(w + tmp + align_up(4 * (size_t)std::max<int64_t>(E, 1)));
+    if (E > 0) {
+        k_tri_count<<<fph_blocks(E, 256), 256, 0, s>>>(edge_i, edge_j, E, adj_ptr, adj_idx,
+                                                      adj_rank, mst_mask, own, cof);
+        FPH_LAUNCH_CHECK();
+    }
+    int rc = scan_counts(own, tri_off, E, (long long*)totals, ws, tmp, s);
+    if (rc) return rc;
+    return scan_counts(cof, cof_off, E, (long long*)totals + 1, ws, tmp, s);
+}
+
+extern "C" int fph_triangle_emit(const int32_t* edge_i, const int32_t* edge_j, int64_t E,
+                                 const int32_t* adj_ptr, const int32_t* adj_idx,
+                                 const int32_t* adj_rank, const uint8_t* mst_mask,
+                                 const int32_t* tri_off, int32_t* tri_w, int32_t* tri_e,
+                                 int32_t* tri_ru, int32_t* tri_rv, int32_t* tri_cmf,
+                                 void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (E < 0) return FPH_ERR_ARG;
+    if (E == 0) return FPH_OK;
+    k_tri_emit<<<fph_blocks(E, 256), 256, 0, s>>>(edge_i, edge_j, E, adj_ptr, adj_idx, adj_rank,
+                                                 mst_mask, tri_off, tri_w, tri_e, tri_ru, tri_rv,
+                                                 tri_cmf);
+    FPH_LAUNCH_CHECK();
+    return FPH_OK;
+}
+
+extern "C" int fph_cofacets_h1(const int32_t* edge_i, const int32_t* edge_j, int64_t E,
+                               int64_t T, const int32_t* adj_ptr, const int32_t* adj_idx,
+                               const int32_t* adj_rank, const uint8_t* mst_mask,
+                               const int32_t* tri_off, const int32_t* tri_w,
+                               const int32_t* cof_off, int64_t C, uint32_t* cof_data, void* ws,
+                               size_t* ws_bytes, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (E < 0 || C < 0 || !ws_bytes) return FPH_ERR_ARG;
+    if (E + T >= 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
+    const size_t tmp = segsort_bytes(C, E);
+    const size_t need = tmp + align_up(4 * (size_t)std::max<int64_t>(C, 1));
+    if (!ws) { *ws_bytes = need; return FPH_OK; }
+    if (*ws_bytes < need) return FPH_ERR_ARG;
+    if (C == 0) return FPH_OK;
+    unsigned* raw = (unsigned*)((char*)ws + tmp);
+    k_cof1_emit<<<fph_blocks(E, 256), 256, 0, s>>>(edge_i, edge_j, E, adj_ptr, adj_idx, adj_rank,
+                                                  mst_mask, tri_off, tri_w, cof_off, raw);
+    FPH_LAUNCH_CHECK();
+    size_t t = tmp;
+    const int end_bit = std::max(1, bit_width((unsigned long long)(E + T)));
+    FPH_CHECK_HIP(rocprim::segmented_radix_sort_keys(ws, t, raw, (unsigned*)cof_data, (unsigned)C,
+                                                     (unsigned)E, cof_off, cof_off + 1, 0, end_bit,
+                                                     s));
+    return FPH_OK;
+}
+
+extern "C" int fph_apparent(int64_t K, int64_t col_base, const int32_t* col_off,
+                            const uint32_t* col_data, const uint8_t* skip_mask,
+                            const int32_t* row_cmf, int64_t row_base, int64_t nrows,
+                            uint8_t* kind, int32_t* own, int32_t* pivot, int32_t* res_list,
+                            int32_t* res_count, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (K < 0 || nrows < 0) return FPH_ERR_ARG;
+    if (nrows > 0) {
+        k_fill_i32<<<fph_blocks(nrows, 256), 256, 0, s>>>(own, nrows, -1);
+        FPH_LAUNCH_CHECK();
+    }
+    k_fill_i32<<<1, 64, 0, s>>>(res_count, 1, 0);
+    FPH_LAUNCH_CHECK();
+    if (K == 0) return FPH_OK;
+    k_apparent<<<fph_blocks(K, 256), 256, 0, s>>>(K, col_base, col_off, col_data, skip_mask,
+                                                 row_cmf, row_base, kind, own, pivot, res_list,
+                                                 res_count);
+    FPH_LAUNCH_CHECK();
+    return FPH_OK;
+}
+
+extern "C" int fph_clear_pivots(int64_t K, const uint8_t* kind, const int32_t* pivot,
+                                int64_t row_base, int64_t nrows, uint8_t* cleared,
+                                void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (K < 0 || nrows < 0) return FPH_ERR_ARG;
+    if (nrows > 0) {
+        k_fill_u8<<<fph_blocks(nrows, 256), 256, 0, s>>>(cleared, nrows);
+        FPH_LAUNCH_CHECK();
+    }
+    if (K == 0) return FPH_OK;
+    k_clear_pivots<<<fph_blocks(K, 256), 256, 0, s>>>(K, kind, pivot, row_base, cleared);
+    FPH_LAUNCH_CHECK();
+    return FPH_OK;
+}
+
+extern "C" int fph_tet_count(int64_t E, const int32_t* adj_ptr, const int32_t* adj_idx,
+                             const int32_t* adj_rank, const int32_t* tri_off,
+                             const int32_t* tri_w, int32_t* tet_off, int64_t* total, void* ws,
+                             size_t* ws_bytes, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (E < 0 || !ws_bytes) return FPH_ERR_ARG;
+    const size_t tmp = scan_bytes(E);
+    const size_t need = tmp + align_up(4 * (size_t)std::max<int64_t>(E, 1));
+    if (!ws) { *ws_bytes = need; return FPH_OK; }
+    if (*ws_bytes < need) return FPH_ERR_ARG;
+    int* cnt = (int*)((char*)ws + tmp);
+    if (E > 0) {
+        k_tet_count<<<fph_blocks(E, 256), 256, 0, s>>>(E, adj_ptr, adj_idx, adj_rank, tri_off,
+                                                      tri_w, cnt);
+        FPH_LAUNCH_CHECK();
+    }
+    return scan_counts(cnt, tet_off, E, (long long*)total, ws, tmp, s);
+}
+
+extern "C" int fph_tet_emit(int64_t E, const int32_t* edge_i, const int32_t* edge_j,
+                            const int32_t* adj_ptr, const int32_t* adj_idx,
+                            const int32_t* adj_rank, const int32_t* tri_off,
+                            const int32_t* tri_w, const int32_t* tri_ru, const int32_t* tri_rv,
+                            const uint8_t* cleared, const int32_t* tet_off, uint32_t* tet_key,
+                            int32_t* tet_e, int32_t* tet_cmf, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (E < 0) return FPH_ERR_ARG;
+    if (E == 0) return FPH_OK;
+    k_tet_emit<<<fph_blocks(E, 256), 256, 0, s>>>(E, edge_i, edge_j, adj_ptr, adj_idx, adj_rank,
+                                                 tri_off, tri_w, tri_ru, tri_rv, cleared, tet_off,
+                                                 tet_key, tet_e, tet_cmf);
+    FPH_LAUNCH_CHECK();
+    return FPH_OK;
+}
+
+extern "C" int fph_cofacets_h2_count(int64_t T, const int32_t* edge_i, const int32_t* edge_j,
+                                     const int32_t* adj_ptr, const int32_t* adj_idx,
+                                     const int32_t* tri_w, const int32_t* tri_e,
+                                     const uint8_t* cleared, int32_t* cof_off, int64_t* total,
+                                     void* ws, size_t* ws_bytes, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (T < 0 || !ws_bytes) return FPH_ERR_ARG;
+    const size_t tmp = scan_bytes(T);
+    const size_t need = tmp + align_up(4 * (size_t)std::max<int64_t>(T, 1));
+    if (!ws) { *ws_bytes = need; return FPH_OK; }
+    if (*ws_bytes < need) return FPH_ERR_ARG;
+    int* cnt = (int*)((char*)ws + tmp);
+    if (T > 0) {
+        k_cof2_count<<<fph_blocks(T, 256), 256, 0, s>>>(T, edge_i, edge_j, adj_ptr, adj_idx,
+                                                       tri_w, tri_e, cleared, cnt);
+        FPH_LAUNCH_CHECK();
+    }
+    return scan_counts(cnt, cof_off, T, (long long*)total, ws, tmp, s);
+}
+
+extern "C" int fph_cofacets_h2(int64_t T, int64_t E, int64_t Q, const int32_t* edge_i,
+                               const int32_t* edge_j, const int32_t* adj_ptr,
+                               const int32_t* adj_idx, const int32_t* adj_rank,
+                               const int32_t* tri_w, const int32_t* tri_e,
+                               const uint8_t* cleared, const int32_t* tet_off,
+                               const uint32_t* tet_key, const int32_t* cof_off, int64_t C,
+                               uint32_t* cof_data, void* ws, size_t* ws_bytes, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (T < 0 || C < 0 || !ws_bytes) return FPH_ERR_ARG;
+    if (E + T + Q >= 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
+    const size_t tmp = segsort_bytes(C, T);
+    const size_t need = tmp + align_up(4 * (size_t)std::max<int64_t>(C, 1));
+    if (!ws) { *ws_bytes = need; return FPH_OK; }
+    if (*ws_bytes < need) return FPH_ERR_ARG;
+    if (C == 0) return FPH_OK;
+    unsigned* raw = (unsigned*)((char*)ws + tmp);
+    k_cof2_emit<<<fph_blocks(T, 256), 256, 0, s>>>(T, E, edge_i, edge_j, adj_ptr, adj_idx,
+                                                  adj_rank, tri_w, tri_e, cleared, tet_off,
+                                                  tet_key, cof_off, raw);
+    FPH_LAUNCH_CHECK();
+    size_t t = tmp;
+    const int end_bit = std::max(1, bit_width((unsigned long long)(E + T + Q)));
+    FPH_CHECK_HIP(rocprim::segmented_radix_sort_keys(ws, t, raw, (unsigned*)cof_data, (unsigned)C,
+                                                     (unsigned)T, cof_off, cof_off + 1, 0, end_bit,
+                                                     s));
+    return FPH_OK;
+}

@@ -1,0 +1,333 @@
+"""Host orchestration of the exact Vietoris-Rips pipeline on one GPU.
+
+Mirrors the reference's control flow (persistence.py:68-206,
+rips.py:98-218, reduce_h1_gpu.py:438-604) over the libflashph entry points.
+All arrays stay on the device; the host reads back only the sizes that
+shape the next allocation (edge count; triangle/cofacet counts; the row
+counts of the H1 and H2 diagrams; tetrahedron/cofacet counts).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from flash_ph import _lib
+from flash_ph._lib import call_ws, check, load, ptr, stream_of
+
+# Safety limits (kernels/triangle_kernel.py:295,377-381).
+MAX_TRIANGLES = 10_000_000
+# Screened-pair buffer size for the first attempt: at least this many edges,
+# or 64 per point; a larger count triggers one exact-size retry.
+_EDGE_CAP_MIN = 1 << 20
+_EDGE_CAP_PER_POINT = 64
+# Reduction pool (32-bit words): max(this, 8 x cofacet entries).
+_POOL_MIN_WORDS = 1 << 20
+_MAX_ROUNDS = 1 << 22
+
+
+class StageTimer:
+    """Optional HIP-event timing of named launches on the current stream.
+
+    Disabled by default; ``enable()`` makes the pipeline record an event pair
+    around each named entry-point call (used by bench.py for live per-kernel
+    timing)."""
+
+    def __init__(self):
+        self.enabled = False
+        self.records: dict = {}
+
+    def enable(self, on: bool = True):
+        self.enabled = on
+        self.records = {}
+
+    def start(self, name, device):
+        if not self.enabled:
+            return None
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record(torch.cuda.current_stream(device))
+        return (name, ev, device)
+
+    def stop(self, tok):
+        if tok is None:
+            return
+        name, ev0, device = tok
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev1.record(torch.cuda.current_stream(device))
+        self.records.setdefault(name, []).append((ev0, ev1))
+
+    def summary(self) -> dict:
+        """name -> (launches, mean ms); synchronises."""
+        torch.cuda.synchronize()
+        return {k: (len(v), sum(a.elapsed_time(b) for a, b in v) / len(v))
+                for k, v in self.records.items()}
+
+
+TIMER = StageTimer()
+
+
+def threshold_sq_f32(max_edge_length: float) -> float:
+    """t*t in Python double rounded to FP32, as the reference's Triton scalar
+    argument (rips.py:125, edge_kernel.py:125)."""
+    return float(np.float32(float(max_edge_length) * float(max_edge_length)))
+
+
+def _empty(dtype, device, *shape):
+    return torch.empty(*shape, dtype=dtype, device=device)
+
+
+@dataclass
+class Complex:
+    """Device state of one point cloud's filtration (all tensors on GPU)."""
+    n: int
+    device: torch.device
+    E: int = 0
+    edge_i: torch.Tensor | None = None
+    edge_j: torch.Tensor | None = None
+    edge_dsq: torch.Tensor | None = None
+    edge_filt: torch.Tensor | None = None
+    adj_ptr: torch.Tensor | None = None
+    adj_idx: torch.Tensor | None = None
+    adj_rank: torch.Tensor | None = None
+    adj_dsq: torch.Tensor | None = None
+    mst_mask: torch.Tensor | None = None
+    comp_label: torch.Tensor | None = None
+    mst_idx: torch.Tensor | None = None
+    h0: torch.Tensor | None = None
+    info: dict = field(default_factory=dict)
+    # H1
+    T: int = 0
+    C1: int = 0
+    tri_off: torch.Tensor | None = None
+    tri_w: torch.Tensor | None = None
+    tri_e: torch.Tensor | None = None
+    tri_ru: torch.Tensor | None = None
+    tri_rv: torch.Tensor | None = None
+    tri_cmf: torch.Tensor | None = None
+    cof_off: torch.Tensor | None = None
+    cof_data: torch.Tensor | None = None
+    kind1: torch.Tensor | None = None
+    pivot1: torch.Tensor | None = None
+    h1: torch.Tensor | None = None
+    # H2
+    Q: int = 0
+    C2: int = 0
+    cleared: torch.Tensor | None = None
+    tet_off: torch.Tensor | None = None
+    tet_key: torch.Tensor | None = None
+    tet_e: torch.Tensor | None = None
+    tet_cmf: torch.Tensor | None = None
+    cof2_off: torch.Tensor | None = None
+    cof2_data: torch.Tensor | None = None
+    kind2: torch.Tensor | None = None
+    pivot2: torch.Tensor | None = None
+    h2: torch.Tensor | None = None
+
+
+def build_edges(pts: torch.Tensor, max_edge_length: float) -> Complex:
+    """Stage (a): screened edges in filtration order plus the vertex CSR."""
+    lib = load()
+    dev = pts.device
+    s = stream_of(dev)
+    n, d = pts.shape
+    cx = Complex(n=n, device=dev)
+    thr = threshold_sq_f32(max_edge_length)
+    cap = max(_EDGE_CAP_MIN, _EDGE_CAP_PER_POINT * n)
+    cnt = _empty(torch.int64, dev, 1)
+    while True:
+        keys = _empty(torch.int64, dev, cap)
+        vals = _empty(torch.int64, dev, cap)
+        tok = TIMER.start("edge_screen", dev)
+        check(lib.fph_edge_screen(ptr(pts), n, d, thr, ptr(keys), ptr(vals), cap, ptr(cnt), s),
+              "fph_edge_screen")
+        TIMER.stop(tok)
+        E = int(cnt.item())
+        if E <= cap:
+            break
+        cap = E
+    # edge_kernel.py:203-205
+    assert E < 2**31, f"Edge count {E} exceeds int32 range; reduce max_edge_length"
+    cx.E = E
+    cx.edge_i = _empty(torch.int32, dev, E)
+    cx.edge_j = _empty(torch.int32, dev, E)
+    cx.edge_dsq = _empty(torch.float32, dev, E)
+    cx.edge_filt = _empty(torch.float32, dev, E)
+    if E > 0:
+        ktmp = _empty(torch.int64, dev, E)
+        vtmp = _empty(torch.int64, dev, E)
+        call_ws("fph_edge_sort",
+                (ptr(keys), ptr(vals), E, n, ptr(ktmp), ptr(vtmp), ptr(cx.edge_i),
+                 ptr(cx.edge_j), ptr(cx.edge_dsq), ptr(cx.edge_filt)), (s,), dev)
+        del ktmp, vtmp
+    del keys, vals
+    cx.adj_ptr = _empty(torch.int32, dev, n + 1)
+    cx.adj_idx = _empty(torch.int32, dev, 2 * E)
+    cx.adj_rank = _empty(torch.int32, dev, 2 * E)
+    cx.adj_dsq = _empty(torch.float32, dev, 2 * E)
+    call_ws("fph_vertex_csr",
+            (ptr(cx.edge_i), ptr(cx.edge_j), ptr(cx.edge_dsq), E, n, ptr(cx.adj_ptr),
+             ptr(cx.adj_idx), ptr(cx.adj_rank), ptr(cx.adj_dsq)), (s,), dev)
+    return cx
+
+
+def build_mst(cx: Complex, want_labels: bool = False) -> None:
+    """Stage (b): Boruvka forest, component labels and the H0 diagram."""
+    lib = load()
+    dev, n, E = cx.device, cx.n, cx.E
+    s = stream_of(dev)
+    cx.mst_mask = _empty(torch.uint8, dev, max(E, 1))
+    cx.comp_label = _empty(torch.int64, dev, n) if want_labels else None
+    conv = _empty(torch.int32, dev, 1)
+    scratch = _lib.WORKSPACE.get(lib.fph_boruvka_scratch_bytes(n), dev)
+    check(lib.fph_boruvka_mst(ptr(cx.edge_i), ptr(cx.edge_j), E, n, ptr(cx.mst_mask),
+                              ptr(cx.comp_label), ptr(conv), ptr(scratch), scratch.numel(), s),
+          "fph_boruvka_mst")
+    cx.h0 = _empty(torch.float32, dev, n, 2)
+    cx.mst_idx = _empty(torch.int64, dev, max(n, 1))
+    nm = _empty(torch.int64, dev, 1)
+    call_ws("fph_h0_diagram",
+            (ptr(cx.mst_mask), ptr(cx.edge_dsq), E, n, ptr(cx.h0), ptr(cx.mst_idx), ptr(nm)),
+            (s,), dev)
+    cx.info["converged"] = conv
+    cx.info["n_mst"] = nm
+
+
+def build_h1(cx: Complex, max_triangles: int = MAX_TRIANGLES) -> None:
+    """Stages (d), (c), (e) for H1 and the H1 diagram."""
+    lib = load()
+    dev, n, E = cx.device, cx.n, cx.E
+    s = stream_of(dev)
+    cx.tri_off = _empty(torch.int32, dev, E + 1)
+    cx.cof_off = _empty(torch.int32, dev, E + 1)
+    totals = _empty(torch.int64, dev, 2)
+    call_ws("fph_triangle_count",
+            (ptr(cx.edge_i), ptr(cx.edge_j), E, ptr(cx.adj_ptr), ptr(cx.adj_idx),
+             ptr(cx.adj_rank), ptr(cx.mst_mask), ptr(cx.tri_off), ptr(cx.cof_off), ptr(totals)),
+            (s,), dev)
+    T, C = (int(x) for x in totals.tolist())
+    if not int(cx.info["converged"].item()):
+        raise _lib.FlashPHError("Boruvka did not converge")
+    if T > max_triangles:
+        raise ValueError(
+            f"Triangle count {T} exceeds max_triangles={max_triangles}. "
+            f"Reduce max_edge_length or increase max_triangles.")
+    if E + T >= 2**31 - 1:
+        raise ValueError("edge + triangle count exceeds the int32 rank range")
+    cx.T, cx.C1 = T, C
+    cx.tri_w = _empty(torch.int32, dev, max(T, 1))
+    cx.tri_e = _empty(torch.int32, dev, max(T, 1))
+    cx.tri_ru = _empty(torch.int32, dev, max(T, 1))
+    cx.tri_rv = _empty(torch.int32, dev, max(T, 1))
+    cx.tri_cmf = _empty(torch.int32, dev, max(T, 1))
+    check(lib.fph_triangle_emit(ptr(cx.edge_i), ptr(cx.edge_j), E, ptr(cx.adj_ptr),
+                                ptr(cx.adj_idx), ptr(cx.adj_rank), ptr(cx.mst_mask),
+                                ptr(cx.tri_off), ptr(cx.tri_w), ptr(cx.tri_e), ptr(cx.tri_ru),
+                                ptr(cx.tri_rv), ptr(cx.tri_cmf), s), "fph_triangle_emit")
+    cx.cof_data = _empty(torch.int32, dev, max(C, 1))
+    call_ws("fph_cofacets_h1",
+            (ptr(cx.edge_i), ptr(cx.edge_j), E, T, ptr(cx.adj_ptr), ptr(cx.adj_idx),
+             ptr(cx.adj_rank), ptr(cx.mst_mask), ptr(cx.tri_off), ptr(cx.tri_w),
+             ptr(cx.cof_off), C, ptr(cx.cof_data)), (s,), dev)
+    cx.kind1, cx.pivot1, cx.h1, stats = _reduce_and_diagram(
+        K=E, col_base=0, off=cx.cof_off, data=cx.cof_data, skip=cx.mst_mask, row_cmf=cx.tri_cmf,
+        row_base=E, nrows=T, C=C, col_edge=None, row_edge=cx.tri_e, edge_filt=cx.edge_filt,
+        E=E, ess_ascending=(T == 0), device=dev)
+    cx.info["h1_stats"] = stats
+
+
+def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, C, col_edge,
+                        row_edge, edge_filt, E, ess_ascending, device):
+    """Apparent pairs, residual reduction and the diagram of one dimension."""
+    lib = load()
+    s = stream_of(device)
+    kind = _empty(torch.uint8, device, max(K, 1))
+    own = _empty(torch.int32, device, max(nrows, 1))
+    pivot = _empty(torch.int32, device, max(K, 1))
+    res_list = _empty(torch.int32, device, max(K, 1))
+    # info32: [0..2] reduction status, [3] residual count, [4] diagram rows
+    info32 = torch.zeros(8, dtype=torch.int32, device=device)
+    status_p = info32.data_ptr()
+    rescnt_p = info32.data_ptr() + 12
+    count_p = info32.data_ptr() + 16
+    scratch_bytes = lib.fph_reduce_scratch_bytes(K)
+    pool_words = max(_POOL_MIN_WORDS, 8 * C)
+    out = _empty(torch.float32, device, max(K, 1), 2)
+    while True:
+        check(lib.fph_apparent(K, col_base, ptr(off), ptr(data), ptr(skip), ptr(row_cmf),
+                               row_base, nrows, ptr(kind), ptr(own), ptr(pivot), ptr(res_list),
+                               rescnt_p, s), "fph_apparent")
+        pool = _empty(torch.int32, device, pool_words)
+        scratch = _empty(torch.uint8, device, scratch_bytes)
+        tok = TIMER.start("reduce", device)
+        check(lib.fph_reduce(K, ptr(off), ptr(data), row_base, ptr(kind), ptr(own), ptr(pivot),
+                             ptr(res_list), rescnt_p, ptr(pool), pool_words, ptr(scratch),
+                             scratch_bytes, status_p, _MAX_ROUNDS, s), "fph_reduce")
+        TIMER.stop(tok)
+        call_ws("fph_diagram",
+                (K, ptr(kind), ptr(off), ptr(data), ptr(pivot), ptr(col_edge), row_base,
+                 ptr(row_edge), ptr(edge_filt), E, int(bool(ess_ascending)), ptr(out), count_p),
+                (s,), device)
+        st = info32.tolist()
+        if st[0] == 0:
+            break
+        if st[0] == 1:
+            pool_words *= 4
+            continue
+        raise _lib.FlashPHError(f"fph_reduce failed with status {st[0]}")
+    stats = {"rounds": st[1], "xors": st[2], "residual": st[3]}
+    return kind, pivot, out[: st[4]], stats
+
+
+def build_h2(cx: Complex) -> None:
+    """H1 -> H2 clearing, tetrahedra, H2 columns, reduction and diagram."""
+    lib = load()
+    dev, E, T = cx.device, cx.E, cx.T
+    s = stream_of(dev)
+    cx.cleared = _empty(torch.uint8, dev, max(T, 1))
+    check(lib.fph_clear_pivots(E, ptr(cx.kind1), ptr(cx.pivot1), E, T, ptr(cx.cleared), s),
+          "fph_clear_pivots")
+    cx.tet_off = _empty(torch.int32, dev, E + 1)
+    cx.cof2_off = _empty(torch.int32, dev, T + 1)
+    totals = _empty(torch.int64, dev, 2)
+    call_ws("fph_tet_count",
+            (E, ptr(cx.adj_ptr), ptr(cx.adj_idx), ptr(cx.adj_rank), ptr(cx.tri_off),
+             ptr(cx.tri_w), ptr(cx.tet_off), totals.data_ptr()), (s,), dev)
+    call_ws("fph_cofacets_h2_count",
+            (T, ptr(cx.edge_i), ptr(cx.edge_j), ptr(cx.adj_ptr), ptr(cx.adj_idx), ptr(cx.tri_w),
+             ptr(cx.tri_e), ptr(cx.cleared), ptr(cx.cof2_off), totals.data_ptr() + 8), (s,), dev)
+    Q, C2 = (int(x) for x in totals.tolist())
+    if E + T + Q >= 2**31 - 1:
+        raise ValueError("simplex count exceeds the int32 rank range")
+    cx.Q, cx.C2 = Q, C2
+    cx.tet_key = _empty(torch.int32, dev, max(Q, 1))
+    cx.tet_e = _empty(torch.int32, dev, max(Q, 1))
+    cx.tet_cmf = _empty(torch.int32, dev, max(Q, 1))
+    check(lib.fph_tet_emit(E, ptr(cx.edge_i), ptr(cx.edge_j), ptr(cx.adj_ptr), ptr(cx.adj_idx),
+                           ptr(cx.adj_rank), ptr(cx.tri_off), ptr(cx.tri_w), ptr(cx.tri_ru),
+                           ptr(cx.tri_rv), ptr(cx.cleared), ptr(cx.tet_off), ptr(cx.tet_key),
+                           ptr(cx.tet_e), ptr(cx.tet_cmf), s), "fph_tet_emit")
+    cx.cof2_data = _empty(torch.int32, dev, max(C2, 1))
+    call_ws("fph_cofacets_h2",
+            (T, E, Q, ptr(cx.edge_i), ptr(cx.edge_j), ptr(cx.adj_ptr), ptr(cx.adj_idx),
+             ptr(cx.adj_rank), ptr(cx.tri_w), ptr(cx.tri_e), ptr(cx.cleared), ptr(cx.tet_off),
+             ptr(cx.tet_key), ptr(cx.cof2_off), C2, ptr(cx.cof2_data)), (s,), dev)
+    cx.kind2, cx.pivot2, cx.h2, stats = _reduce_and_diagram(
+        K=T, col_base=E, off=cx.cof2_off, data=cx.cof2_data, skip=cx.cleared,
+        row_cmf=cx.tet_cmf, row_base=E + T, nrows=Q, C=C2, col_edge=cx.tri_e,
+        row_edge=cx.tet_e, edge_filt=cx.edge_filt, E=E, ess_ascending=False, device=dev)
+    cx.info["h2_stats"] = stats
+
+
+def compute(pts: torch.Tensor, max_edge_length: float, max_dim: int) -> tuple[list, Complex]:
+    """Diagrams [H0, ..., H_max_dim] of a CUDA point cloud with n >= 2."""
+    cx = build_edges(pts, max_edge_length)
+    build_mst(cx)
+    out = [cx.h0]
+    if max_dim >= 1:
+        build_h1(cx)
+        out.append(cx.h1)
+    if max_dim >= 2:
+        build_h2(cx)
+        out.append(cx.h2)
+    return out, cx

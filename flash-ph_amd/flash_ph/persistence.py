@@ -1,0 +1,140 @@
+"""Public API for Vietoris-Rips persistent homology (H0 + H1 + H2).
+
+Mirrors the reference persistence.py.  Pipeline, all on one GPU:
+edge screening -> filtration sort and CSR -> Boruvka forest (H0) ->
+triangles, apparent pairs and residual reduction (H1) -> H1->H2 clearing,
+tetrahedra, apparent pairs and residual reduction (H2).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+from torch import Tensor
+
+from flash_ph import _lib, _pipeline
+
+_VERTEX_PACK_BITS = 16
+_VALID_BACKENDS = ("auto", "gpu", "giotto")
+
+
+def _rips_h1h2_giotto(filt, n, max_dim, max_edge_length, device):
+    """H1 (+H2) through giotto-ph, kept for ``backend='giotto'``
+    (reference persistence.py:27-61).  Raises ModuleNotFoundError when
+    giotto-ph is not installed."""
+    from gph import ripser_parallel
+    import scipy.sparse
+
+    row = filt.edge_i.cpu().numpy().astype(np.int64)
+    col = filt.edge_j.cpu().numpy().astype(np.int64)
+    dist = torch.sqrt(filt.edge_dist_sq).cpu().numpy().astype(np.float32)
+    diag = np.arange(n, dtype=np.int64)
+    row = np.concatenate([row, diag])
+    col = np.concatenate([col, diag])
+    dist = np.concatenate([dist, np.zeros(n, dtype=np.float32)])
+    dm = scipy.sparse.coo_matrix((dist, (row, col)), shape=(n, n))
+    result = ripser_parallel(dm, maxdim=max_dim, thresh=max_edge_length,
+                             metric="precomputed", collapse_edges=True, n_threads=-1)
+    diagrams = []
+    for d in range(1, max_dim + 1):
+        dgm_np = result["dgms"][d]
+        if len(dgm_np) > 0:
+            t = torch.from_numpy(dgm_np).float().to(device)
+            t = t[t[:, 0].argsort(stable=True)]
+        else:
+            t = torch.empty(0, 2, dtype=torch.float32, device=device)
+        diagrams.append(t)
+    return diagrams
+
+
+def rips_persistence(
+    points,
+    max_edge_length: float,
+    max_dim: int = 1,
+    backend: str = 'auto',
+) -> list[Tensor]:
+    """Exact Vietoris-Rips persistent homology (H0 + H1 + optional H2).
+
+    Parameters
+    ----------
+    points : (n, d) array-like
+        Point cloud coordinates (any d >= 1).  NumPy arrays are converted to
+        a float32 tensor on the GPU.  CPU tensors are computed on the GPU and
+        the diagrams returned on the CPU.
+    max_edge_length : float
+        Maximum edge length (strict < inequality).
+    max_dim : int
+        0 (H0 only), 1 (H0 + H1), or 2 (H0 + H1 + H2).  Default 1.
+        H2 uses 16-bit vertex packing, requiring n < 65536.
+    backend : str
+        H1 reduction backend (validated when ``max_dim=1``).  ``'auto'`` and
+        ``'gpu'`` run the GPU reduction; ``'giotto'`` hands H1 to giotto-ph.
+
+    Returns
+    -------
+    list[Tensor]
+        [H0, H1, ...], each (K, 2) float32 on the input's device: H0 has one
+        (0, inf) row per connected component; H1/H2 include essential
+        (birth, inf) rows; rows sorted stably by birth.
+
+    Reference: persistence.py:68-206.
+    """
+    if isinstance(points, np.ndarray):
+        points = torch.from_numpy(points).float()
+        if torch.cuda.is_available():
+            points = points.cuda()
+
+    if max_edge_length < 0:
+        raise ValueError(
+            f"max_edge_length must be non-negative, got {max_edge_length}"
+        )
+    if points.ndim != 2:
+        raise ValueError(
+            f"points must have shape (n, d), got {tuple(points.shape)}"
+        )
+    if max_dim not in (0, 1, 2):
+        raise ValueError(f"max_dim must be 0, 1, or 2, got {max_dim}")
+    if max_dim == 2 and points.shape[0] >= (1 << _VERTEX_PACK_BITS):
+        raise ValueError(
+            f"H2 requires n < {1 << _VERTEX_PACK_BITS} (16-bit vertex packing), "
+            f"got n={points.shape[0]}. Use max_dim=1 or reduce n."
+        )
+
+    device = points.device
+    n = points.shape[0]
+
+    if n == 0:
+        return [
+            torch.empty(0, 2, dtype=torch.float32, device=device)
+            for _ in range(max_dim + 1)
+        ]
+    if n == 1:
+        h0 = torch.tensor([[0.0, float('inf')]], dtype=torch.float32, device=device)
+        return [h0] + [
+            torch.empty(0, 2, dtype=torch.float32, device=device)
+            for _ in range(max_dim)
+        ]
+
+    if max_dim == 1 and backend not in _VALID_BACKENDS:
+        # reduce_h1_gpu.py:462-465
+        raise ValueError(
+            f"backend must be one of {_VALID_BACKENDS}, got {backend!r}"
+        )
+
+    _lib.require_gpu()
+    pts = points.float().contiguous()
+    if pts.device.type != "cuda":
+        pts = pts.cuda()
+
+    if max_dim == 1 and backend == "giotto":
+        from flash_ph.rips import RipsFiltration
+        cx = _pipeline.build_edges(pts, max_edge_length)
+        _pipeline.build_mst(cx)
+        filt = RipsFiltration(cx.edge_i, cx.edge_j, cx.edge_dsq, cx.adj_ptr, cx.adj_idx,
+                              cx.adj_rank, cx.adj_dsq, n)
+        h1 = _rips_h1h2_giotto(filt, n, 1, max_edge_length, pts.device)[0]
+        out = [cx.h0, h1]
+    else:
+        out, _ = _pipeline.compute(pts, max_edge_length, max_dim)
+    if device.type != "cuda" or out[0].device != device:
+        out = [d.to(device) for d in out]
+    return out

@@ -1,0 +1,261 @@
+/*
+ * flashph.h -- C ABI of the MI355X (gfx950) exact Vietoris-Rips library
+ * (libflashph.so, built from flash-ph_amd/csrc).
+ *
+ * Conventions
+ *   - Every pointer argument is a device pointer unless stated otherwise; the
+ *     caller owns all buffers.
+ *   - `stream` is a hipStream_t passed as void*; every entry point only
+ *     enqueues work on it (stream-ordered, asynchronous, reentrant).
+ *   - Entry points taking (ws, ws_bytes) need device workspace: call with
+ *     ws == NULL to receive the size in *ws_bytes (nothing is launched), then
+ *     call again with a buffer of at least that size.
+ *   - Return value: FPH_OK or a negative FPH_ERR_* code.  Nothing throws
+ *     across the ABI.
+ *   - Data-dependent sizes (edge, triangle, tetrahedron counts) are written to
+ *     device memory; the caller reads them and sizes the next stage.
+ *
+ * Reference citations are relative to the reference's src/flash_ph/.
+ */
+#ifndef FLASHPH_H
+#define FLASHPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FPH_OK 0
+#define FPH_ERR_ARG (-1)      /* invalid argument or workspace too small */
+#define FPH_ERR_CAPACITY (-2) /* output capacity exceeded (counts still written) */
+#define FPH_ERR_LIMIT (-3)    /* size beyond the int32 index range */
+#define FPH_ERR_HIP (-4)      /* HIP runtime error */
+
+/* Library version string (host pointer). */
+const char *fph_version(void);
+
+/* ---- stage (a): edges ------------------------------------------------- */
+
+/*
+ * Screen all pairs i < j of `pts` (n x d, row-major FP32) and append those
+ * with dsq < thr_sq, where dsq = fma-chain sum_k (x_i[k]-x_j[k])^2 in FP32,
+ * as unordered (key, val) pairs:
+ *   key = float_bits(dsq) << 32 | (j(j-1)/2 + i) mod 2^32,  val = j << 32 | i.
+ * *count_dev receives the total number of qualifying pairs even when it
+ * exceeds `cap` (only the first `cap` are written).
+ * Replaces kernels/edge_kernel.py:23-149 (count + emit kernels) and
+ * direct_edge_enumerate (kernels/edge_kernel.py:152-224).
+ */
+int fph_edge_screen(const float *pts, int64_t n, int d, float thr_sq, uint64_t *keys,
+                    uint64_t *vals, int64_t cap, uint64_t *count_dev, void *stream);
+
+/*
+ * Sort the E screened pairs by key (ties, possible only for n > 92682, in
+ * the reference's emission order) and unpack them into edge_i, edge_j,
+ * edge_dsq and edge_filt = sqrt(edge_dsq) (correctly rounded; may be NULL).
+ * keys_tmp / vals_tmp: E-element scratch.
+ * Replaces _pack_sort_key + argsort(stable=True) + gathers, rips.py:30-50,
+ * 188-202.
+ */
+int fph_edge_sort(const uint64_t *keys_in, const uint64_t *vals_in, int64_t E, int64_t n,
+                  uint64_t *keys_tmp, uint64_t *vals_tmp, int32_t *edge_i, int32_t *edge_j,
+                  float *edge_dsq, float *edge_filt, void *ws, size_t *ws_bytes, void *stream);
+
+/*
+ * Per-vertex CSR of the sorted edges: adj_ptr (n+1), and per half-edge the
+ * neighbour (ascending within a row), its edge rank and dsq (2E each).
+ * Replaces _build_vertex_csr, rips.py:53-95.
+ */
+int fph_vertex_csr(const int32_t *edge_i, const int32_t *edge_j, const float *edge_dsq,
+                   int64_t E, int64_t n, int32_t *adj_ptr, int32_t *adj_idx,
+                   int32_t *adj_rank, float *adj_dsq, void *ws, size_t *ws_bytes,
+                   void *stream);
+
+/* ---- stage (b): minimum spanning forest and H0 ------------------------ */
+
+/* Number of Boruvka rounds issued for n vertices (ceil(log2 n) + 2). */
+int fph_boruvka_rounds(int64_t n);
+/* Bytes of device scratch fph_boruvka_mst needs for n vertices. */
+size_t fph_boruvka_scratch_bytes(int64_t n);
+
+/*
+ * Boruvka minimum spanning forest over edge ranks 0..E-1 (the sorted edge
+ * order).  mst_mask[e] = 1 for forest edges.  comp_label (n, int64, may be
+ * NULL) = smallest vertex id of each vertex's component.  *converged = 1
+ * when the issued rounds sufficed (always, for the round count above).
+ * Replaces gpu_boruvka_mst, boruvka.py:17-113.
+ */
+int fph_boruvka_mst(const int32_t *edge_i, const int32_t *edge_j, int64_t E, int64_t n,
+                    uint8_t *mst_mask, int64_t *comp_label, int32_t *converged,
+                    void *scratch, size_t scratch_bytes, void *stream);
+
+/*
+ * H0 diagram: exactly n rows (n x 2 FP32): (0, sqrt(dsq)) for forest edges in
+ * rank order, then (0, inf) per component.  mst_idx (int64, capacity n, may
+ * be NULL) = forest edge indices ascending; *n_mst (device int64, may be NULL).
+ * Replaces persistence.py:169-192.
+ */
+int fph_h0_diagram(const uint8_t *mst_mask, const float *edge_dsq, int64_t E, int64_t n,
+                   float *h0, int64_t *mst_idx, int64_t *n_mst, void *ws, size_t *ws_bytes,
+                   void *stream);
+
+/* ---- stage (d): triangles and H1 cofacet columns ---------------------- */
+
+/*
+ * Per edge e = (u, v): tri_off = exclusive scan of the number of triangles
+ * whose largest edge is e (E+1 entries); cof_off = exclusive scan of the
+ * number of triangles containing e, 0 for forest edges (E+1 entries).
+ * totals[0] = T, totals[1] = total cofacet entries (device int64[2]).
+ * Replaces _triangle_count_kernel + prefix, kernels/triangle_kernel.py:28-79,
+ * 363-385.
+ */
+int fph_triangle_count(const int32_t *edge_i, const int32_t *edge_j, int64_t E,
+                       const int32_t *adj_ptr, const int32_t *adj_idx, const int32_t *adj_rank,
+                       const uint8_t *mst_mask, int32_t *tri_off, int32_t *cof_off,
+                       int64_t *totals, void *ws, size_t *ws_bytes, void *stream);
+
+/*
+ * Triangles in filtration order (global rank E + t): tri_w (third vertex,
+ * ascending per owning edge), tri_e (owning = largest edge), tri_ru / tri_rv
+ * (ranks of (u,w), (v,w)), tri_cmf (largest non-forest face rank or -1).
+ * Replaces _triangle_emit_kernel (kernels/triangle_kernel.py:81-144), the
+ * face lookup (reduce_h1_gpu.py:65-100) and the triangle ranking
+ * (reduce_h1_gpu.py:509-521).
+ */
+int fph_triangle_emit(const int32_t *edge_i, const int32_t *edge_j, int64_t E,
+                      const int32_t *adj_ptr, const int32_t *adj_idx, const int32_t *adj_rank,
+                      const uint8_t *mst_mask, const int32_t *tri_off, int32_t *tri_w,
+                      int32_t *tri_e, int32_t *tri_ru, int32_t *tri_rv, int32_t *tri_cmf,
+                      void *stream);
+
+/*
+ * Cofacet columns of the non-forest edges: ascending triangle global ranks
+ * (C = totals[1] entries, rows by cof_off).
+ * Replaces _build_cofacet_csr_h1, reduce_h1_gpu.py:107-152.
+ */
+int fph_cofacets_h1(const int32_t *edge_i, const int32_t *edge_j, int64_t E, int64_t T,
+                    const int32_t *adj_ptr, const int32_t *adj_idx, const int32_t *adj_rank,
+                    const uint8_t *mst_mask, const int32_t *tri_off, const int32_t *tri_w,
+                    const int32_t *cof_off, int64_t C, uint32_t *cof_data, void *ws,
+                    size_t *ws_bytes, void *stream);
+
+/* ---- stage (c): apparent pairs ---------------------------------------- */
+
+/*
+ * Classify K columns (global rank col_base + c) with cofacet rows
+ * (col_off, col_data; global row ranks, row 0 = row_base, nrows rows):
+ * kind = 0 skipped (skip_mask), 1 no cofacet (essential), 2 apparent
+ * (row_cmf[youngest - row_base] == column rank), 3 residual.  own (nrows) is
+ * reset to -1 then own[youngest - row_base] = c for apparent columns;
+ * pivot[c] = youngest for apparent columns, -1 otherwise; residual columns
+ * are listed (unordered) in res_list, their number in *res_count (device).
+ * Replaces _gpu_apparent_pair_predetect, reduce_h1_gpu.py:159-223, for H1
+ * (col_base 0, row_base E) and H2 (col_base E, row_base E + T).
+ */
+int fph_apparent(int64_t K, int64_t col_base, const int32_t *col_off, const uint32_t *col_data,
+                 const uint8_t *skip_mask, const int32_t *row_cmf, int64_t row_base,
+                 int64_t nrows, uint8_t *kind, int32_t *own, int32_t *pivot, int32_t *res_list,
+                 int32_t *res_count, void *stream);
+
+/* ---- stage (e): residual reduction ------------------------------------ */
+
+/* Bytes of device scratch fph_reduce needs for K columns. */
+size_t fph_reduce_scratch_bytes(int64_t K);
+
+/*
+ * Z/2 reduction of the residual columns listed by fph_apparent.  On return
+ * pivot[c] holds each residual column's final pivot row (global rank) or -1
+ * (reduced to zero: essential).  Reduced columns are written to `pool`
+ * (pool_cap 32-bit words).  status (device int32[3]): [0] = 0 ok,
+ * 1 pool too small (retry with a larger pool), 2 round cap reached,
+ * 3 inconsistent input; [1] = rounds; [2] = column additions performed.
+ * Replaces _general_cohomology_reduce_v2 (cohomology_general.py:202-371)
+ * with its pool prefill (_numba_utils.py:68-104, reduce_h1_gpu.py:242-303).
+ */
+int fph_reduce(int64_t K, const int32_t *col_off, const uint32_t *col_data, int64_t row_base,
+               const uint8_t *kind, int32_t *own, int32_t *pivot, const int32_t *res_list,
+               const int32_t *res_count, uint32_t *pool, int64_t pool_cap, void *scratch,
+               size_t scratch_bytes, int32_t *status, int max_rounds, void *stream);
+
+/* ---- diagrams --------------------------------------------------------- */
+
+/*
+ * Rows (birth, death) of a dimension in the reference's order: apparent
+ * pairs ascending, reduced pairs descending, essentials descending (or
+ * ascending when ess_ascending), then stably by birth value; only pairs with
+ * death > birth.  Birth of column c = edge_filt[col_edge ? col_edge[c] : c];
+ * death of row r = edge_filt[row_edge[r - row_base]].  out: K x 2 FP32
+ * capacity; *count (device int32) = rows written.
+ * Replaces reduce_h1_gpu.py:495-502, 574-604 (and persistence.py:202-205
+ * for H2).
+ */
+int fph_diagram(int64_t K, const uint8_t *kind, const int32_t *col_off,
+                const uint32_t *col_data, const int32_t *pivot, const int32_t *col_edge,
+                int64_t row_base, const int32_t *row_edge, const float *edge_filt, int64_t E,
+                int ess_ascending, float *out, int32_t *count, void *ws, size_t *ws_bytes,
+                void *stream);
+
+/* ---- H2: clearing cascade, tetrahedra, cofacet columns ---------------- */
+
+/*
+ * cleared[t] = 1 for every triangle that is the pivot of an H1 column
+ * (apparent or reduced, including zero persistence).  nrows = T.
+ * Replaces the all_pivots cascade, cohomology_general.py:171-173.
+ */
+int fph_clear_pivots(int64_t K, const uint8_t *kind, const int32_t *pivot, int64_t row_base,
+                     int64_t nrows, uint8_t *cleared, void *stream);
+
+/*
+ * tet_off = exclusive scan of the number of tetrahedra whose largest edge is
+ * e (E+1 entries); *total = Q (device int64).
+ * Replaces _tet_count_kernel + prefix, kernels/triangle_kernel.py:152-204,
+ * 468-489.
+ */
+int fph_tet_count(int64_t E, const int32_t *adj_ptr, const int32_t *adj_idx,
+                  const int32_t *adj_rank, const int32_t *tri_off, const int32_t *tri_w,
+                  int32_t *tet_off, int64_t *total, void *ws, size_t *ws_bytes, void *stream);
+
+/*
+ * Tetrahedra in (max_rank, colex4) order (global rank E + T + q):
+ * tet_key = y << 16 | x for the two vertices off the largest edge (requires
+ * n < 65536), tet_e = largest edge, tet_cmf = largest uncleared triangle
+ * face (global rank) or -1.
+ * Replaces _tet_emit_kernel (kernels/triangle_kernel.py:206-270) and the
+ * colex4 order of kernels/apparent_pair_h2_kernel.py:52-58.
+ */
+int fph_tet_emit(int64_t E, const int32_t *edge_i, const int32_t *edge_j,
+                 const int32_t *adj_ptr, const int32_t *adj_idx, const int32_t *adj_rank,
+                 const int32_t *tri_off, const int32_t *tri_w, const int32_t *tri_ru,
+                 const int32_t *tri_rv, const uint8_t *cleared, const int32_t *tet_off,
+                 uint32_t *tet_key, int32_t *tet_e, int32_t *tet_cmf, void *stream);
+
+/*
+ * cof_off (T+1) = exclusive scan of the number of tetrahedra containing each
+ * uncleared triangle; *total = total entries (device int64).
+ * Replaces the 3-way merge count of kernels/apparent_pair_h2_kernel.py:
+ * 202-260 / _numba_utils.py:186-230.
+ */
+int fph_cofacets_h2_count(int64_t T, const int32_t *edge_i, const int32_t *edge_j,
+                          const int32_t *adj_ptr, const int32_t *adj_idx, const int32_t *tri_w,
+                          const int32_t *tri_e, const uint8_t *cleared, int32_t *cof_off,
+                          int64_t *total, void *ws, size_t *ws_bytes, void *stream);
+
+/*
+ * Cofacet columns of the uncleared triangles: ascending tetrahedron global
+ * ranks.  Restates the H2 column layout of cohomology_general.py:32-199
+ * (columns = triangles, cofacets = tetrahedra).
+ */
+int fph_cofacets_h2(int64_t T, int64_t E, int64_t Q, const int32_t *edge_i,
+                    const int32_t *edge_j, const int32_t *adj_ptr, const int32_t *adj_idx,
+                    const int32_t *adj_rank, const int32_t *tri_w, const int32_t *tri_e,
+                    const uint8_t *cleared, const int32_t *tet_off, const uint32_t *tet_key,
+                    const int32_t *cof_off, int64_t C, uint32_t *cof_data, void *ws,
+                    size_t *ws_bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLASHPH_H */

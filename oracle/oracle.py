@@ -1,0 +1,128 @@
+"""ctypes front end of the CPU oracle (fph_oracle.c).
+
+Test infrastructure only: tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use it as the checker.  The flash_ph package never imports it.
+
+``run(points, max_edge_length, max_dim)`` returns every intermediate array of
+the restated reference pipeline (edges, CSR, MST, triangles, cofacets,
+apparent pairs, pairs, diagrams) as numpy arrays keyed by name, plus the
+diagrams in the reference's output format (list of (K, 2) float32 arrays).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+# Array ids: keep in the order of the enum in fph_oracle.c.
+NAMES = [
+    "edge_i", "edge_j", "edge_dsq",
+    "adj_ptr", "adj_idx", "adj_rank", "adj_dsq",
+    "mst_idx", "comp", "mst_kruskal", "h0",
+    "tri_v0", "tri_v1", "tri_v2", "tri_max_rank", "tri_global_rank",
+    "cof_off", "cof_data", "cmf1", "apparent1",
+    "h1_pb", "h1_pd", "h1_eb", "h1_allpiv", "h1",
+    "tet_v0", "tet_v1", "tet_v2", "tet_v3", "tet_max_rank", "tet_global_rank",
+    "cof2_off", "cof2_data", "apparent2",
+    "h2_pb", "h2_pd", "h2_eb", "h2",
+    "stats",
+]
+STATS = ["E", "T", "Q", "boruvka_rounds", "residual1", "residual2",
+         "pool_drops", "truncations", "xor1", "xor2", "ncomp"]
+
+_DTYPES = {
+    "edge_dsq": np.float32, "adj_dsq": np.float32,
+    "mst_idx": np.int64, "comp": np.int64, "mst_kruskal": np.uint8,
+    "apparent1": np.uint8, "apparent2": np.uint8,
+    "h0": np.float32, "h1": np.float32, "h2": np.float32,
+    "stats": np.int64,
+}
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        lib = ctypes.CDLL(_LIB_PATH)
+        lib.orc_run.restype = ctypes.c_void_p
+        lib.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                                ctypes.c_float, ctypes.c_int]
+        lib.orc_len.restype = ctypes.c_int64
+        lib.orc_len.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib.orc_get.restype = ctypes.c_int
+        lib.orc_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        lib.orc_free.restype = None
+        lib.orc_free.argtypes = [ctypes.c_void_p]
+        lib.orc_dist_sq.restype = ctypes.c_float
+        lib.orc_dist_sq.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64,
+                                    ctypes.c_int64]
+        lib.orc_fmaf.restype = ctypes.c_float
+        lib.orc_fmaf.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        _lib = lib
+    return _lib
+
+
+def threshold_sq_f32(max_edge_length: float) -> np.float32:
+    """t*t in Python double, rounded to FP32 (rips.py:125 as a Triton fp32 arg)."""
+    return np.float32(float(max_edge_length) * float(max_edge_length))
+
+
+def run(points, max_edge_length: float, max_dim: int = 1) -> dict:
+    lib = _load()
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float32))
+    if pts.ndim != 2:
+        raise ValueError("points must be (n, d)")
+    n, d = pts.shape
+    out: dict = {}
+    if n <= 1:
+        h0 = (np.array([[0.0, np.inf]], dtype=np.float32) if n == 1
+              else np.empty((0, 2), np.float32))
+        out["diagrams"] = [h0] + [np.empty((0, 2), np.float32) for _ in range(max_dim)]
+        out["stats"] = {k: 0 for k in STATS}
+        return out
+    h = lib.orc_run(pts.ctypes.data, n, d, float(threshold_sq_f32(max_edge_length)),
+                    int(max_dim))
+    try:
+        for idx, name in enumerate(NAMES):
+            ln = lib.orc_len(h, idx)
+            dt = _DTYPES.get(name, np.int32)
+            shape = (ln, 2) if name in ("h0", "h1", "h2") else (ln,)
+            arr = np.empty(shape, dtype=dt)
+            if arr.size:
+                lib.orc_get(h, idx, arr.ctypes.data)
+            out[name] = arr
+    finally:
+        lib.orc_free(h)
+    out["stats"] = dict(zip(STATS, (int(x) for x in out["stats"])))
+    dg = [out["h0"]]
+    if max_dim >= 1:
+        dg.append(out["h1"])
+    if max_dim >= 2:
+        dg.append(out["h2"])
+    out["diagrams"] = dg
+    return out
+
+
+def dist_sq(points, i: int, j: int) -> np.float32:
+    lib = _load()
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float32))
+    return np.float32(lib.orc_dist_sq(pts.ctypes.data, pts.shape[1], i, j))
+
+
+def fmaf(a, b, c) -> np.float32:
+    return np.float32(_load().orc_fmaf(float(a), float(b), float(c)))

@@ -1,0 +1,84 @@
+"""Seeded point clouds shared by the parity tests.
+
+Inputs are generated with numpy default_rng in FP32 (portable bit for bit);
+the families follow the reference's tests (testing/test_reduce_h1.py:59-268,
+testing/test_persistence.py:288-306, 476-492, 541-570).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def randn(n, d, seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal((n, d)) * scale).astype(np.float32)
+
+
+def clifford(n, seed, noise=0.0):
+    rng = np.random.default_rng(seed)
+    th = rng.uniform(0, 2 * np.pi, n).astype(np.float32)
+    ph = rng.uniform(0, 2 * np.pi, n).astype(np.float32)
+    pts = np.column_stack([np.cos(th), np.sin(th), np.cos(ph), np.sin(ph)]).astype(
+        np.float32) / np.sqrt(2)
+    if noise:
+        pts = pts + rng.normal(0, noise, pts.shape)
+    return pts.astype(np.float32)
+
+
+def o3(n, seed):
+    rng = np.random.default_rng(seed)
+    mats = rng.standard_normal((n, 3, 3))
+    out = np.empty((n, 9), dtype=np.float32)
+    for i in range(n):
+        q, r = np.linalg.qr(mats[i])
+        dd = np.sign(np.diag(r))
+        dd[dd == 0] = 1.0
+        out[i] = (q * dd[np.newaxis, :]).ravel()
+    return out
+
+
+def sphere(n, d, seed, noise=0.0):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((n, d))
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    if noise:
+        x = x + rng.normal(0, noise, x.shape)
+    return x.astype(np.float32)
+
+
+def circle(n):
+    th = np.linspace(0, 2 * np.pi, n, endpoint=False).astype(np.float32)
+    return np.column_stack([np.cos(th), np.sin(th)]).astype(np.float32)
+
+
+def grid3():
+    return np.array([[x, y, z] for x in range(3) for y in range(3) for z in range(3)],
+                    dtype=np.float32)
+
+
+# (name, points, max_edge_length, max_dim)
+SMALL = [
+    ("randn_d2_s42", lambda: randn(30, 2, 42), 1.5, 2),
+    ("randn_d2_s43", lambda: randn(30, 2, 43), 1.5, 2),
+    ("randn_d2_s44", lambda: randn(30, 2, 44), 1.5, 2),
+    ("randn_d4_s42", lambda: randn(25, 4, 42), 2.0, 2),
+    ("randn_d10_s42", lambda: randn(20, 10, 42), 3.0, 2),
+    ("randn_d3_n100", lambda: randn(100, 3, 42), 1.5, 2),
+    ("clifford_200", lambda: clifford(200, 42), 0.6, 2),
+    ("o3_200", lambda: o3(200, 42), 1.8, 1),
+    ("grid3_ties", grid3, 2.0, 2),
+    ("circle_30", lambda: circle(30), 1.5, 2),
+    ("circle_20_small_t", lambda: circle(20), 0.5, 1),
+    ("cluster_tight", lambda: randn(10, 2, 42, 0.01), 0.5, 2),
+    ("no_edges", lambda: randn(50, 3, 7), 1e-10, 2),
+    ("sphere2_noisy", lambda: sphere(60, 3, 5, 0.05), 1.2, 2),
+    ("d50_n30", lambda: randn(30, 50, 3), 10.0, 1),
+]
+
+MEDIUM = [
+    ("sphere3_192", lambda: sphere(192, 4, 42), 2.001, 1),
+    ("o3_1024", lambda: o3(1024, 42), 1.8, 1),
+    ("randn_d10_n2000", lambda: randn(2000, 10, 1), 2.2, 2),
+    ("clifford_1000_noisy", lambda: clifford(1000, 3, 0.02), 0.45, 2),
+    ("randn_d10_n10000_cfg2", lambda: randn(10000, 10, 0), 1.6, 2),
+]

@@ -1,0 +1,129 @@
+"""CPU-side checks of the drop-in boundary: the C-ABI library loads and
+exports every entry point declared in include/flashph.h, argument
+validation matches the reference's exceptions and messages, and the trivial
+inputs (n = 0, 1) are answered without a GPU."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+import flash_ph
+from flash_ph import RipsComplex, auto_threshold, enclosing_radius, rips_persistence
+from flash_ph import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    src = open(os.path.join(ROOT, "include", "flashph.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(fph_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_declared_symbols():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = _declared_symbols()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in _lib.SIGNATURES, f"{name} not bound in flash_ph._lib"
+
+
+def test_version_string():
+    assert flash_ph.__version__ == "0.1.0"
+    assert b"gfx950" in _lib.load().fph_version()
+
+
+def test_workspace_queries_without_gpu():
+    lib = _lib.load()
+    assert lib.fph_boruvka_rounds(10_000) == 16
+    assert lib.fph_boruvka_scratch_bytes(10_000) > 4 * 10_000
+    assert lib.fph_reduce_scratch_bytes(1000) >= 20 * 1000
+
+
+@pytest.mark.parametrize("bad", [-1.0, -1e-9])
+def test_negative_threshold(bad):
+    with pytest.raises(ValueError, match="max_edge_length must be non-negative"):
+        rips_persistence(torch.zeros(4, 2), bad)
+    with pytest.raises(ValueError, match="max_edge_length must be non-negative"):
+        RipsComplex(torch.zeros(4, 2), bad)
+
+
+def test_bad_shapes_and_dims():
+    with pytest.raises(ValueError, match=r"points must have shape \(n, d\)"):
+        rips_persistence(torch.zeros(4), 1.0)
+    with pytest.raises(ValueError, match="max_dim must be 0, 1, or 2"):
+        rips_persistence(torch.zeros(4, 2), 1.0, max_dim=3)
+    with pytest.raises(ValueError, match="H2 requires n < 65536"):
+        rips_persistence(torch.zeros(65536, 2), 1.0, max_dim=2)
+
+
+def test_bad_backend():
+    with pytest.raises(ValueError, match="backend must be one of"):
+        rips_persistence(torch.zeros(4, 2), 1.0, max_dim=1, backend="cuda")
+
+
+@pytest.mark.parametrize("max_dim", [0, 1, 2])
+def test_empty_and_single(max_dim):
+    d0 = rips_persistence(torch.empty(0, 3), 1.0, max_dim=max_dim)
+    assert len(d0) == max_dim + 1 and all(d.shape == (0, 2) for d in d0)
+    d1 = rips_persistence(torch.tensor([[1.0, 2.0, 3.0]]), 1.0, max_dim=max_dim)
+    assert len(d1) == max_dim + 1
+    assert d1[0].shape == (1, 2) and d1[0][0, 0] == 0 and d1[0][0, 1] == float("inf")
+    assert all(d.shape == (0, 2) and d.dtype == torch.float32 for d in d1[1:])
+    assert all(d.device.type == "cpu" for d in d0 + d1)
+
+
+def test_numpy_single_point():
+    d = rips_persistence(np.zeros((1, 4), np.float32), 1.0)
+    assert d[0].shape == (1, 2)
+
+
+@pytest.mark.skipif(torch.cuda.is_available(), reason="checks the no-GPU error")
+def test_no_gpu_fails_loudly():
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        rips_persistence(torch.randn(10, 3), 1.0)
+
+
+def test_complex_errors():
+    rc = RipsComplex(torch.zeros(3, 2), 1.0)
+    with pytest.raises(RuntimeError, match="Call compute_persistence"):
+        rc.betti_numbers()
+    with pytest.raises(ValueError, match="Only Z/2Z coefficients"):
+        rc.compute_persistence(homology_coeff_field=3)
+    assert rc.num_vertices() == 3
+
+
+def test_complex_single_point():
+    rc = RipsComplex(torch.zeros(1, 2), 1.0)
+    rc.compute_persistence(max_dim=1)
+    assert rc.betti_numbers() == [1, 0]
+    assert rc.persistence_intervals_in_dimension(1).shape == (0, 2)
+    assert rc.persistence_intervals_in_dimension(5).shape == (0, 2)
+    assert rc.persistent_betti_numbers(0.0, 1.0) == [1, 0]
+    assert rc.persistence() == [(0, (0.0, float("inf")))]
+
+
+def test_auto_threshold_and_radius_cpu():
+    rng = np.random.default_rng(0)
+    pts = torch.from_numpy(rng.standard_normal((300, 3)).astype(np.float32))
+    t = auto_threshold(pts, k=10, percentile=90)
+    assert isinstance(t, float) and t > 0
+    # brute-force k-th neighbour distances in FP64
+    x = pts.double()
+    dd = torch.cdist(x, x)
+    dd.fill_diagonal_(float("inf"))
+    kth = dd.topk(10, dim=1, largest=False).values[:, -1]
+    assert abs(t - float(torch.quantile(kth, 0.9))) < 1e-4
+    r = enclosing_radius(pts)
+    dd.fill_diagonal_(0.0)
+    assert abs(r - float(dd.max(dim=1).values.min())) < 1e-4
+    assert auto_threshold(pts[:1]) == 0.0
+    assert auto_threshold(pts[:3], k=20) > 0
+    r_sub = enclosing_radius(pts, subsample=100)
+    assert 0 < r_sub and abs(r_sub - r) / r < 0.5

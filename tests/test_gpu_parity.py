@@ -1,0 +1,105 @@
+"""GPU parity: every stage of the HIP pipeline against the CPU oracle, bit
+for bit (integer, index and FP32 arrays compared with array_equal on the raw
+bits)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint32) if a.dtype == np.float32 else a
+
+
+def _eq(name, got, want):
+    got = np.asarray(got)
+    want = np.asarray(want)
+    assert got.shape == want.shape, f"{name}: shape {got.shape} vs {want.shape}"
+    assert np.array_equal(_bits(got), _bits(want)), f"{name}: values differ"
+
+
+def _cpu(t):
+    return t.detach().cpu().numpy()
+
+
+def _check_case(pts, t, max_dim):
+    import oracle
+    from flash_ph import _pipeline
+
+    ref = oracle.run(pts, t, max_dim)
+    x = torch.from_numpy(pts).cuda()
+    dgms, cx = _pipeline.compute(x, t, max_dim)
+    torch.cuda.synchronize()
+    E = cx.E
+    # stage (a): edges and CSR
+    _eq("edge_i", _cpu(cx.edge_i), ref["edge_i"])
+    _eq("edge_j", _cpu(cx.edge_j), ref["edge_j"])
+    _eq("edge_dsq", _cpu(cx.edge_dsq), ref["edge_dsq"])
+    _eq("adj_ptr", _cpu(cx.adj_ptr), ref["adj_ptr"])
+    _eq("adj_idx", _cpu(cx.adj_idx), ref["adj_idx"])
+    _eq("adj_rank", _cpu(cx.adj_rank), ref["adj_rank"])
+    _eq("adj_dsq", _cpu(cx.adj_dsq), ref["adj_dsq"])
+    # stage (b): forest and H0
+    mst = np.nonzero(_cpu(cx.mst_mask)[:E])[0]
+    _eq("mst_idx", mst, ref["mst_idx"])
+    _eq("mst_kruskal", mst, np.nonzero(ref["mst_kruskal"])[0])
+    _eq("h0", _cpu(dgms[0]), ref["h0"])
+    if max_dim < 1:
+        return
+    # stage (d): triangles in filtration order
+    T = cx.T
+    assert T == ref["stats"]["T"]
+    if T:
+        te = _cpu(cx.tri_e)[:T]
+        ours = np.sort(np.stack([_cpu(cx.edge_i)[te], _cpu(cx.edge_j)[te], _cpu(cx.tri_w)[:T]], 1),
+                       axis=1)
+        order = np.argsort(ref["tri_global_rank"])
+        theirs = np.stack([ref["tri_v0"], ref["tri_v1"], ref["tri_v2"]], 1)[order]
+        _eq("triangles_by_rank", ours, theirs)
+        # cofacet columns of non-forest edges
+        off, dat = _cpu(cx.cof_off), _cpu(cx.cof_data).view(np.uint32)
+        roff, rdat = ref["cof_off"], ref["cof_data"]
+        mask = _cpu(cx.mst_mask)[:E]
+        for e in range(E):
+            if mask[e]:
+                assert off[e + 1] == off[e]
+                continue
+            _eq(f"cof[{e}]", dat[off[e]:off[e + 1]].astype(np.int64),
+                rdat[roff[e]:roff[e + 1]].astype(np.int64))
+        # stage (c): apparent pairs
+        _eq("apparent1", (_cpu(cx.kind1)[:E] == 2).astype(np.uint8), ref["apparent1"])
+    # H1 diagram, row order included
+    _eq("h1", _cpu(dgms[1]), ref["h1"])
+    if max_dim < 2:
+        return
+    Q = cx.Q
+    assert Q == ref["stats"]["Q"]
+    if Q:
+        qe = _cpu(cx.tet_e)[:Q]
+        key = _cpu(cx.tet_key)[:Q].view(np.uint32)
+        ours = np.sort(np.stack([_cpu(cx.edge_i)[qe], _cpu(cx.edge_j)[qe],
+                                 (key >> 16).astype(np.int32), (key & 0xFFFF).astype(np.int32)],
+                                1), axis=1)
+        order = np.argsort(ref["tet_global_rank"])
+        theirs = np.stack([ref["tet_v0"], ref["tet_v1"], ref["tet_v2"], ref["tet_v3"]], 1)[order]
+        _eq("tetrahedra_by_rank", ours, theirs)
+        _eq("apparent2", (_cpu(cx.kind2)[:T] == 2).astype(np.uint8), ref["apparent2"])
+    _eq("h2", _cpu(dgms[2]), ref["h2"])
+
+
+@pytest.mark.parametrize("name,make,t,max_dim", cases.SMALL, ids=[c[0] for c in cases.SMALL])
+def test_stage_parity_small(name, make, t, max_dim):
+    print(f"[gpu] parity {name}", flush=True)
+    _check_case(make(), t, max_dim)
+
+
+@pytest.mark.parametrize("name,make,t,max_dim", cases.MEDIUM, ids=[c[0] for c in cases.MEDIUM])
+def test_stage_parity_medium(name, make, t, max_dim):
+    print(f"[gpu] parity {name}", flush=True)
+    _check_case(make(), t, max_dim)
