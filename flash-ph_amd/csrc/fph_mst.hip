@@ -178,8 +178,10 @@ extern "C" int fph_boruvka_mst(const int32_t* edge_i, const int32_t* edge_j, int
         k_bvk_jump<<<fph_blocks(n, 256), 256, 0, s>>>(st, (int)n);
     }
     FPH_LAUNCH_CHECK();
-    k_bvk_minlabel_init<<<fph_blocks(n, 256), 256, 0, s>>>(lab, (int)n);
-    k_bvk_minlabel<<<fph_blocks(n, 256), 256, 0, s>>>(st.comp, lab, (int)n);
+    if (comp_label) {
+        k_bvk_minlabel_init<<<fph_blocks(n, 256), 256, 0, s>>>(lab, (int)n);
+        k_bvk_minlabel<<<fph_blocks(n, 256), 256, 0, s>>>(st.comp, lab, (int)n);
+    }
     k_bvk_label<<<fph_blocks(n, 256), 256, 0, s>>>(st.comp, lab, (long long*)comp_label, (int)n,
                                                   st.flags, R, converged);
     FPH_LAUNCH_CHECK();

@@ -3,59 +3,75 @@
 // Replaces the Numba loop of cohomology_general.py:202-371 (with the pool
 // prefill of _numba_utils.py:68-104 and the hash map of _numba_utils.py:14-40).
 //
-// Columns (edges for H1, triangles for H2) are processed by the reference in
-// descending rank order; a column's pivot is its smallest row; a column whose
-// pivot is owned by an earlier (higher-rank) column is XORed with that
-// column.  Every reduced matrix obtained by adding earlier columns to later
-// ones has the same pivot pairs, so the same pairs are obtained here without
-// the serial order:
+// Columns (edges for H1, triangles for H2) are reduced by the reference in
+// descending rank order: a column's pivot is its smallest row, and a column
+// whose pivot is owned by an earlier (higher-rank) column is XORed with that
+// column.  Any reduction that only adds higher-rank columns to lower-rank ones
+// ends with the same pivot pairs, which lets the columns proceed concurrently:
 //
-//   phase A  (all residual columns at once, one wavefront per column)
-//            XOR with apparent columns while the pivot is owned by one.
-//            Apparent columns own their pivot forever: no column of higher
-//            rank ever contains it.
-//   phase B  (one persistent workgroup, rounds separated by barriers)
-//            each column claims its pivot with atomicMax(own[pivot], column);
-//            the highest-rank claimant owns it and every other claimant (and
-//            any displaced previous owner) is XORed with the owner in the
-//            next round, then chains apparent XORs again.  Owners never
-//            change inside a round, so reads and writes never overlap.
-//            Stops when no two nonzero columns share a pivot.
+//   own[row] (64 bits) = (owner column + 1) << 32 | pool offset of the owner's
+//   content.  A column owns its pivot while own[pivot] names it; an owner's
+//   content never changes while it owns, and every content starts with its
+//   owner's pivot, so any column of lower rank may XOR with whatever own[]
+//   currently names.
 //
-// A column is a sorted list of row ranks.  The XOR of two sorted lists is a
-// merge with cancellation, computed by a whole wavefront: every element
-// binary-searches the other list; for a kept element a = A[i],
-//     position = 2 * (#kept A before i) + lower_bound(B, a) - i,
-// and symmetrically for B.  Lists live in LDS when they fit, in a
-// bump-allocated global pool otherwise.
+//   phase A  (every residual column at once, one wavefront each): XOR with
+//            apparent owners while the pivot has one.  Apparent columns own
+//            their pivot forever (no higher-rank column contains it).
+//   phase B  (one persistent workgroup of 16 wavefronts): rounds over a
+//            worklist in descending column rank.  A wavefront takes a column
+//            and XORs it with the current owner of its pivot until the pivot
+//            is free or owned by a lower-rank column; then it publishes the
+//            content and claims the pivot with one 64-bit atomicMax.  A lost
+//            race continues with the winner; a displaced lower-rank owner is
+//            queued for the next round.  Stops when a round displaces nobody.
+//
+// A column is a sorted list of row ranks stored in a bump-allocated pool as
+// [length, rows...].  XOR of two sorted lists is a merge with cancellation
+// computed by a whole wavefront: every element binary-searches the other
+// list; a kept a = A[i] lands at 2 * (#kept A before i) + lower_bound(B, a) - i
+// (symmetrically for B).  The working column lives in LDS while it fits.
 #include "fph_common.h"
+
+#include <rocprim/rocprim.hpp>
 
 namespace {
 
-constexpr int kCapW = 320;  // LDS words per working buffer (two per wave)
-constexpr int kCapO = 256;  // LDS words for the staged owner column
-constexpr int kWaveLds = 2 * kCapW + kCapO;
+// Per-wavefront LDS arena: [half 0 | half 1 | stage].  The working column
+// lives in one half, an XOR writes the other half (no copy back), and a long
+// owner column is staged in the stage area.  Phase B: 16 waves x 9.75 KB.
+constexpr int kHalfA = 768, kStageA = 512;    // phase A: 4 waves x 8 KB
+constexpr int kHalfB = 1024, kStageB = 448;   // phase B: 16 waves x 9.75 KB
+constexpr int kArenaA = 2 * kHalfA + kStageA;
+constexpr int kArenaB = 2 * kHalfB + kStageB;
 constexpr int kPhaseAWaves = 4;
 constexpr int kPhaseBWaves = 16;
+constexpr unsigned long long kNoOwner = 0ull;
 
 struct RedArgs {
     const int* off;                 // CSR row offsets, K + 1
     const unsigned* data;           // CSR rows, ascending global row ranks
     long long row_base;             // global rank of row 0
+    long long nrows;                // rows (length of own)
+    long long K;                    // columns
     const unsigned char* kind;      // 2 = apparent, 3 = residual
-    int* own;                       // nrows: owning column of a pivot row or -1
-    int* pivot;                     // K: current pivot (global rank) or -1
-    unsigned long long* col_ptr;    // K: address of the current column content
-    int* col_len;                   // K
+    unsigned long long* own;        // nrows: owner encoding (see above), 0 = none
+    int* pivot;                     // K: final pivot (global rank) or -1
+    unsigned* desc;                 // K: pool offset of each residual column
     unsigned* pool;
     unsigned long long pool_cap;
     unsigned long long* pool_ptr;
-    int* status;                    // [0] pool overflow, [1] rounds, [2] xors
+    int* status;  // [0] error, [1] rounds, [2] XORs, [3] longest column, [4] off-chip XORs
 };
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ unsigned long long load_own(const RedArgs& a, unsigned p) {
+    return __hip_atomic_load(&a.own[(long long)p - a.row_base], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ int lb_u32(const unsigned* x, int n, unsigned v) {
@@ -68,44 +84,58 @@ __device__ __forceinline__ int lb_u32(const unsigned* x, int n, unsigned v) {
     return lo;
 }
 
-// C = A xor B for sorted duplicate-free lists; C must not alias A or B.
+// C = A xor B for sorted duplicate-free lists (na, nb >= 1); C must not
+// alias A or B.  Only the heads up to min(max A, max B) interleave: with
+// sa = na, sb = lower_bound(B, max A + 1) when max A <= max B (or the mirror
+// image), every element of a tail sits after all of the other list, so the
+// tail moves by a constant (its list's index + the other length - 2 * the
+// number of cancelled pairs m).  Head elements find their position by a
+// binary search in the other head:
+//     pos(A[i]) = 2 * (#kept A before i) + lower_bound(B, A[i]) - i.
 __device__ int wave_xor(const unsigned* A, int na, const unsigned* B, int nb, unsigned* C) {
     const int lane = fph_lane();
     const unsigned long long lt = fph_lanemask_lt();
-    int run_a = 0;
-    for (int base = 0; base < na; base += FPH_WAVE) {
+    const unsigned amax = A[na - 1], bmax = B[nb - 1];
+    int sa = na, sb = nb;
+    if (amax <= bmax) sb = lb_u32(B, nb, amax + 1u);
+    else sa = lb_u32(A, na, bmax + 1u);
+    int kept_a = 0;
+    for (int base = 0; base < sa; base += FPH_WAVE) {
         const int i = base + lane;
         bool keep = false;
         unsigned a = 0;
         int pos = 0;
-        if (i < na) {
+        if (i < sa) {
             a = A[i];
-            const int lb = lb_u32(B, nb, a);
-            keep = !(lb < nb && B[lb] == a);
+            const int lb = lb_u32(B, sb, a);
+            keep = !(lb < sb && B[lb] == a);
             pos = lb - i;
         }
         const unsigned long long bal = __ballot(keep);
-        if (keep) C[2 * (run_a + __popcll(bal & lt)) + pos] = a;
-        run_a += __popcll(bal);
+        if (keep) C[2 * (kept_a + __popcll(bal & lt)) + pos] = a;
+        kept_a += __popcll(bal);
     }
-    int run_b = 0;
-    for (int base = 0; base < nb; base += FPH_WAVE) {
+    const int m = sa - kept_a;  // cancelled pairs (all inside the heads)
+    int kept_b = 0;
+    for (int base = 0; base < sb; base += FPH_WAVE) {
         const int j = base + lane;
         bool keep = false;
         unsigned b = 0;
         int pos = 0;
-        if (j < nb) {
+        if (j < sb) {
             b = B[j];
-            const int lb = lb_u32(A, na, b);
-            keep = !(lb < na && A[lb] == b);
+            const int lb = lb_u32(A, sa, b);
+            keep = !(lb < sa && A[lb] == b);
             pos = lb - j;
         }
         const unsigned long long bal = __ballot(keep);
-        if (keep) C[2 * (run_b + __popcll(bal & lt)) + pos] = b;
-        run_b += __popcll(bal);
+        if (keep) C[2 * (kept_b + __popcll(bal & lt)) + pos] = b;
+        kept_b += __popcll(bal);
     }
+    for (int i = sa + lane; i < na; i += FPH_WAVE) C[i + nb - 2 * m] = A[i];
+    for (int j = sb + lane; j < nb; j += FPH_WAVE) C[j + na - 2 * m] = B[j];
     wave_sync();
-    return run_a + run_b;
+    return na + nb - 2 * m;
 }
 
 __device__ __forceinline__ void wave_copy(const unsigned* src, int n, unsigned* dst) {
@@ -113,266 +143,395 @@ __device__ __forceinline__ void wave_copy(const unsigned* src, int n, unsigned* 
     wave_sync();
 }
 
-// Wave-uniform pool allocation; returns nullptr on overflow (flag raised).
-__device__ unsigned* pool_alloc(const RedArgs& a, int n) {
-    unsigned long long at = 0;
-    if (fph_lane() == 0) at = atomicAdd(a.pool_ptr, (unsigned long long)n);
-    at = __shfl(at, 0);
-    if (at + (unsigned long long)n > a.pool_cap) {
-        if (fph_lane() == 0) atomicExch(&a.status[0], 1);
-        return nullptr;
+// Pool allocations are rounded up to whole 128-byte lines (32 words), so a
+// speculative one-line read at the start of a column never touches another
+// column's words.  Each wavefront carves its allocations out of a private
+// chunk and refills the chunk with one atomic.
+constexpr int kLine = 32;
+
+struct Alloc {
+    unsigned long long cur;
+    unsigned long long end;
+    unsigned long long chunk;  // words reserved per refill
+};
+
+// Wave-uniform allocation of n words; returns the word offset, or -1 on
+// overflow (error flag raised).
+__device__ long long pool_alloc(const RedArgs& a, Alloc& al, int n) {
+    const unsigned long long need = ((unsigned long long)n + kLine - 1) / kLine * kLine;
+    if (al.cur + need > al.end) {
+        const unsigned long long take = need > al.chunk ? need : al.chunk;
+        unsigned long long at = 0;
+        if (fph_lane() == 0) at = atomicAdd(a.pool_ptr, take);
+        at = __shfl(at, 0);
+        if (at + take > a.pool_cap || at + take > 0xFFFFFFFFull) {
+            if (fph_lane() == 0) atomicExch(&a.status[0], 1);
+            return -1;
+        }
+        al.cur = at;
+        al.end = at + take;
     }
-    return a.pool + at;
+    const unsigned long long at = al.cur;
+    al.cur += need;
+    return (long long)at;
+}
+
+// Speculative read of the first line of a pooled column [len, rows...]:
+// lane l < 32 holds word l; returns len (one round trip for short columns).
+__device__ __forceinline__ int read_line(const unsigned* src, unsigned& v) {
+    v = fph_lane() < kLine ? src[fph_lane()] : 0u;
+    return (int)__shfl(v, 0);
+}
+
+// Finish copying the rows of a column whose first line is in v.
+__device__ __forceinline__ void finish_copy(const unsigned* src, unsigned v, int len,
+                                            unsigned* dst) {
+    const int lane = fph_lane();
+    if (lane >= 1 && lane < kLine && lane <= len) dst[lane - 1] = v;
+    for (int i = kLine + lane; i <= len; i += FPH_WAVE) dst[i - 1] = src[i];
+    wave_sync();
 }
 
 struct Work {
-    const unsigned* ptr;
+    const unsigned* ptr;  // current rows (arena half, pool or CSR, read-only)
     int len;
-    int where;  // 0 external (read-only), 1 LDS buf0, 2 LDS buf1, 3 pool
+    int half;             // arena half holding the rows, -1 when off chip
 };
 
-__device__ __forceinline__ void work_load(Work& w, const unsigned* src, int n, unsigned* lds) {
-    if (n <= kCapW) {
-        wave_copy(src, n, lds);
-        w.ptr = lds;
-        w.where = 1;
+// Start from the pooled column [len, rows...] at src: on chip (half 0) when
+// it fits, else read in place.
+__device__ __forceinline__ void work_load(Work& w, const unsigned* src, unsigned* arena, int H) {
+    unsigned v;
+    const int len = read_line(src, v);
+    w.len = len;
+    if (len <= H) {
+        finish_copy(src, v, len, arena);
+        w.ptr = arena;
+        w.half = 0;
     } else {
-        w.ptr = src;
-        w.where = 0;
+        w.ptr = src + 1;
+        w.half = -1;
     }
-    w.len = n;
 }
 
-// w <- w xor B.  Returns false on pool overflow.
-__device__ bool xor_step(Work& w, const unsigned* B, int nb, unsigned* lds, const RedArgs& a) {
-    unsigned* b0 = lds;
-    unsigned* b1 = lds + kCapW;
-    unsigned* st = lds + 2 * kCapW;
-    const int need = w.len + nb;
-    unsigned* out;
-    int where;
-    if (need <= kCapW) {
-        out = (w.where == 1) ? b1 : b0;
-        where = (w.where == 1) ? 2 : 1;
-    } else {
-        out = pool_alloc(a, need);
-        if (!out) return false;
-        where = 3;
-    }
-    const unsigned* bs = B;
-    if (nb <= kCapO) {
-        wave_copy(B, nb, st);
-        bs = st;
-    }
+// w <- w xor (pooled column at ob), whose pivot equals w's.  False on error.
+template <int H, int O>
+__device__ bool xor_step(Work& w, const unsigned* ob, unsigned* arena, const RedArgs& a,
+                         Alloc& al) {
+    unsigned v;
+    const int nb = read_line(ob, v);
     const unsigned p_old = w.ptr[0];
-    w.len = wave_xor(w.ptr, w.len, bs, nb, out);
-    if (where == 3) __threadfence();
+    const int na = w.len;
+    unsigned* out;
+    int out_half = -1;
+    if (w.half >= 0 && na + nb <= H) {
+        out_half = 1 - w.half;
+        out = arena + out_half * H;
+    } else {
+        if (fph_lane() == 0) atomicAdd(&a.status[4], 1);
+        const long long at = pool_alloc(a, al, na + nb);
+        if (at < 0) return false;
+        out = a.pool + at;
+    }
+    const unsigned* bs = ob + 1;
+    if (nb <= O) {
+        finish_copy(ob, v, nb, arena + 2 * H);
+        bs = arena + 2 * H;
+    }
+    int len = wave_xor(w.ptr, na, bs, nb, out);
+    if (out_half < 0 && len <= H) {  // bring an off-chip result back on chip
+        const int h = w.half >= 0 ? 1 - w.half : 0;
+        for (int i = fph_lane(); i < len; i += FPH_WAVE) arena[h * H + i] = out[i];
+        wave_sync();
+        out = arena + h * H;
+        out_half = h;
+    }
     w.ptr = out;
-    w.where = where;
-    // The XOR of two columns with the same pivot removes it, so the pivot
+    w.half = out_half;
+    w.len = len;
+    // XOR of two columns with the same pivot removes it, so the pivot
     // strictly increases; anything else means inconsistent inputs.
-    if (w.len > 0 && !(w.ptr[0] > p_old)) {
+    if (len > 0 && !(w.ptr[0] > p_old)) {
         if (fph_lane() == 0) atomicExch(&a.status[0], 3);
         return false;
     }
     return true;
 }
 
-// XOR with apparent owners while the pivot has one.
-__device__ bool chain_apparent(Work& w, unsigned* lds, const RedArgs& a, int* nxor) {
-    while (w.len > 0) {
-        const unsigned p = w.ptr[0];
-        const int o = a.own[(long long)p - a.row_base];
-        if (o < 0 || a.kind[o] != 2) break;
-        const int lo = a.off[o];
-        if (!xor_step(w, a.data + lo, a.off[o + 1] - lo, lds, a)) return false;
-        (*nxor)++;
-    }
-    return true;
+// Copy the working column into the pool as [len, rows...]; returns the
+// offset or -1.
+__device__ long long publish(const Work& w, const RedArgs& a, Alloc& al) {
+    const long long at = pool_alloc(a, al, w.len + 1);
+    if (at < 0) return -1;
+    if (fph_lane() == 0) a.pool[at] = (unsigned)w.len;
+    wave_copy(w.ptr, w.len, a.pool + at + 1);
+    return at;
 }
 
-// Publish the working column as column c's content.
-__device__ bool work_store(Work& w, int c, const unsigned* orig, const RedArgs& a, bool changed) {
-    const unsigned* final_ptr = w.ptr;
-    if (!changed) {
-        final_ptr = orig;
-    } else if (w.where == 1 || w.where == 2) {
-        unsigned* dst = w.len > 0 ? pool_alloc(a, w.len) : a.pool;
-        if (!dst) return false;
-        wave_copy(w.ptr, w.len, dst);
-        __threadfence();
-        final_ptr = dst;
-    }
-    const unsigned piv = w.len > 0 ? w.ptr[0] : 0u;
-    if (fph_lane() == 0) {
-        a.col_ptr[c] = (unsigned long long)(uintptr_t)final_ptr;
-        a.col_len[c] = w.len;
-        a.pivot[c] = w.len > 0 ? (int)piv : -1;
-    }
-    return true;
+__device__ __forceinline__ void note_len(const RedArgs& a, int len) {
+    if (fph_lane() == 0 && len > 0) atomicMax(&a.status[3], len);
 }
 
+// Apparent columns are laid out at pool offset off[c] + c as [len, rows...]
+// (no allocation), and own[] records them.  Dynamic allocations start after
+// this region.
+__global__ void k_publish_apparent(RedArgs a, long long K) {
+    const int wpb = blockDim.x / FPH_WAVE;
+    const long long nw = (long long)gridDim.x * wpb;
+    for (long long c = (long long)blockIdx.x * wpb + threadIdx.x / FPH_WAVE; c < K; c += nw) {
+        if (a.kind[c] != 2) continue;
+        const int lo = a.off[c], len = a.off[c + 1] - lo;
+        const long long at = (long long)lo + c;
+        if (fph_lane() == 0) {
+            a.pool[at] = (unsigned)len;
+            a.own[(long long)a.data[lo] - a.row_base] =
+                ((unsigned long long)(c + 1) << 32) | (unsigned long long)at;
+        }
+        for (int i = fph_lane(); i < len; i += FPH_WAVE) a.pool[at + 1 + i] = a.data[lo + i];
+    }
+}
+
+// Phase A: one wavefront per residual column, chaining apparent owners.
 __global__ __launch_bounds__(kPhaseAWaves * FPH_WAVE) void k_reduce_chain(
     RedArgs a, const int* __restrict__ res_list, const int* __restrict__ res_count) {
     extern __shared__ unsigned smem[];
     const int wid = threadIdx.x / FPH_WAVE;
-    unsigned* lds = smem + wid * kWaveLds;
+    unsigned* arena = smem + wid * kArenaA;
     const int R = *res_count;
     const long long nw = (long long)gridDim.x * kPhaseAWaves;
     int nxor = 0;
+    Alloc al = {0, 0, 128};
     for (long long r = (long long)blockIdx.x * kPhaseAWaves + wid; r < R; r += nw) {
         if (a.status[0]) return;
         const int c = res_list[r];
-        const int lo = a.off[c];
-        const unsigned* orig = a.data + lo;
+        const int lo = a.off[c], n0 = a.off[c + 1] - lo;
         Work w;
-        work_load(w, orig, a.off[c + 1] - lo, lds);
-        const int before = nxor;
-        if (!chain_apparent(w, lds, a, &nxor)) return;
-        if (!work_store(w, c, orig, a, nxor != before)) return;
+        w.len = n0;
+        if (n0 <= kHalfA) {
+            wave_copy(a.data + lo, n0, arena);
+            w.ptr = arena;
+            w.half = 0;
+        } else {
+            w.ptr = a.data + lo;  // read-only; XOR results go elsewhere
+            w.half = -1;
+        }
+        while (w.len > 0) {
+            const unsigned long long v = a.own[(long long)w.ptr[0] - a.row_base];
+            if (v == kNoOwner) break;
+            if (!xor_step<kHalfA, kStageA>(w, a.pool + (v & 0xFFFFFFFFull), arena, a, al))
+                return;
+            nxor++;
+        }
+        note_len(a, w.len);
+        const long long at = publish(w, a, al);
+        if (at < 0) return;
+        if (fph_lane() == 0) {
+            a.desc[c] = (unsigned)at;
+            a.pivot[c] = w.len > 0 ? (int)w.ptr[0] : -1;
+        }
     }
     if (fph_lane() == 0 && nxor) atomicAdd(&a.status[2], nxor);
 }
 
-__device__ __forceinline__ void claim(const RedArgs& a, int c, int* wl, int* cnt) {
-    const int p = a.pivot[c];
-    const int old = atomicMax(&a.own[(long long)p - a.row_base], c);
-    if (old > c) wl[atomicAdd(cnt, 1)] = c;
-    else if (old >= 0) wl[atomicAdd(cnt, 1)] = old;
+// Phase B work item: reduce column c against current owners, then claim.
+// Returns the displaced column (>= 0), -1 if none, -2 on error.
+__device__ int reduce_claim(const RedArgs& a, int c, unsigned* arena, int* nxor, Alloc& al) {
+    Work w;
+    work_load(w, a.pool + a.desc[c], arena, kHalfB);
+    while (true) {
+        if (w.len == 0) {
+            if (fph_lane() == 0) a.pivot[c] = -1;
+            return -1;
+        }
+        const unsigned p = w.ptr[0];
+        unsigned long long v = load_own(a, p);
+        if ((int)(v >> 32) - 1 <= c) {
+            const long long at = publish(w, a, al);
+            if (at < 0) return -2;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            const unsigned long long mine =
+                ((unsigned long long)(c + 1) << 32) | (unsigned long long)at;
+            unsigned long long old = 0;
+            if (fph_lane() == 0) old = atomicMax(&a.own[(long long)p - a.row_base], mine);
+            old = __shfl(old, 0);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const int oc = (int)(old >> 32) - 1;
+            if (oc < c) {
+                if (fph_lane() == 0) {
+                    a.desc[c] = (unsigned)at;
+                    a.pivot[c] = (int)p;
+                }
+                note_len(a, w.len);
+                return oc;  // displaced owner (or -1)
+            }
+            v = old;  // a higher-rank column claimed p first: reduce by it
+        }
+        if (!xor_step<kHalfB, kStageB>(w, a.pool + (v & 0xFFFFFFFFull), arena, a, al))
+            return -2;
+        (*nxor)++;
+    }
 }
 
 __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_rounds(
-    RedArgs a, const int* __restrict__ res_list, const int* __restrict__ res_count,
+    RedArgs a, const unsigned* __restrict__ order, const int* __restrict__ res_count,
     int* __restrict__ wl0, int* __restrict__ wl1, int max_rounds) {
     extern __shared__ unsigned smem[];
-    __shared__ int s_cnt[2];
-    __shared__ int s_abort;
+    __shared__ int s_head, s_cnt, s_next_cnt, s_abort;
     const int tid = threadIdx.x;
     const int wid = tid / FPH_WAVE;
-    unsigned* lds = smem + wid * kWaveLds;
-    const int R = *res_count;
+    unsigned* arena = smem + wid * kArenaB;
     if (tid == 0) {
-        s_cnt[0] = 0;
-        s_cnt[1] = 0;
         s_abort = a.status[0];
+        s_cnt = *res_count;
+    }
+    // Phases A and the apparent publication ran on every XCD; this
+    // workgroup's L2 has none of their output.  Touch one word per 128-byte
+    // line of everything the rounds read so the dependent loads below hit L2.
+    {
+        unsigned acc = 0;
+        const long long used = (long long)*a.pool_ptr;
+        const long long step = (long long)blockDim.x * 32;
+        for (long long i = (long long)tid * 32; i < used; i += step) acc ^= a.pool[i];
+        const unsigned* ow = (const unsigned*)a.own;
+        for (long long i = (long long)tid * 32; i < 2 * a.nrows; i += step) acc ^= ow[i];
+        for (long long i = (long long)tid * 32; i < a.K; i += step)
+            acc ^= a.desc[i] ^ (unsigned)a.pivot[i] ^ (unsigned)a.off[i];
+        for (long long i = (long long)tid * 128; i < a.K; i += step * 4) acc ^= a.kind[i];
+        asm volatile("" ::"v"(acc));
     }
     __syncthreads();
     if (s_abort) return;
-    for (int r = tid; r < R; r += blockDim.x) {
-        const int c = res_list[r];
-        if (a.col_len[c] > 0) claim(a, c, wl0, &s_cnt[0]);
-    }
-    __syncthreads();
-    int cur = 0, rounds = 0, nxor = 0;
+    int rounds = 0, nxor = 0;
+    Alloc al = {0, 0, 4096};
+    // Round 0 walks every residual column in descending rank (the order the
+    // reference processes them); later rounds the columns displaced before.
+    const int* wl = (const int*)order;
+    int* wn = wl0;
     while (true) {
-        const int n = s_cnt[cur];
-        if (n == 0 || s_abort || rounds >= max_rounds) break;
-        int* wl = cur ? wl1 : wl0;
-        int* wn = cur ? wl0 : wl1;
-        for (int k = wid; k < n; k += kPhaseBWaves) {
-            if (s_abort) break;
+        const int n = s_cnt;
+        if (n == 0 || s_abort) break;
+        if (rounds >= max_rounds) {
+            if (tid == 0) atomicExch(&a.status[0], 2);
+            break;
+        }
+        if (tid == 0) {
+            s_head = 0;
+            s_next_cnt = 0;
+        }
+        __syncthreads();
+        while (true) {
+            int k = 0;
+            if (fph_lane() == 0) k = atomicAdd(&s_head, 1);
+            k = __shfl(k, 0);
+            if (k >= n || s_abort) break;
             const int c = wl[k];
-            const int p = a.pivot[c];
-            const int o = a.own[(long long)p - a.row_base];
-            if (o <= c) {  // an owner is always a higher-rank column
-                if (fph_lane() == 0) {
-                    atomicExch(&a.status[0], 3);
-                    s_abort = 1;
-                }
-                break;
-            }
-            const unsigned* ob;
-            int on;
-            if (a.kind[o] == 2) {  // apparent owners keep their CSR row
-                ob = a.data + a.off[o];
-                on = a.off[o + 1] - a.off[o];
-            } else {
-                ob = (const unsigned*)(uintptr_t)a.col_ptr[o];
-                on = a.col_len[o];
-            }
-            Work w;
-            const unsigned* orig = (const unsigned*)(uintptr_t)a.col_ptr[c];
-            work_load(w, orig, a.col_len[c], lds);
-            bool ok = xor_step(w, ob, on, lds, a);
-            nxor++;
-            if (ok) ok = chain_apparent(w, lds, a, &nxor);
-            if (ok) ok = work_store(w, c, orig, a, true);
-            if (!ok) {
+            if (a.pivot[c] < 0) continue;  // zero after phase A
+            const int d = reduce_claim(a, c, arena, &nxor, al);
+            if (d == -2) {
                 if (fph_lane() == 0) s_abort = 1;
                 break;
             }
+            if (d >= 0 && fph_lane() == 0) wn[atomicAdd(&s_next_cnt, 1)] = d;
         }
         __syncthreads();
-        if (tid == 0) s_cnt[cur ^ 1] = 0;
+        if (tid == 0) s_cnt = s_next_cnt;
         __syncthreads();
-        if (!s_abort) {
-            for (int k = tid; k < n; k += blockDim.x) {
-                const int c = wl[k];
-                if (a.col_len[c] > 0) claim(a, c, wn, &s_cnt[cur ^ 1]);
-            }
-        }
-        __syncthreads();
-        cur ^= 1;
+        wl = wn;
+        wn = (wn == wl0) ? wl1 : wl0;
         rounds++;
     }
-    if (tid == 0) {
-        a.status[1] = rounds;
-        if (s_cnt[cur] != 0 && !s_abort) a.status[0] = 2;  // round cap reached
-    }
+    if (tid == 0) a.status[1] = rounds;
     if (fph_lane() == 0 && nxor) atomicAdd(&a.status[2], nxor);
 }
 
-__global__ void k_reduce_init(int* __restrict__ status, unsigned long long* __restrict__ pool_ptr) {
+__global__ void k_reduce_init(int* __restrict__ status, unsigned long long* __restrict__ pool_ptr,
+                              const int* __restrict__ off, long long K) {
     if (threadIdx.x == 0) {
         status[0] = 0;
         status[1] = 0;
         status[2] = 0;
-        pool_ptr[0] = 0;
+        status[3] = 0;
+        status[4] = 0;
+        // apparent region: off[K] + K words, one line of padding (speculative
+        // reads of the last apparent column), then line-aligned dynamic space
+        pool_ptr[0] = ((unsigned long long)off[K] + K + kLine - 1) / kLine * kLine + kLine;
     }
+}
+
+// Sort keys: the residual columns, the unused tail set to 0 (0 sorts last
+// in descending order; only the first res_count sorted keys are read, and a
+// residual column 0 is equal to the padding).
+__global__ void k_order_keys(const int* __restrict__ res_list, const int* __restrict__ res_count,
+                             long long K, unsigned* __restrict__ keys) {
+    long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < K) keys[r] = r < *res_count ? (unsigned)res_list[r] : 0u;
+}
+
+inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+size_t sort_bytes(int64_t K) {
+    size_t tmp = 0;
+    (void)rocprim::radix_sort_keys_desc(nullptr, tmp, (const unsigned*)nullptr, (unsigned*)nullptr,
+                                        (unsigned)std::max<int64_t>(K, 1), 0, 32, (hipStream_t)0);
+    return align_up(tmp);
 }
 
 }  // namespace
 
 extern "C" size_t fph_reduce_scratch_bytes(int64_t K) {
-    const size_t k = (size_t)std::max<int64_t>(K, 1);
-    // col_ptr (8K) + col_len (4K) + two worklists (8K) + pool_ptr + status
-    return 8 * k + 4 * k + 8 * k + 256;
+    const size_t k4 = align_up(4 * (size_t)std::max<int64_t>(K, 1));
+    // desc, two worklists, sort keys in/out, sort temp, pool counter
+    return 5 * k4 + sort_bytes(K) + 256;
 }
 
 extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col_data,
-                          int64_t row_base, const uint8_t* kind, int32_t* own, int32_t* pivot,
+                          int64_t row_base, int64_t nrows, const uint8_t* kind, uint64_t* own,
+                          int32_t* pivot,
                           const int32_t* res_list, const int32_t* res_count, uint32_t* pool,
                           int64_t pool_cap, void* scratch, size_t scratch_bytes,
                           int32_t* status, int max_rounds, void* stream) {
     hipStream_t s = (hipStream_t)stream;
-    if (K < 0 || pool_cap < 0 || !scratch || !status) return FPH_ERR_ARG;
+    if (K < 0 || nrows < 0 || pool_cap < 0 || !scratch || !status) return FPH_ERR_ARG;
     if (scratch_bytes < fph_reduce_scratch_bytes(K)) return FPH_ERR_ARG;
-    const size_t k = (size_t)std::max<int64_t>(K, 1);
+    const size_t k4 = align_up(4 * (size_t)std::max<int64_t>(K, 1));
     char* w = (char*)scratch;
     RedArgs a;
     a.off = col_off;
     a.data = col_data;
     a.row_base = row_base;
+    a.nrows = nrows;
+    a.K = K;
     a.kind = kind;
-    a.own = own;
+    a.own = (unsigned long long*)own;
     a.pivot = pivot;
-    a.col_ptr = (unsigned long long*)w;
-    a.col_len = (int*)(w + 8 * k);
-    int* wl0 = (int*)(w + 12 * k);
-    int* wl1 = (int*)(w + 16 * k);
-    a.pool_ptr = (unsigned long long*)(w + 20 * k + ((8 - (20 * k) % 8) % 8));
+    a.desc = (unsigned*)w;
+    int* wl0 = (int*)(w + k4);
+    int* wl1 = (int*)(w + 2 * k4);
+    unsigned* keys = (unsigned*)(w + 3 * k4);
+    unsigned* order = (unsigned*)(w + 4 * k4);
+    void* sort_tmp = w + 5 * k4;
+    size_t sort_sz = sort_bytes(K);
+    a.pool_ptr = (unsigned long long*)(w + 5 * k4 + sort_sz);
     a.pool = pool;
     a.pool_cap = (unsigned long long)pool_cap;
     a.status = status;
-    k_reduce_init<<<1, 64, 0, s>>>(status, a.pool_ptr);
+    k_reduce_init<<<1, 64, 0, s>>>(status, a.pool_ptr, col_off, K);
     FPH_LAUNCH_CHECK();
     if (K == 0) return FPH_OK;
-    const unsigned grid_a = 1024;
-    k_reduce_chain<<<grid_a, kPhaseAWaves * FPH_WAVE, kPhaseAWaves * kWaveLds * 4, s>>>(
-        a, res_list, res_count);
+    const unsigned grid_w =
+        (unsigned)std::min<int64_t>(1024, (K + kPhaseAWaves - 1) / kPhaseAWaves);
+    k_publish_apparent<<<grid_w, kPhaseAWaves * FPH_WAVE, 0, s>>>(a, K);
+    const size_t lds_a = kPhaseAWaves * kArenaA * 4;
+    k_reduce_chain<<<grid_w, kPhaseAWaves * FPH_WAVE, lds_a, s>>>(a, res_list, res_count);
+    k_order_keys<<<fph_blocks(K, 256), 256, 0, s>>>(res_list, res_count, K, keys);
     FPH_LAUNCH_CHECK();
-    k_reduce_rounds<<<1, kPhaseBWaves * FPH_WAVE, kPhaseBWaves * kWaveLds * 4, s>>>(
-        a, res_list, res_count, wl0, wl1, max_rounds);
+    FPH_CHECK_HIP(rocprim::radix_sort_keys_desc(sort_tmp, sort_sz, keys, order, (unsigned)K, 0,
+                                                32, s));
+    const size_t lds_b = kPhaseBWaves * kArenaB * 4;
+    FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_reduce_rounds,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
+    k_reduce_rounds<<<1, kPhaseBWaves * FPH_WAVE, lds_b, s>>>(a, order, res_count, wl0, wl1,
+                                                               max_rounds);
     FPH_LAUNCH_CHECK();
     return FPH_OK;
 }

@@ -156,7 +156,7 @@ __global__ void k_fill_u8(unsigned char* __restrict__ p, long long n) {
 __global__ void k_apparent(long long K, long long col_base, const int* __restrict__ off,
                            const unsigned* __restrict__ data, const unsigned char* __restrict__ skip,
                            const int* __restrict__ row_cmf, long long row_base,
-                           unsigned char* __restrict__ kind, int* __restrict__ own,
+                           unsigned char* __restrict__ kind,
                            int* __restrict__ pivot, int* __restrict__ res_list,
                            int* __restrict__ res_count) {
     long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -171,7 +171,6 @@ __global__ void k_apparent(long long K, long long col_base, const int* __restric
         const unsigned y = data[off[c]];
         if ((long long)row_cmf[y - row_base] == col_base + c) {
             kd = 2;
-            own[y - row_base] = (int)c;
             p = (int)y;
         } else {
             kd = 3;
@@ -466,19 +465,19 @@ extern "C" int fph_cofacets_h1(const int32_t* edge_i, const int32_t* edge_j, int
 extern "C" int fph_apparent(int64_t K, int64_t col_base, const int32_t* col_off,
                             const uint32_t* col_data, const uint8_t* skip_mask,
                             const int32_t* row_cmf, int64_t row_base, int64_t nrows,
-                            uint8_t* kind, int32_t* own, int32_t* pivot, int32_t* res_list,
+                            uint8_t* kind, uint64_t* own, int32_t* pivot, int32_t* res_list,
                             int32_t* res_count, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     if (K < 0 || nrows < 0) return FPH_ERR_ARG;
     if (nrows > 0) {
-        k_fill_i32<<<fph_blocks(nrows, 256), 256, 0, s>>>(own, nrows, -1);
+        k_fill_i32<<<fph_blocks(2 * nrows, 256), 256, 0, s>>>((int*)own, 2 * nrows, 0);
         FPH_LAUNCH_CHECK();
     }
     k_fill_i32<<<1, 64, 0, s>>>(res_count, 1, 0);
     FPH_LAUNCH_CHECK();
     if (K == 0) return FPH_OK;
     k_apparent<<<fph_blocks(K, 256), 256, 0, s>>>(K, col_base, col_off, col_data, skip_mask,
-                                                 row_cmf, row_base, kind, own, pivot, res_list,
+                                                 row_cmf, row_base, kind, pivot, res_list,
                                                  res_count);
     FPH_LAUNCH_CHECK();
     return FPH_OK;

@@ -242,25 +242,27 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
     lib = load()
     s = stream_of(device)
     kind = _empty(torch.uint8, device, max(K, 1))
-    own = _empty(torch.int32, device, max(nrows, 1))
+    own = _empty(torch.int64, device, max(nrows, 1))
     pivot = _empty(torch.int32, device, max(K, 1))
     res_list = _empty(torch.int32, device, max(K, 1))
-    # info32: [0..2] reduction status, [3] residual count, [4] diagram rows
+    # info32: [0..4] reduction status, [5] residual count, [6] diagram rows
     info32 = torch.zeros(8, dtype=torch.int32, device=device)
     status_p = info32.data_ptr()
-    rescnt_p = info32.data_ptr() + 12
-    count_p = info32.data_ptr() + 16
+    rescnt_p = info32.data_ptr() + 20
+    count_p = info32.data_ptr() + 24
     scratch_bytes = lib.fph_reduce_scratch_bytes(K)
-    pool_words = max(_POOL_MIN_WORDS, 8 * C)
+    # apparent columns in place (C + K words) + line-rounded reduced columns
+    pool_words = C + K + 64 + max(_POOL_MIN_WORDS, 64 * K + 4 * C)
     out = _empty(torch.float32, device, max(K, 1), 2)
     while True:
         check(lib.fph_apparent(K, col_base, ptr(off), ptr(data), ptr(skip), ptr(row_cmf),
                                row_base, nrows, ptr(kind), ptr(own), ptr(pivot), ptr(res_list),
                                rescnt_p, s), "fph_apparent")
-        pool = _empty(torch.int32, device, pool_words)
+        pool = _empty(torch.int32, device, pool_words + 64)  # +64: speculative line reads
         scratch = _empty(torch.uint8, device, scratch_bytes)
         tok = TIMER.start("reduce", device)
-        check(lib.fph_reduce(K, ptr(off), ptr(data), row_base, ptr(kind), ptr(own), ptr(pivot),
+        check(lib.fph_reduce(K, ptr(off), ptr(data), row_base, nrows, ptr(kind), ptr(own),
+                             ptr(pivot),
                              ptr(res_list), rescnt_p, ptr(pool), pool_words, ptr(scratch),
                              scratch_bytes, status_p, _MAX_ROUNDS, s), "fph_reduce")
         TIMER.stop(tok)
@@ -275,8 +277,9 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
             pool_words *= 4
             continue
         raise _lib.FlashPHError(f"fph_reduce failed with status {st[0]}")
-    stats = {"rounds": st[1], "xors": st[2], "residual": st[3]}
-    return kind, pivot, out[: st[4]], stats
+    stats = {"rounds": st[1], "xors": st[2], "longest": st[3], "offchip_xors": st[4],
+             "residual": st[5]}
+    return kind, pivot, out[: st[6]], stats
 
 
 def build_h2(cx: Complex) -> None:

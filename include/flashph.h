@@ -147,8 +147,8 @@ int fph_cofacets_h1(const int32_t *edge_i, const int32_t *edge_j, int64_t E, int
  * Classify K columns (global rank col_base + c) with cofacet rows
  * (col_off, col_data; global row ranks, row 0 = row_base, nrows rows):
  * kind = 0 skipped (skip_mask), 1 no cofacet (essential), 2 apparent
- * (row_cmf[youngest - row_base] == column rank), 3 residual.  own (nrows) is
- * reset to -1 then own[youngest - row_base] = c for apparent columns;
+ * (row_cmf[youngest - row_base] == column rank), 3 residual.  own (nrows,
+ * 64-bit) is cleared for fph_reduce, which records the pivot owners in it;
  * pivot[c] = youngest for apparent columns, -1 otherwise; residual columns
  * are listed (unordered) in res_list, their number in *res_count (device).
  * Replaces _gpu_apparent_pair_predetect, reduce_h1_gpu.py:159-223, for H1
@@ -156,7 +156,7 @@ int fph_cofacets_h1(const int32_t *edge_i, const int32_t *edge_j, int64_t E, int
  */
 int fph_apparent(int64_t K, int64_t col_base, const int32_t *col_off, const uint32_t *col_data,
                  const uint8_t *skip_mask, const int32_t *row_cmf, int64_t row_base,
-                 int64_t nrows, uint8_t *kind, int32_t *own, int32_t *pivot, int32_t *res_list,
+                 int64_t nrows, uint8_t *kind, uint64_t *own, int32_t *pivot, int32_t *res_list,
                  int32_t *res_count, void *stream);
 
 /* ---- stage (e): residual reduction ------------------------------------ */
@@ -167,15 +167,19 @@ size_t fph_reduce_scratch_bytes(int64_t K);
 /*
  * Z/2 reduction of the residual columns listed by fph_apparent.  On return
  * pivot[c] holds each residual column's final pivot row (global rank) or -1
- * (reduced to zero: essential).  Reduced columns are written to `pool`
- * (pool_cap 32-bit words).  status (device int32[3]): [0] = 0 ok,
- * 1 pool too small (retry with a larger pool), 2 round cap reached,
- * 3 inconsistent input; [1] = rounds; [2] = column additions performed.
+ * (reduced to zero: essential), and own[r - row_base] = (owner + 1) << 32 |
+ * pool offset for every pivot row.  Columns are written to `pool`
+ * (pool_cap 32-bit words, as [length, rows...]; the buffer must extend 64
+ * words past pool_cap).  status (device int32[5]): [0] = 0 ok, 1 pool too
+ * small (retry with a larger pool), 2 round cap reached, 3 inconsistent
+ * input; [1] = rounds; [2] = column additions performed; [3] = longest
+ * reduced column; [4] = additions too long for on-chip memory.
  * Replaces _general_cohomology_reduce_v2 (cohomology_general.py:202-371)
  * with its pool prefill (_numba_utils.py:68-104, reduce_h1_gpu.py:242-303).
  */
 int fph_reduce(int64_t K, const int32_t *col_off, const uint32_t *col_data, int64_t row_base,
-               const uint8_t *kind, int32_t *own, int32_t *pivot, const int32_t *res_list,
+               int64_t nrows, const uint8_t *kind, uint64_t *own, int32_t *pivot,
+               const int32_t *res_list,
                const int32_t *res_count, uint32_t *pool, int64_t pool_cap, void *scratch,
                size_t scratch_bytes, int32_t *status, int max_rounds, void *stream);
 
