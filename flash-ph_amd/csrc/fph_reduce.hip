@@ -372,41 +372,51 @@ __device__ int reduce_claim(const RedArgs& a, int c, unsigned* arena, int* nxor,
     }
 }
 
+// Phase B: one workgroup per segment of columns (a point cloud of a batch;
+// the whole complex otherwise).  Segments never share rows, so they reduce
+// independently.
 __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_rounds(
     RedArgs a, const unsigned* __restrict__ order, const int* __restrict__ res_count,
-    int* __restrict__ wl0, int* __restrict__ wl1, int max_rounds) {
+    const int* __restrict__ seg_cols, int* __restrict__ wl0, int* __restrict__ wl1,
+    int max_rounds) {
     extern __shared__ unsigned smem[];
-    __shared__ int s_head, s_cnt, s_next_cnt, s_abort;
+    __shared__ int s_head, s_cnt, s_next_cnt, s_abort, s_lo;
     const int tid = threadIdx.x;
     const int wid = tid / FPH_WAVE;
     unsigned* arena = smem + wid * kArenaB;
+    const int c0 = seg_cols ? seg_cols[blockIdx.x] : 0;
+    const int c1 = seg_cols ? seg_cols[blockIdx.x + 1] : (int)a.K;
     if (tid == 0) {
+        // order[0..R) holds the residual columns in descending order; this
+        // segment's columns [c0, c1) form the run order[lo..hi).
+        const int R = *res_count;
+        int lo = 0, hi = R;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if ((int)order[mid] >= c1) lo = mid + 1;
+            else hi = mid;
+        }
+        int lo2 = lo, hi2 = R;
+        while (lo2 < hi2) {
+            const int mid = (lo2 + hi2) >> 1;
+            if ((int)order[mid] >= c0) lo2 = mid + 1;
+            else hi2 = mid;
+        }
+        s_lo = lo;
+        s_cnt = lo2 - lo;
         s_abort = a.status[0];
-        s_cnt = *res_count;
-    }
-    // Phases A and the apparent publication ran on every XCD; this
-    // workgroup's L2 has none of their output.  Touch one word per 128-byte
-    // line of everything the rounds read so the dependent loads below hit L2.
-    {
-        unsigned acc = 0;
-        const long long used = (long long)*a.pool_ptr;
-        const long long step = (long long)blockDim.x * 32;
-        for (long long i = (long long)tid * 32; i < used; i += step) acc ^= a.pool[i];
-        const unsigned* ow = (const unsigned*)a.own;
-        for (long long i = (long long)tid * 32; i < 2 * a.nrows; i += step) acc ^= ow[i];
-        for (long long i = (long long)tid * 32; i < a.K; i += step)
-            acc ^= a.desc[i] ^ (unsigned)a.pivot[i] ^ (unsigned)a.off[i];
-        for (long long i = (long long)tid * 128; i < a.K; i += step * 4) acc ^= a.kind[i];
-        asm volatile("" ::"v"(acc));
     }
     __syncthreads();
-    if (s_abort) return;
+    if (s_abort || s_cnt == 0) return;
     int rounds = 0, nxor = 0;
     Alloc al = {0, 0, 4096};
-    // Round 0 walks every residual column in descending rank (the order the
-    // reference processes them); later rounds the columns displaced before.
-    const int* wl = (const int*)order;
-    int* wn = wl0;
+    // Round 0 walks every residual column of the segment in descending rank
+    // (the order the reference processes them); later rounds the columns
+    // displaced before.  Worklists live in the segment's own column range.
+    const int* wl = (const int*)order + s_lo;
+    int* w0 = wl0 + c0;
+    int* w1 = wl1 + c0;
+    int* wn = w0;
     while (true) {
         const int n = s_cnt;
         if (n == 0 || s_abort) break;
@@ -437,10 +447,10 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_rounds(
         if (tid == 0) s_cnt = s_next_cnt;
         __syncthreads();
         wl = wn;
-        wn = (wn == wl0) ? wl1 : wl0;
+        wn = (wn == w0) ? w1 : w0;
         rounds++;
     }
-    if (tid == 0) a.status[1] = rounds;
+    if (tid == 0) atomicMax(&a.status[1], rounds);
     if (fph_lane() == 0 && nxor) atomicAdd(&a.status[2], nxor);
 }
 
@@ -485,13 +495,14 @@ extern "C" size_t fph_reduce_scratch_bytes(int64_t K) {
 }
 
 extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col_data,
-                          int64_t row_base, int64_t nrows, const uint8_t* kind, uint64_t* own,
-                          int32_t* pivot,
+                          int64_t row_base, int64_t nrows, const int32_t* seg_cols,
+                          int64_t nseg, const uint8_t* kind, uint64_t* own, int32_t* pivot,
                           const int32_t* res_list, const int32_t* res_count, uint32_t* pool,
                           int64_t pool_cap, void* scratch, size_t scratch_bytes,
                           int32_t* status, int max_rounds, void* stream) {
     hipStream_t s = (hipStream_t)stream;
-    if (K < 0 || nrows < 0 || pool_cap < 0 || !scratch || !status) return FPH_ERR_ARG;
+    if (K < 0 || nrows < 0 || pool_cap < 0 || !scratch || !status || nseg < 1) return FPH_ERR_ARG;
+    if (nseg > 1 && !seg_cols) return FPH_ERR_ARG;
     if (scratch_bytes < fph_reduce_scratch_bytes(K)) return FPH_ERR_ARG;
     const size_t k4 = align_up(4 * (size_t)std::max<int64_t>(K, 1));
     char* w = (char*)scratch;
@@ -530,8 +541,8 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
     const size_t lds_b = kPhaseBWaves * kArenaB * 4;
     FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_reduce_rounds,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
-    k_reduce_rounds<<<1, kPhaseBWaves * FPH_WAVE, lds_b, s>>>(a, order, res_count, wl0, wl1,
-                                                               max_rounds);
+    k_reduce_rounds<<<(unsigned)nseg, kPhaseBWaves * FPH_WAVE, lds_b, s>>>(
+        a, order, res_count, seg_cols, wl0, wl1, max_rounds);
     FPH_LAUNCH_CHECK();
     return FPH_OK;
 }

@@ -25,6 +25,7 @@ _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
 _I32 = ctypes.c_int
 _F32 = ctypes.c_float
+_F64 = ctypes.c_double
 _SZ = ctypes.c_size_t
 _PSZ = ctypes.POINTER(ctypes.c_size_t)
 
@@ -44,10 +45,10 @@ SIGNATURES = {
                                _PSZ, _P]),
     "fph_apparent": (_I32, [_I64, _I64, _P, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P]),
     "fph_reduce_scratch_bytes": (_SZ, [_I64]),
-    "fph_reduce": (_I32, [_I64, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _I64, _P, _SZ, _P,
-                          _I32, _P]),
-    "fph_diagram": (_I32, [_I64, _P, _P, _P, _P, _P, _I64, _P, _P, _I64, _I32, _P, _P, _P,
-                           _PSZ, _P]),
+    "fph_reduce": (_I32, [_I64, _P, _P, _I64, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _P,
+                          _SZ, _P, _I32, _P]),
+    "fph_diagram": (_I32, [_I64, _P, _P, _P, _P, _P, _I64, _P, _P, _I64, _I32, _P, _P, _P, _I64,
+                           _P, _P, _P, _PSZ, _P]),
     "fph_clear_pivots": (_I32, [_I64, _P, _P, _I64, _I64, _P, _P]),
     "fph_tet_count": (_I32, [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _PSZ, _P]),
     "fph_tet_emit": (_I32, [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
@@ -56,6 +57,11 @@ SIGNATURES = {
                                      _P]),
     "fph_cofacets_h2": (_I32, [_I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                _I64, _P, _P, _PSZ, _P]),
+    "fph_knn_threshold_batch": (_I32, [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I32, _F64, _I32,
+                                       _P, _P, _P, _P, _PSZ, _P]),
+    "fph_edge_screen_batch": (_I32, [_P, _P, _P, _P, _P, _P, _I64, _I64, _P, _P, _I64, _P, _P]),
+    "fph_edge_sort_batch": (_I32, [_P, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _PSZ, _P]),
+    "fph_h0_diagram_batch": (_I32, [_P, _P, _I64, _P, _P, _I64, _I64, _P, _P, _PSZ, _P]),
 }
 
 _lib = None

@@ -123,6 +123,9 @@ class Complex:
     kind2: torch.Tensor | None = None
     pivot2: torch.Tensor | None = None
     h2: torch.Tensor | None = None
+    # batches: per-cloud edge ranges (device, C+1) and vertex offsets (host)
+    edge_off: torch.Tensor | None = None
+    vbase_host: list | None = None
 
 
 def build_edges(pts: torch.Tensor, max_edge_length: float) -> Complex:
@@ -205,12 +208,27 @@ def build_h1(cx: Complex, max_triangles: int = MAX_TRIANGLES) -> None:
             (ptr(cx.edge_i), ptr(cx.edge_j), E, ptr(cx.adj_ptr), ptr(cx.adj_idx),
              ptr(cx.adj_rank), ptr(cx.mst_mask), ptr(cx.tri_off), ptr(cx.cof_off), ptr(totals)),
             (s,), dev)
-    T, C = (int(x) for x in totals.tolist())
+    segs = None
+    if cx.edge_off is None:
+        T, C = (int(x) for x in totals.tolist())
+        tmax = T
+    else:
+        # batch: triangles of cloud c = those whose largest edge lies in its
+        # edge range; the limit applies per cloud, as for one cloud.
+        eo = cx.edge_off.long()
+        tc = cx.tri_off[eo[1:]] - cx.tri_off[eo[:-1]]
+        host = torch.cat([totals, tc.long(), eo]).tolist()
+        T, C = host[0], host[1]
+        nc = len(cx.vbase_host) - 1
+        tcs = host[2:2 + nc]
+        tmax = max(tcs) if tcs else 0
+        segs = {"cols": cx.edge_off, "edges": cx.edge_off,
+                "ess_asc": (tc == 0).to(torch.uint8), "cols_host": host[2 + nc:]}
     if not int(cx.info["converged"].item()):
         raise _lib.FlashPHError("Boruvka did not converge")
-    if T > max_triangles:
+    if tmax > max_triangles:
         raise ValueError(
-            f"Triangle count {T} exceeds max_triangles={max_triangles}. "
+            f"Triangle count {tmax} exceeds max_triangles={max_triangles}. "
             f"Reduce max_edge_length or increase max_triangles.")
     if E + T >= 2**31 - 1:
         raise ValueError("edge + triangle count exceeds the int32 rank range")
@@ -229,27 +247,36 @@ def build_h1(cx: Complex, max_triangles: int = MAX_TRIANGLES) -> None:
             (ptr(cx.edge_i), ptr(cx.edge_j), E, T, ptr(cx.adj_ptr), ptr(cx.adj_idx),
              ptr(cx.adj_rank), ptr(cx.mst_mask), ptr(cx.tri_off), ptr(cx.tri_w),
              ptr(cx.cof_off), C, ptr(cx.cof_data)), (s,), dev)
-    cx.kind1, cx.pivot1, cx.h1, stats = _reduce_and_diagram(
+    cx.kind1, cx.pivot1, h1, stats = _reduce_and_diagram(
         K=E, col_base=0, off=cx.cof_off, data=cx.cof_data, skip=cx.mst_mask, row_cmf=cx.tri_cmf,
         row_base=E, nrows=T, C=C, col_edge=None, row_edge=cx.tri_e, edge_filt=cx.edge_filt,
-        E=E, ess_ascending=(T == 0), device=dev)
+        E=E, ess_ascending=(T == 0), device=dev, segs=segs)
+    cx.h1 = h1 if segs is not None else h1[0]
     cx.info["h1_stats"] = stats
 
 
 def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, C, col_edge,
-                        row_edge, edge_filt, E, ess_ascending, device):
-    """Apparent pairs, residual reduction and the diagram of one dimension."""
+                        row_edge, edge_filt, E, ess_ascending, device, segs=None):
+    """Apparent pairs, residual reduction and the diagram of one dimension.
+
+    ``segs`` (batches) = dict(cols=, edges=, ess_asc= device tensors,
+    cols_host= list): the columns split into independent point clouds.
+    Returns (kind, pivot, [diagram per segment], stats)."""
     lib = load()
     s = stream_of(device)
+    nseg = 1 if segs is None else len(segs["cols_host"]) - 1
+    seg_cols = None if segs is None else segs["cols"]
+    seg_edges = None if segs is None else segs["edges"]
+    seg_ess_asc = None if segs is None else segs["ess_asc"]
     kind = _empty(torch.uint8, device, max(K, 1))
     own = _empty(torch.int64, device, max(nrows, 1))
     pivot = _empty(torch.int32, device, max(K, 1))
     res_list = _empty(torch.int32, device, max(K, 1))
-    # info32: [0..4] reduction status, [5] residual count, [6] diagram rows
-    info32 = torch.zeros(8, dtype=torch.int32, device=device)
+    # info32: [0..4] reduction status, [5] residual count, [8..8+nseg) rows
+    info32 = torch.zeros(8 + nseg, dtype=torch.int32, device=device)
     status_p = info32.data_ptr()
     rescnt_p = info32.data_ptr() + 20
-    count_p = info32.data_ptr() + 24
+    counts_p = info32.data_ptr() + 32
     scratch_bytes = lib.fph_reduce_scratch_bytes(K)
     # apparent columns in place (C + K words) + line-rounded reduced columns
     pool_words = C + K + 64 + max(_POOL_MIN_WORDS, 64 * K + 4 * C)
@@ -261,14 +288,15 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
         pool = _empty(torch.int32, device, pool_words + 64)  # +64: speculative line reads
         scratch = _empty(torch.uint8, device, scratch_bytes)
         tok = TIMER.start("reduce", device)
-        check(lib.fph_reduce(K, ptr(off), ptr(data), row_base, nrows, ptr(kind), ptr(own),
-                             ptr(pivot),
-                             ptr(res_list), rescnt_p, ptr(pool), pool_words, ptr(scratch),
-                             scratch_bytes, status_p, _MAX_ROUNDS, s), "fph_reduce")
+        check(lib.fph_reduce(K, ptr(off), ptr(data), row_base, nrows, ptr(seg_cols), nseg,
+                             ptr(kind), ptr(own), ptr(pivot), ptr(res_list), rescnt_p,
+                             ptr(pool), pool_words, ptr(scratch), scratch_bytes, status_p,
+                             _MAX_ROUNDS, s), "fph_reduce")
         TIMER.stop(tok)
         call_ws("fph_diagram",
                 (K, ptr(kind), ptr(off), ptr(data), ptr(pivot), ptr(col_edge), row_base,
-                 ptr(row_edge), ptr(edge_filt), E, int(bool(ess_ascending)), ptr(out), count_p),
+                 ptr(row_edge), ptr(edge_filt), E, int(bool(ess_ascending)), ptr(seg_cols),
+                 ptr(seg_edges), ptr(seg_ess_asc), nseg, ptr(out), counts_p),
                 (s,), device)
         st = info32.tolist()
         if st[0] == 0:
@@ -279,7 +307,10 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
         raise _lib.FlashPHError(f"fph_reduce failed with status {st[0]}")
     stats = {"rounds": st[1], "xors": st[2], "longest": st[3], "offchip_xors": st[4],
              "residual": st[5]}
-    return kind, pivot, out[: st[6]], stats
+    counts = st[8:8 + nseg]
+    starts = [0] if segs is None else segs["cols_host"][:-1]
+    dgms = [out[a:a + c] for a, c in zip(starts, counts)]
+    return kind, pivot, dgms, stats
 
 
 def build_h2(cx: Complex) -> None:
@@ -315,10 +346,11 @@ def build_h2(cx: Complex) -> None:
             (T, E, Q, ptr(cx.edge_i), ptr(cx.edge_j), ptr(cx.adj_ptr), ptr(cx.adj_idx),
              ptr(cx.adj_rank), ptr(cx.tri_w), ptr(cx.tri_e), ptr(cx.cleared), ptr(cx.tet_off),
              ptr(cx.tet_key), ptr(cx.cof2_off), C2, ptr(cx.cof2_data)), (s,), dev)
-    cx.kind2, cx.pivot2, cx.h2, stats = _reduce_and_diagram(
+    cx.kind2, cx.pivot2, h2, stats = _reduce_and_diagram(
         K=T, col_base=E, off=cx.cof2_off, data=cx.cof2_data, skip=cx.cleared,
         row_cmf=cx.tet_cmf, row_base=E + T, nrows=Q, C=C2, col_edge=cx.tri_e,
         row_edge=cx.tet_e, edge_filt=cx.edge_filt, E=E, ess_ascending=False, device=dev)
+    cx.h2 = h2[0]
     cx.info["h2_stats"] = stats
 
 
@@ -334,3 +366,148 @@ def compute(pts: torch.Tensor, max_edge_length: float, max_dim: int) -> tuple[li
         build_h2(cx)
         out.append(cx.h2)
     return out, cx
+
+
+# ----------------------------------------------------------------- batches
+
+_KNN_TILE = 256
+_SCREEN_TILE = 64
+# Largest cloud whose local colex index fits the 32-bit key field, so that
+# keys are unique and the per-cloud order is the key order (rips.py:36-49).
+MAX_BATCH_CLOUD = 92682
+
+
+@dataclass
+class BatchLayout:
+    """C clouds concatenated on one device (see fph_edge_screen_batch)."""
+    C: int
+    N: int
+    ns: list
+    ds: list
+    vbase_host: list
+    pts: torch.Tensor       # flat float32, cloud c at foff[c]
+    foff: torch.Tensor      # int64 (C+1)
+    vbase: torch.Tensor     # int64 (C+1)
+    tile_off: torch.Tensor  # int64 (C+1)
+    qtile_off: torch.Tensor  # int64 (C+1)
+    dims: torch.Tensor      # int32 (C)
+    n_tiles: int
+    n_qtiles: int
+    max_d: int
+
+
+def layout_batch(clouds: list, device) -> BatchLayout:
+    """Pack float32 (n_c, d_c) CUDA tensors into one buffer plus offsets."""
+    C = len(clouds)
+    ns = [int(c.shape[0]) for c in clouds]
+    ds = [int(c.shape[1]) for c in clouds]
+    ns_a = np.asarray(ns, dtype=np.int64)
+    ds_a = np.asarray(ds, dtype=np.int64)
+    z = np.zeros(1, dtype=np.int64)
+    vbase = np.concatenate([z, np.cumsum(ns_a)])
+    foff = np.concatenate([z, np.cumsum(ns_a * ds_a)])
+    nt = (ns_a + _SCREEN_TILE - 1) // _SCREEN_TILE
+    tile_off = np.concatenate([z, np.cumsum(nt * (nt + 1) // 2)])
+    qtile_off = np.concatenate([z, np.cumsum((ns_a + _KNN_TILE - 1) // _KNN_TILE)])
+    host = np.concatenate([foff, vbase, tile_off, qtile_off])
+    dev_i64 = torch.from_numpy(host).to(device, non_blocking=False)
+    parts = [c.reshape(-1) for c in clouds if c.numel() > 0]
+    pts = torch.cat(parts) if parts else torch.zeros(1, dtype=torch.float32, device=device)
+    k = C + 1
+    return BatchLayout(
+        C=C, N=int(vbase[-1]), ns=ns, ds=ds, vbase_host=vbase.tolist(), pts=pts,
+        foff=dev_i64[0:k], vbase=dev_i64[k:2 * k], tile_off=dev_i64[2 * k:3 * k],
+        qtile_off=dev_i64[3 * k:4 * k],
+        dims=torch.from_numpy(ds_a.astype(np.int32)).to(device),
+        n_tiles=int(tile_off[-1]), n_qtiles=int(qtile_off[-1]), max_d=max(ds) if ds else 1)
+
+
+def knn_threshold_batch(L: BatchLayout, k: int, percentile: float,
+                        want_kth: bool = False):
+    """Per-cloud auto_threshold on the device: (t, thr_sq[, kth]) tensors."""
+    dev = L.pts.device
+    t = _empty(torch.float32, dev, L.C)
+    thr_sq = _empty(torch.float32, dev, L.C)
+    kth = _empty(torch.float32, dev, max(L.N, 1)) if want_kth else None
+    call_ws("fph_knn_threshold_batch",
+            (ptr(L.pts), ptr(L.foff), ptr(L.dims), ptr(L.vbase), ptr(L.qtile_off), L.n_qtiles,
+             L.C, L.N, int(k), float(percentile), max(1, L.max_d), ptr(kth), ptr(t),
+             ptr(thr_sq)), (stream_of(dev),), dev)
+    return (t, thr_sq, kth) if want_kth else (t, thr_sq)
+
+
+def build_edges_batch(L: BatchLayout, thr_sq: torch.Tensor) -> Complex:
+    """Stage (a) for a batch: per-cloud screening and filtration order, one
+    CSR over all clouds (global vertex ids)."""
+    lib = load()
+    dev = L.pts.device
+    s = stream_of(dev)
+    N = L.N
+    cx = Complex(n=N, device=dev)
+    cx.vbase_host = L.vbase_host
+    cap = max(_EDGE_CAP_MIN, _EDGE_CAP_PER_POINT * N)
+    cnt = _empty(torch.int64, dev, 1)
+    while True:
+        keys = _empty(torch.int64, dev, cap)
+        vals = _empty(torch.int64, dev, cap)
+        tok = TIMER.start("edge_screen", dev)
+        check(lib.fph_edge_screen_batch(ptr(L.pts), ptr(L.foff), ptr(L.dims), ptr(L.vbase),
+                                        ptr(thr_sq), ptr(L.tile_off), L.n_tiles, L.C,
+                                        ptr(keys), ptr(vals), cap, ptr(cnt), s),
+              "fph_edge_screen_batch")
+        TIMER.stop(tok)
+        E = int(cnt.item())
+        if E <= cap:
+            break
+        cap = E
+    assert E < 2**31, f"Edge count {E} exceeds int32 range; reduce max_edge_length"
+    cx.E = E
+    cx.edge_i = _empty(torch.int32, dev, max(E, 1))
+    cx.edge_j = _empty(torch.int32, dev, max(E, 1))
+    cx.edge_dsq = _empty(torch.float32, dev, max(E, 1))
+    cx.edge_filt = _empty(torch.float32, dev, max(E, 1))
+    cx.edge_off = _empty(torch.int32, dev, L.C + 1)
+    call_ws("fph_edge_sort_batch",
+            (ptr(keys), ptr(vals), E, ptr(L.vbase), L.C, ptr(cx.edge_i), ptr(cx.edge_j),
+             ptr(cx.edge_dsq), ptr(cx.edge_filt), ptr(cx.edge_off)), (s,), dev)
+    del keys, vals
+    cx.adj_ptr = _empty(torch.int32, dev, N + 1)
+    cx.adj_idx = _empty(torch.int32, dev, max(2 * E, 1))
+    cx.adj_rank = _empty(torch.int32, dev, max(2 * E, 1))
+    cx.adj_dsq = _empty(torch.float32, dev, max(2 * E, 1))
+    call_ws("fph_vertex_csr",
+            (ptr(cx.edge_i), ptr(cx.edge_j), ptr(cx.edge_dsq), E, N, ptr(cx.adj_ptr),
+             ptr(cx.adj_idx), ptr(cx.adj_rank), ptr(cx.adj_dsq)), (s,), dev)
+    return cx
+
+
+def build_mst_batch(L: BatchLayout, cx: Complex) -> None:
+    """Stage (b) for a batch: one forest over the union, H0 per cloud."""
+    lib = load()
+    dev, N, E = cx.device, cx.n, cx.E
+    s = stream_of(dev)
+    cx.mst_mask = _empty(torch.uint8, dev, max(E, 1))
+    conv = _empty(torch.int32, dev, 1)
+    scratch = _lib.WORKSPACE.get(lib.fph_boruvka_scratch_bytes(N), dev)
+    check(lib.fph_boruvka_mst(ptr(cx.edge_i), ptr(cx.edge_j), E, N, ptr(cx.mst_mask), None,
+                              ptr(conv), ptr(scratch), scratch.numel(), s), "fph_boruvka_mst")
+    cx.h0 = _empty(torch.float32, dev, max(N, 1), 2)
+    call_ws("fph_h0_diagram_batch",
+            (ptr(cx.mst_mask), ptr(cx.edge_dsq), E, ptr(cx.edge_off), ptr(L.vbase), L.C, N,
+             ptr(cx.h0)), (s,), dev)
+    cx.info["converged"] = conv
+
+
+def compute_batch(L: BatchLayout, thr_sq: torch.Tensor, max_dim: int,
+                  max_triangles: int = MAX_TRIANGLES) -> tuple[list, Complex]:
+    """Diagrams [[H0, H1?] per cloud] of a packed batch (max_dim <= 1)."""
+    cx = build_edges_batch(L, thr_sq)
+    build_mst_batch(L, cx)
+    vb = L.vbase_host
+    h0s = [cx.h0[vb[c]:vb[c + 1]] for c in range(L.C)]
+    if max_dim == 0:
+        if not int(cx.info["converged"].item()):
+            raise _lib.FlashPHError("Boruvka did not converge")
+        return [[h] for h in h0s], cx
+    build_h1(cx, max_triangles)
+    return [[h0, h1] for h0, h1 in zip(h0s, cx.h1)], cx

@@ -174,12 +174,15 @@ size_t fph_reduce_scratch_bytes(int64_t K);
  * small (retry with a larger pool), 2 round cap reached, 3 inconsistent
  * input; [1] = rounds; [2] = column additions performed; [3] = longest
  * reduced column; [4] = additions too long for on-chip memory.
+ * seg_cols (nseg+1 column offsets, device, may be NULL when nseg == 1)
+ * splits the columns into independent segments (the point clouds of a
+ * batch: no row is shared between segments); one workgroup reduces each.
  * Replaces _general_cohomology_reduce_v2 (cohomology_general.py:202-371)
  * with its pool prefill (_numba_utils.py:68-104, reduce_h1_gpu.py:242-303).
  */
 int fph_reduce(int64_t K, const int32_t *col_off, const uint32_t *col_data, int64_t row_base,
-               int64_t nrows, const uint8_t *kind, uint64_t *own, int32_t *pivot,
-               const int32_t *res_list,
+               int64_t nrows, const int32_t *seg_cols, int64_t nseg, const uint8_t *kind,
+               uint64_t *own, int32_t *pivot, const int32_t *res_list,
                const int32_t *res_count, uint32_t *pool, int64_t pool_cap, void *scratch,
                size_t scratch_bytes, int32_t *status, int max_rounds, void *stream);
 
@@ -191,15 +194,19 @@ int fph_reduce(int64_t K, const int32_t *col_off, const uint32_t *col_data, int6
  * ascending when ess_ascending), then stably by birth value; only pairs with
  * death > birth.  Birth of column c = edge_filt[col_edge ? col_edge[c] : c];
  * death of row r = edge_filt[row_edge[r - row_base]].  out: K x 2 FP32
- * capacity; *count (device int32) = rows written.
+ * capacity.  Segments (seg_cols: nseg+1 column offsets, seg_edges: nseg+1
+ * edge offsets, seg_ess_asc: per-segment flag; all NULL for one segment
+ * spanning [0, K) and edges [0, E)) are the clouds of a batch: segment s
+ * writes counts[s] rows at out[seg_cols[s] ...].
  * Replaces reduce_h1_gpu.py:495-502, 574-604 (and persistence.py:202-205
  * for H2).
  */
 int fph_diagram(int64_t K, const uint8_t *kind, const int32_t *col_off,
                 const uint32_t *col_data, const int32_t *pivot, const int32_t *col_edge,
                 int64_t row_base, const int32_t *row_edge, const float *edge_filt, int64_t E,
-                int ess_ascending, float *out, int32_t *count, void *ws, size_t *ws_bytes,
-                void *stream);
+                int ess_ascending, const int32_t *seg_cols, const int32_t *seg_edges,
+                const uint8_t *seg_ess_asc, int64_t nseg, float *out, int32_t *counts,
+                void *ws, size_t *ws_bytes, void *stream);
 
 /* ---- H2: clearing cascade, tetrahedra, cofacet columns ---------------- */
 
@@ -257,6 +264,70 @@ int fph_cofacets_h2(int64_t T, int64_t E, int64_t Q, const int32_t *edge_i,
                     const uint8_t *cleared, const int32_t *tet_off, const uint32_t *tet_key,
                     const int32_t *cof_off, int64_t C, uint32_t *cof_data, void *ws,
                     size_t *ws_bytes, void *stream);
+
+/* ---- batches of point clouds ------------------------------------------ */
+/*
+ * A batch is C clouds concatenated: cloud c has n_c = vbase[c+1] - vbase[c]
+ * points of dimension dims[c], stored row-major at pts + foff[c] (floats).
+ * Vertices get global ids vbase[c] + local id.  A batch is one Vietoris-Rips
+ * complex (no edges between clouds); after fph_edge_sort_batch the
+ * single-cloud entry points (CSR, MST, triangles, cofacets, apparent pairs)
+ * run on it unchanged, and fph_reduce / fph_diagram take the per-cloud
+ * column segments.  The reference runs clouds one by one through the same
+ * functions; these entry points have no other counterpart.
+ */
+
+#define FPH_KNN_MAX_K 128
+
+/*
+ * Per-cloud threshold of auto_threshold (threshold.py:20-78): the k-th
+ * nearest-neighbour distance of every point (k clamped to [1, n_c - 1]),
+ * then the linear-interpolation quantile at percentile/100 (clamped to
+ * [0, 100]) in FP32 as torch.quantile; t_out[c] = 0 for n_c <= 1.
+ * Distances are the filtration's FP32 FMA-chain values, correctly rounded
+ * sqrt.  Query tiles of 256 points: qtile_off (C+1, exclusive scan of
+ * ceil(n_c / 256)); max_dim_host = max_c dims[c].  Optional outputs:
+ * kth_dist (N floats), thr_sq_out[c] = fp32(double(t)^2).  k <= 128.
+ * Replaces _tiled_knn (threshold.py:133-176) and the quantile at :77.
+ */
+int fph_knn_threshold_batch(const float *pts, const int64_t *foff, const int32_t *dims,
+                            const int64_t *vbase, const int64_t *qtile_off, int64_t n_qtiles,
+                            int64_t C, int64_t N, int k, double percentile, int max_dim_host,
+                            float *kth_dist, float *t_out, float *thr_sq_out, void *ws,
+                            size_t *ws_bytes, void *stream);
+
+/*
+ * Screen every cloud against its own FP32 threshold thr_sq[c] (strict <),
+ * with the FMA chain of fph_edge_screen.  Tiles of 64 x 64 points:
+ * tile_off (C+1) = exclusive scan of nt_c (nt_c + 1) / 2 with
+ * nt_c = ceil(n_c / 64).  keys = bits(dsq) << 32 | local colex, vals =
+ * global j << 32 | global i; *count_dev = total (may exceed cap: rerun).
+ * Replaces direct_edge_enumerate (kernels/edge_kernel.py:152-224) per cloud.
+ */
+int fph_edge_screen_batch(const float *pts, const int64_t *foff, const int32_t *dims,
+                          const int64_t *vbase, const float *thr_sq, const int64_t *tile_off,
+                          int64_t n_tiles, int64_t C, uint64_t *keys, uint64_t *vals,
+                          int64_t cap, uint64_t *count_dev, void *stream);
+
+/*
+ * Edges grouped by cloud, each cloud in its own filtration order (the
+ * reference's stable key sort, rips.py:188-202, per cloud; requires every
+ * n_c <= 92682 so that keys are unique).  edge_off (C+1): the edge range of
+ * each cloud.  edge_filt = sqrt(dsq) correctly rounded.
+ */
+int fph_edge_sort_batch(const uint64_t *keys_in, const uint64_t *vals_in, int64_t E,
+                        const int64_t *vbase, int64_t C, int32_t *edge_i, int32_t *edge_j,
+                        float *edge_dsq, float *edge_filt, int32_t *edge_off, void *ws,
+                        size_t *ws_bytes, void *stream);
+
+/*
+ * H0 of every cloud at rows vbase[c] ... vbase[c+1] - 1 of h0 (N x 2):
+ * forest edges of the cloud in filtration order as (0, sqrt(dsq)), then
+ * (0, inf) per component (persistence.py:169-192, per cloud).
+ */
+int fph_h0_diagram_batch(const uint8_t *mst_mask, const float *edge_dsq, int64_t E,
+                         const int32_t *edge_off, const int64_t *vbase, int64_t C, int64_t N,
+                         float *h0, void *ws, size_t *ws_bytes, void *stream);
 
 #ifdef __cplusplus
 }
