@@ -4,9 +4,15 @@ max_edge_length=1.6, H0+H1+H2, on one MI355X.
 One step = one full rips_persistence(points, 1.6, max_dim=2) call on a point
 cloud already resident in HBM (edges, sort, CSR, forest, H0, triangles,
 apparent pairs, H1 reduction, clearing, tetrahedra, H2 reduction, diagrams).
-Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+The "batch" object times the batched path on 256 uneven clouds (n
+log-uniform in [200, 5000], d in {4, 10, 50}, unit sphere + N(0, 0.05^2),
+per-cloud threshold = 95th percentile of 20-NN distances, H0+H1): one
+rips_persistence_batch call per step, beside the plain per-cloud loop
+(auto_threshold + rips_persistence per cloud), and checks that both give
+bit-identical diagrams.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
 """
 from __future__ import annotations
 
@@ -53,12 +59,65 @@ def cpu_baseline(pts, budget_s: float = 12.0):
     }
 
 
+def bench_batch(steps: int) -> dict:
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cases
+    from flash_ph import auto_threshold, rips_persistence, rips_persistence_batch
+    from flash_ph import _pipeline
+
+    clouds_np = cases.batch_uneven(256, seed=0)
+    clouds = [torch.from_numpy(c).cuda() for c in clouds_np]
+    torch.cuda.synchronize()
+    dg_b = rips_persistence_batch(clouds)                      # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dg_b = rips_persistence_batch(clouds)
+    torch.cuda.synchronize()
+    ms_batch = 1000.0 * (time.perf_counter() - t0) / steps
+
+    _pipeline.TIMER.enable(True)
+    rips_persistence_batch(clouds)
+    summ = _pipeline.TIMER.summary()
+    _pipeline.TIMER.enable(False)
+
+    def plain():
+        return [rips_persistence(c, auto_threshold(c), max_dim=1) for c in clouds]
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dg_p = plain()
+    torch.cuda.synchronize()
+    ms_plain = 1000.0 * (time.perf_counter() - t0)
+
+    same = all(torch.equal(a.view(torch.int32), b.view(torch.int32))
+               for pa, pb in zip(dg_b, dg_p) for a, b in zip(pa, pb))
+    C = len(clouds)
+    return {
+        "workload": "256 clouds, n log-uniform [200,5000], d in {4,10,50}, unit sphere "
+                    "+ N(0,0.05^2), seed 0; per-cloud auto_threshold k=20 p=95; H0+H1",
+        "sum_n": int(sum(c.shape[0] for c in clouds_np)),
+        "steps": steps,
+        "ms_per_batch": round(ms_batch, 3),
+        "clouds_per_s": round(C / (ms_batch * 1e-3), 1),
+        "plain_ms": round(ms_plain, 3),
+        "plain_clouds_per_s": round(C / (ms_plain * 1e-3), 1),
+        "speedup_vs_plain": round(ms_plain / ms_batch, 2),
+        "bit_exact_vs_plain": bool(same),
+        "stage_ms": {"edge_screen": round(summ.get("edge_screen", (0, 0.0))[1], 4),
+                     "reduce_h1": round(summ.get("reduce", (0, 0.0))[1], 4)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -122,6 +181,8 @@ def main():
         },
         "stage_ms": {"edge_screen": round(screen_ms, 4), "reduce_h1_h2_mean": round(reduce_ms, 4)},
     }
+    if not args.no_batch:
+        out["batch"] = bench_batch(args.batch_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pts_np)
     print(json.dumps(out), flush=True)

@@ -323,6 +323,107 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(
     if (active) kth[vbase[c] + q] = n > 1 ? r : 0.0f;
 }
 
+// Largest squared distance from each point to the rest of its cloud
+// (threshold.py:179-200 takes the minimum of these over the cloud); the
+// cloud's minimum goes to rmin[c] as FP32 bits (non-negative floats order
+// as their bit patterns, so an unsigned atomicMin is exact).
+template <int DP>
+__device__ __forceinline__ float far_scan(const float* __restrict__ P, int n, int d, int q,
+                                          bool active, float* cand) {
+    float xq[DP];
+#pragma unroll
+    for (int x = 0; x < DP; x++) xq[x] = (active && x < d) ? P[(long long)q * d + x] : 0.0f;
+    float best = 0.0f;
+    for (int j0 = 0; j0 < n; j0 += kKnnCand) {
+        const int nc = min(kKnnCand, n - j0);
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < nc * DP; idx += kKnnThreads) {
+            const int jj = idx / DP, x = idx % DP;
+            cand[idx] = x < d ? P[(long long)(j0 + jj) * d + x] : 0.0f;
+        }
+        __syncthreads();
+        for (int jj = 0; jj < nc; jj++) {
+            const float* y = cand + jj * DP;
+            float acc = 0.0f;
+#pragma unroll
+            for (int x = 0; x < DP; x++) {
+                const float diff = xq[x] - y[x];
+                acc = __builtin_fmaf(diff, diff, acc);
+            }
+            best = fmaxf(best, acc);
+        }
+    }
+    return best;
+}
+
+__device__ __forceinline__ float far_scan_generic(const float* __restrict__ P, int n, int d,
+                                                  int q, bool active, float* cand) {
+    float best = 0.0f;
+    const int per = max(1, kKnnLdsFloats / d);
+    const float* xq = P + (long long)(active ? q : 0) * d;
+    for (int j0 = 0; j0 < n; j0 += per) {
+        const int nc = min(per, n - j0);
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < nc * d; idx += kKnnThreads)
+            cand[idx] = P[(long long)j0 * d + idx];
+        __syncthreads();
+        for (int jj = 0; jj < nc; jj++) {
+            const float* y = cand + jj * d;
+            float acc = 0.0f;
+            for (int x = 0; x < d; x++) {
+                const float diff = xq[x] - y[x];
+                acc = __builtin_fmaf(diff, diff, acc);
+            }
+            best = fmaxf(best, acc);
+        }
+    }
+    return best;
+}
+
+__global__ __launch_bounds__(kKnnThreads) void k_enclosing_batch(
+    const float* __restrict__ pts, const long long* __restrict__ foff,
+    const int* __restrict__ dims, const long long* __restrict__ vbase, int C,
+    const long long* __restrict__ qtile_off, unsigned* __restrict__ rmin) {
+    extern __shared__ float cand[];
+    const long long t = blockIdx.x;
+    const int c = find_seg(qtile_off, C, t);
+    const int n = (int)(vbase[c + 1] - vbase[c]);
+    const int d = dims[c];
+    const float* P = pts + foff[c];
+    const int q = (int)(t - qtile_off[c]) * kKnnThreads + threadIdx.x;
+    const bool active = q < n;
+    float r;
+    if (d <= 4) r = far_scan<4>(P, n, d, q, active, cand);
+    else if (d <= 8) r = far_scan<8>(P, n, d, q, active, cand);
+    else if (d <= 16) r = far_scan<16>(P, n, d, q, active, cand);
+    else if (d <= 32) r = far_scan<32>(P, n, d, q, active, cand);
+    else if (d <= 64) r = far_scan<64>(P, n, d, q, active, cand);
+    else r = far_scan_generic(P, n, d, q, active, cand);
+    unsigned b;
+    memcpy(&b, &r, 4);
+    b = active ? b : 0xFFFFFFFFu;
+    // wave minimum, then one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) b = min(b, (unsigned)__shfl_xor((int)b, o));
+    if (fph_lane() == 0 && b != 0xFFFFFFFFu) atomicMin(&rmin[c], b);
+}
+
+__global__ void k_fill_u32(unsigned* p, long long n, unsigned v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// r_out[c] = sqrt of the cloud's minimum (0 for n <= 1, as threshold.py:107).
+__global__ void k_enclosing_finish(const unsigned* __restrict__ rmin,
+                                   const long long* __restrict__ vbase, int C,
+                                   float* __restrict__ r_out) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    float v;
+    const unsigned b = rmin[c];
+    memcpy(&v, &b, 4);
+    r_out[c] = (vbase[c + 1] - vbase[c] > 1) ? fph_sqrt_rn(v) : 0.0f;
+}
+
 __global__ void k_sqrt_bits(const float* __restrict__ dsq, long long N, unsigned* __restrict__ o,
                             float* __restrict__ dist) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -417,6 +518,33 @@ extern "C" int fph_knn_threshold_batch(const float* pts, const int64_t* foff, co
     const float qf = (float)(std::max(0.0, std::min(100.0, percentile)) / 100.0);
     k_quantile<<<fph_blocks(C, 256), 256, 0, s>>>(sorted, (const long long*)vbase, (int)C, qf,
                                                  t_out, thr_sq_out);
+    FPH_LAUNCH_CHECK();
+    return FPH_OK;
+}
+
+extern "C" int fph_enclosing_radius_batch(const float* pts, const int64_t* foff,
+                                          const int32_t* dims, const int64_t* vbase,
+                                          const int64_t* qtile_off, int64_t n_qtiles, int64_t C,
+                                          int max_dim_host, float* r_out, void* ws,
+                                          size_t* ws_bytes, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (C < 1 || max_dim_host < 1 || !ws_bytes || n_qtiles < 0 || n_qtiles > 0x7FFFFFFFLL)
+        return FPH_ERR_ARG;
+    const size_t need = align_up(4 * (size_t)C);
+    if (!ws) { *ws_bytes = need; return FPH_OK; }
+    if (*ws_bytes < need) return FPH_ERR_ARG;
+    unsigned* rmin = (unsigned*)ws;
+    k_fill_u32<<<fph_blocks(C, 256), 256, 0, s>>>(rmin, C, 0x7F800000u);  // +inf
+    FPH_LAUNCH_CHECK();
+    if (n_qtiles > 0) {
+        const size_t lds = 4 * (size_t)(max_dim_host > 64 ? kKnnLdsFloats : kKnnCand * 64);
+        k_enclosing_batch<<<(unsigned)n_qtiles, kKnnThreads, lds, s>>>(
+            pts, (const long long*)foff, dims, (const long long*)vbase, (int)C,
+            (const long long*)qtile_off, rmin);
+        FPH_LAUNCH_CHECK();
+    }
+    k_enclosing_finish<<<fph_blocks(C, 256), 256, 0, s>>>(rmin, (const long long*)vbase, (int)C,
+                                                         r_out);
     FPH_LAUNCH_CHECK();
     return FPH_OK;
 }

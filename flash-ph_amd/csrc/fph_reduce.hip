@@ -138,6 +138,81 @@ __device__ int wave_xor(const unsigned* A, int na, const unsigned* B, int nb, un
     return na + nb - 2 * m;
 }
 
+// C = A xor B for long lists: the merged order is cut into 64 contiguous
+// pieces along the merge path (ties: A first), one per lane; a piece never
+// separates an element of A from its equal in B.  Each lane walks its piece
+// sequentially twice: once to count the survivors (then a wave prefix sum
+// gives its output offset), once to write them.  Sequential reads replace
+// the per-element binary searches of wave_xor, which cost one dependent
+// memory round trip per step when the lists live off chip.
+__device__ int wave_xor_long(const unsigned* A, int na, const unsigned* B, int nb, unsigned* C,
+                             int limit = 0x7FFFFFFF) {
+    const int lane = fph_lane();
+    const int total = na + nb;
+    const int per = (total + FPH_WAVE - 1) / FPH_WAVE;
+    const int k = min(lane * per, total);
+    int lo = max(0, k - nb), hi = min(k, na);
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (A[mid] <= B[k - mid - 1]) lo = mid + 1;
+        else hi = mid;
+    }
+    int i0 = lo;
+    const int j0 = k - lo;
+    if (i0 > 0 && j0 < nb && A[i0 - 1] == B[j0]) i0--;
+    int i1 = __shfl(i0, (lane + 1) & (FPH_WAVE - 1));
+    int j1 = __shfl(j0, (lane + 1) & (FPH_WAVE - 1));
+    if (lane == FPH_WAVE - 1) {
+        i1 = na;
+        j1 = nb;
+    }
+    constexpr unsigned kEnd = 0xFFFFFFFFu;  // above every row rank
+    int cnt = 0;
+    {
+        int ia = i0, ib = j0;
+        unsigned av = ia < i1 ? A[ia] : kEnd, bv = ib < j1 ? B[ib] : kEnd;
+        while (av != kEnd || bv != kEnd) {
+            if (av < bv) {
+                cnt++;
+                av = ++ia < i1 ? A[ia] : kEnd;
+            } else if (bv < av) {
+                cnt++;
+                bv = ++ib < j1 ? B[ib] : kEnd;
+            } else {
+                av = ++ia < i1 ? A[ia] : kEnd;
+                bv = ++ib < j1 ? B[ib] : kEnd;
+            }
+        }
+    }
+    int incl = cnt;
+    for (int o = 1; o < FPH_WAVE; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    const int len = min(__shfl(incl, FPH_WAVE - 1), limit);
+    int at = incl - cnt;
+    if (at < len) {
+        // with a limit, only the first `limit` survivors are kept (a prefix
+        // of the sorted result)
+        int ia = i0, ib = j0;
+        unsigned av = ia < i1 ? A[ia] : kEnd, bv = ib < j1 ? B[ib] : kEnd;
+        while ((av != kEnd || bv != kEnd) && at < len) {
+            if (av < bv) {
+                C[at++] = av;
+                av = ++ia < i1 ? A[ia] : kEnd;
+            } else if (bv < av) {
+                C[at++] = bv;
+                bv = ++ib < j1 ? B[ib] : kEnd;
+            } else {
+                av = ++ia < i1 ? A[ia] : kEnd;
+                bv = ++ib < j1 ? B[ib] : kEnd;
+            }
+        }
+    }
+    wave_sync();
+    return len;
+}
+
 __device__ __forceinline__ void wave_copy(const unsigned* src, int n, unsigned* dst) {
     for (int i = fph_lane(); i < n; i += FPH_WAVE) dst[i] = src[i];
     wave_sync();
@@ -153,7 +228,21 @@ struct Alloc {
     unsigned long long cur;
     unsigned long long end;
     unsigned long long chunk;  // words reserved per refill
+    // Two off-chip working buffers per wavefront (pool offsets, capacities
+    // in words), reused by every long XOR of the wavefront; grown by
+    // doubling when a longer result comes along.
+    unsigned long long big_off[2];
+    int big_cap[2];
 };
+
+__device__ __forceinline__ Alloc make_alloc(unsigned long long chunk) {
+    Alloc al;
+    al.cur = al.end = 0;
+    al.chunk = chunk;
+    al.big_off[0] = al.big_off[1] = 0;
+    al.big_cap[0] = al.big_cap[1] = 0;
+    return al;
+}
 
 // Wave-uniform allocation of n words; returns the word offset, or -1 on
 // overflow (error flag raised).
@@ -214,33 +303,54 @@ __device__ __forceinline__ void work_load(Work& w, const unsigned* src, unsigned
     }
 }
 
+// Off-chip working buffer i of the wavefront with room for n words, or
+// nullptr on pool overflow.
+__device__ unsigned* big_buffer(const RedArgs& a, Alloc& al, int i, int n) {
+    if (al.big_cap[i] < n) {
+        int cap = 4096;
+        while (cap < n) cap *= 2;
+        const long long at = pool_alloc(a, al, cap);
+        if (at < 0) return nullptr;
+        al.big_off[i] = (unsigned long long)at;
+        al.big_cap[i] = cap;
+    }
+    return a.pool + al.big_off[i];
+}
+
+// Work::half: 0 / 1 = LDS arena halves, 2 / 3 = the wavefront's off-chip
+// buffers, -1 = read-only memory (pool or CSR).
+//
 // w <- w xor (pooled column at ob), whose pivot equals w's.  False on error.
 template <int H, int O>
 __device__ bool xor_step(Work& w, const unsigned* ob, unsigned* arena, const RedArgs& a,
-                         Alloc& al) {
+                         Alloc& al, int limit = 0x7FFFFFFF) {
     unsigned v;
     const int nb = read_line(ob, v);
     const unsigned p_old = w.ptr[0];
     const int na = w.len;
     unsigned* out;
-    int out_half = -1;
-    if (w.half >= 0 && na + nb <= H) {
+    int out_half;
+    const bool on_chip = (w.half == 0 || w.half == 1) && na + nb <= H;
+    if (on_chip) {
         out_half = 1 - w.half;
         out = arena + out_half * H;
     } else {
         if (fph_lane() == 0) atomicAdd(&a.status[4], 1);
-        const long long at = pool_alloc(a, al, na + nb);
-        if (at < 0) return false;
-        out = a.pool + at;
+        const int bi = w.half == 2 ? 1 : 0;
+        out = big_buffer(a, al, bi, na + nb);
+        if (!out) return false;
+        out_half = 2 + bi;
     }
     const unsigned* bs = ob + 1;
     if (nb <= O) {
         finish_copy(ob, v, nb, arena + 2 * H);
         bs = arena + 2 * H;
     }
-    int len = wave_xor(w.ptr, na, bs, nb, out);
-    if (out_half < 0 && len <= H) {  // bring an off-chip result back on chip
-        const int h = w.half >= 0 ? 1 - w.half : 0;
+    int len;
+    if (on_chip && nb <= O) len = wave_xor(w.ptr, na, bs, nb, out);
+    else len = wave_xor_long(w.ptr, na, bs, nb, out, limit);
+    if (!on_chip && len <= H) {  // bring an off-chip result back on chip
+        const int h = (w.half == 0 || w.half == 1) ? 1 - w.half : 0;
         for (int i = fph_lane(); i < len; i += FPH_WAVE) arena[h * H + i] = out[i];
         wave_sync();
         out = arena + h * H;
@@ -275,7 +385,7 @@ __device__ __forceinline__ void note_len(const RedArgs& a, int len) {
 // Apparent columns are laid out at pool offset off[c] + c as [len, rows...]
 // (no allocation), and own[] records them.  Dynamic allocations start after
 // this region.
-__global__ void k_publish_apparent(RedArgs a, long long K) {
+__global__ void k_publish_apparent(RedArgs a, long long K, int set_own) {
     const int wpb = blockDim.x / FPH_WAVE;
     const long long nw = (long long)gridDim.x * wpb;
     for (long long c = (long long)blockIdx.x * wpb + threadIdx.x / FPH_WAVE; c < K; c += nw) {
@@ -284,8 +394,9 @@ __global__ void k_publish_apparent(RedArgs a, long long K) {
         const long long at = (long long)lo + c;
         if (fph_lane() == 0) {
             a.pool[at] = (unsigned)len;
-            a.own[(long long)a.data[lo] - a.row_base] =
-                ((unsigned long long)(c + 1) << 32) | (unsigned long long)at;
+            if (set_own)
+                a.own[(long long)a.data[lo] - a.row_base] =
+                    ((unsigned long long)(c + 1) << 32) | (unsigned long long)at;
         }
         for (int i = fph_lane(); i < len; i += FPH_WAVE) a.pool[at + 1 + i] = a.data[lo + i];
     }
@@ -300,7 +411,7 @@ __global__ __launch_bounds__(kPhaseAWaves * FPH_WAVE) void k_reduce_chain(
     const int R = *res_count;
     const long long nw = (long long)gridDim.x * kPhaseAWaves;
     int nxor = 0;
-    Alloc al = {0, 0, 128};
+    Alloc al = make_alloc(128);
     for (long long r = (long long)blockIdx.x * kPhaseAWaves + wid; r < R; r += nw) {
         if (a.status[0]) return;
         const int c = res_list[r];
@@ -409,7 +520,7 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_rounds(
     __syncthreads();
     if (s_abort || s_cnt == 0) return;
     int rounds = 0, nxor = 0;
-    Alloc al = {0, 0, 4096};
+    Alloc al = make_alloc(4096);
     // Round 0 walks every residual column of the segment in descending rank
     // (the order the reference processes them); later rounds the columns
     // displaced before.  Worklists live in the segment's own column range.
@@ -454,6 +565,140 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_rounds(
     if (fph_lane() == 0 && nxor) atomicAdd(&a.status[2], nxor);
 }
 
+// ---------------------------------------------------------------------
+// The reference's own semantics, one wavefront per segment, in its order.
+//
+// _general_cohomology_reduce_v2 (cohomology_general.py:202-371) stores
+// reduced columns in a pool of pool_cap entries (reduce_h1_gpu.py:286-289)
+// and drops a column that does not fit: its pivot is still paired, but a
+// later column reaching that pivot stops there instead of adding it.  With
+// drops (and MAX_COL truncation, _numba_utils.py:10, 43-65) the pairs depend
+// on the order of the work, so this kernel follows it exactly: apparent
+// columns enter the store in ascending column order (_numba_utils.py:68-104),
+// then residual columns are reduced one at a time in descending rank order.
+// seg_pool[2 s] = word offset of segment s's store, seg_pool[2 s + 1] = its
+// capacity in reference entries; a stored column takes len + 1 words here.
+constexpr int kSerialMaxCol = 1048576;
+
+__global__ __launch_bounds__(FPH_WAVE) void k_reduce_serial(
+    RedArgs a, const unsigned* __restrict__ order, const int* __restrict__ res_count,
+    const int* __restrict__ seg_cols, const long long* __restrict__ seg_pool) {
+    extern __shared__ unsigned smem[];
+    unsigned* arena = smem;
+    const int lane = fph_lane();
+    const unsigned long long lt = fph_lanemask_lt();
+    const int seg = blockIdx.x;
+    const int c0 = seg_cols ? seg_cols[seg] : 0;
+    const int c1 = seg_cols ? seg_cols[seg + 1] : (int)a.K;
+    const long long base = seg_pool[2 * seg];
+    const long long cap = seg_pool[2 * seg + 1];
+    long long used = 0;    // reference entries in the store
+    long long stored = 0;  // words written at base
+    // 1. apparent columns, ascending: store while pool_ptr + cn <= pool_cap
+    for (int cb = c0; cb < c1; cb += FPH_WAVE) {
+        const int c = cb + lane;
+        const bool app = c < c1 && a.kind[c] == 2;
+        const int cn = app ? a.off[c + 1] - a.off[c] : 0;
+        int incl = cn;
+        for (int o = 1; o < FPH_WAVE; o <<= 1) {
+            const int y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        int p = 0;
+        while (p < FPH_WAVE) {
+            const int excl_p = p == 0 ? 0 : __shfl(incl, p - 1);
+            const bool fits = used + (incl - excl_p) <= cap;
+            const unsigned long long bad = __ballot(lane >= p && cn > 0 && !fits);
+            const int f = bad ? __builtin_ctzll(bad) : FPH_WAVE;
+            if (lane >= p && lane < f && cn > 0) {
+                const long long lo = a.off[c];
+                a.own[(long long)a.data[lo] - a.row_base] =
+                    ((unsigned long long)(c + 1) << 32) | (unsigned long long)(lo + c);
+            }
+            if (f > p) used += __shfl(incl, f - 1) - excl_p;
+            p = f + 1;  // column f (if any) is dropped
+        }
+    }
+    (void)lt;
+    wave_sync();
+    // 2. residual columns of the segment, descending rank
+    const int R = *res_count;
+    int lo = 0, hi = R;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int)order[mid] >= c1) lo = mid + 1;
+        else hi = mid;
+    }
+    int lo2 = lo, hi2 = R;
+    while (lo2 < hi2) {
+        const int mid = (lo2 + hi2) >> 1;
+        if ((int)order[mid] >= c0) lo2 = mid + 1;
+        else hi2 = mid;
+    }
+    Alloc al = make_alloc(4096);
+    int nxor = 0, longest = 0;
+    for (int r = lo; r < lo2; r++) {
+        const int c = (int)order[r];
+        const int lo_c = a.off[c];
+        const int n0 = min(a.off[c + 1] - lo_c, kSerialMaxCol);
+        Work w;
+        w.len = n0;
+        if (n0 <= kHalfB) {
+            wave_copy(a.data + lo_c, n0, arena);
+            w.ptr = arena;
+            w.half = 0;
+        } else {
+            w.ptr = a.data + lo_c;
+            w.half = -1;
+        }
+        while (w.len > 0) {
+            const unsigned long long v = a.own[(long long)w.ptr[0] - a.row_base];
+            if (v == kNoOwner) break;
+            if (!xor_step<kHalfB, kStageB>(w, a.pool + (v & 0xFFFFFFFFull), arena, a, al,
+                                           kSerialMaxCol))
+                return;
+            nxor++;
+        }
+        if (w.len == 0) {
+            if (lane == 0) a.pivot[c] = -1;
+            continue;
+        }
+        const unsigned piv = w.ptr[0];
+        if (lane == 0) a.pivot[c] = (int)piv;
+        longest = max(longest, w.len);
+        if (used + w.len <= cap) {
+            const long long at = base + stored;
+            if (lane == 0) a.pool[at] = (unsigned)w.len;
+            wave_copy(w.ptr, w.len, a.pool + at + 1);
+            if (lane == 0)
+                a.own[(long long)piv - a.row_base] =
+                    ((unsigned long long)(c + 1) << 32) | (unsigned long long)at;
+            used += w.len;
+            stored += w.len + 1;
+        }
+        wave_sync();
+    }
+    if (lane == 0) {
+        if (nxor) atomicAdd(&a.status[2], nxor);
+        if (longest) atomicMax(&a.status[3], longest);
+        atomicMax(&a.status[1], 1);
+    }
+}
+
+// status[5] = pool words handed out (in 32-word lines, saturating): what a
+// retry after a pool overflow needs at least.
+__global__ void k_reduce_fini(int* __restrict__ status,
+                              const unsigned long long* __restrict__ pool_ptr) {
+    if (threadIdx.x == 0) {
+        const unsigned long long lines = pool_ptr[0] / kLine;
+        status[5] = lines > 0x7FFFFFFFull ? 0x7FFFFFFF : (int)lines;
+    }
+}
+
+__global__ void k_set_u64(unsigned long long* p, unsigned long long v) {
+    if (threadIdx.x == 0) p[0] = v;
+}
+
 __global__ void k_reduce_init(int* __restrict__ status, unsigned long long* __restrict__ pool_ptr,
                               const int* __restrict__ off, long long K) {
     if (threadIdx.x == 0) {
@@ -462,6 +707,7 @@ __global__ void k_reduce_init(int* __restrict__ status, unsigned long long* __re
         status[2] = 0;
         status[3] = 0;
         status[4] = 0;
+        status[5] = 0;
         // apparent region: off[K] + K words, one line of padding (speculative
         // reads of the last apparent column), then line-aligned dynamic space
         pool_ptr[0] = ((unsigned long long)off[K] + K + kLine - 1) / kLine * kLine + kLine;
@@ -492,6 +738,61 @@ extern "C" size_t fph_reduce_scratch_bytes(int64_t K) {
     const size_t k4 = align_up(4 * (size_t)std::max<int64_t>(K, 1));
     // desc, two worklists, sort keys in/out, sort temp, pool counter
     return 5 * k4 + sort_bytes(K) + 256;
+}
+
+extern "C" int fph_reduce_serial(int64_t K, const int32_t* col_off, const uint32_t* col_data,
+                                 int64_t row_base, int64_t nrows, const int32_t* seg_cols,
+                                 int64_t nseg, const int64_t* seg_pool, int64_t pool_start,
+                                 const uint8_t* kind, uint64_t* own, int32_t* pivot,
+                                 const int32_t* res_list, const int32_t* res_count,
+                                 uint32_t* pool, int64_t pool_cap, void* scratch,
+                                 size_t scratch_bytes, int32_t* status, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (K < 0 || nrows < 0 || pool_cap < 0 || !scratch || !status || nseg < 1 || !seg_pool)
+        return FPH_ERR_ARG;
+    if (nseg > 1 && !seg_cols) return FPH_ERR_ARG;
+    if (scratch_bytes < fph_reduce_scratch_bytes(K)) return FPH_ERR_ARG;
+    if (pool_start < 0 || pool_start > pool_cap || pool_cap > 0xFFFFFFFFLL) return FPH_ERR_ARG;
+    const size_t k4 = align_up(4 * (size_t)std::max<int64_t>(K, 1));
+    char* w = (char*)scratch;
+    RedArgs a;
+    a.off = col_off;
+    a.data = col_data;
+    a.row_base = row_base;
+    a.nrows = nrows;
+    a.K = K;
+    a.kind = kind;
+    a.own = (unsigned long long*)own;
+    a.pivot = pivot;
+    a.desc = (unsigned*)w;
+    unsigned* keys = (unsigned*)(w + 3 * k4);
+    unsigned* order = (unsigned*)(w + 4 * k4);
+    void* sort_tmp = w + 5 * k4;
+    size_t sort_sz = sort_bytes(K);
+    a.pool_ptr = (unsigned long long*)(w + 5 * k4 + sort_sz);
+    a.pool = pool;
+    a.pool_cap = (unsigned long long)pool_cap;
+    a.status = status;
+    k_reduce_init<<<1, 64, 0, s>>>(status, a.pool_ptr, col_off, K);
+    FPH_LAUNCH_CHECK();
+    k_set_u64<<<1, 64, 0, s>>>(a.pool_ptr, (unsigned long long)pool_start);
+    FPH_LAUNCH_CHECK();
+    if (K > 0) {
+        const unsigned grid_w =
+            (unsigned)std::min<int64_t>(1024, (K + kPhaseAWaves - 1) / kPhaseAWaves);
+        k_publish_apparent<<<grid_w, kPhaseAWaves * FPH_WAVE, 0, s>>>(a, K, 0);
+        k_order_keys<<<fph_blocks(K, 256), 256, 0, s>>>(res_list, res_count, K, keys);
+        FPH_LAUNCH_CHECK();
+        FPH_CHECK_HIP(rocprim::radix_sort_keys_desc(sort_tmp, sort_sz, keys, order, (unsigned)K,
+                                                    0, 32, s));
+        const size_t lds = kArenaB * 4;
+        k_reduce_serial<<<(unsigned)nseg, FPH_WAVE, lds, s>>>(a, order, res_count, seg_cols,
+                                                             (const long long*)seg_pool);
+        FPH_LAUNCH_CHECK();
+    }
+    k_reduce_fini<<<1, 64, 0, s>>>(status, a.pool_ptr);
+    FPH_LAUNCH_CHECK();
+    return FPH_OK;
 }
 
 extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col_data,
@@ -528,10 +829,14 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
     a.status = status;
     k_reduce_init<<<1, 64, 0, s>>>(status, a.pool_ptr, col_off, K);
     FPH_LAUNCH_CHECK();
-    if (K == 0) return FPH_OK;
+    if (K == 0) {
+        k_reduce_fini<<<1, 64, 0, s>>>(status, a.pool_ptr);
+        FPH_LAUNCH_CHECK();
+        return FPH_OK;
+    }
     const unsigned grid_w =
         (unsigned)std::min<int64_t>(1024, (K + kPhaseAWaves - 1) / kPhaseAWaves);
-    k_publish_apparent<<<grid_w, kPhaseAWaves * FPH_WAVE, 0, s>>>(a, K);
+    k_publish_apparent<<<grid_w, kPhaseAWaves * FPH_WAVE, 0, s>>>(a, K, 1);
     const size_t lds_a = kPhaseAWaves * kArenaA * 4;
     k_reduce_chain<<<grid_w, kPhaseAWaves * FPH_WAVE, lds_a, s>>>(a, res_list, res_count);
     k_order_keys<<<fph_blocks(K, 256), 256, 0, s>>>(res_list, res_count, K, keys);
@@ -543,6 +848,8 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
     k_reduce_rounds<<<(unsigned)nseg, kPhaseBWaves * FPH_WAVE, lds_b, s>>>(
         a, order, res_count, seg_cols, wl0, wl1, max_rounds);
+    FPH_LAUNCH_CHECK();
+    k_reduce_fini<<<1, 64, 0, s>>>(status, a.pool_ptr);
     FPH_LAUNCH_CHECK();
     return FPH_OK;
 }

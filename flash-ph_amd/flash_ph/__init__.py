@@ -9,5 +9,7 @@ __version__ = "0.1.0"
 from flash_ph.persistence import rips_persistence
 from flash_ph.threshold import auto_threshold, enclosing_radius
 from flash_ph.complex import RipsComplex
+from flash_ph.batch import auto_threshold_batch, rips_persistence_batch
 
-__all__ = ["rips_persistence", "auto_threshold", "enclosing_radius", "RipsComplex"]
+__all__ = ["rips_persistence", "auto_threshold", "enclosing_radius", "RipsComplex",
+           "rips_persistence_batch", "auto_threshold_batch"]

@@ -47,6 +47,8 @@ SIGNATURES = {
     "fph_reduce_scratch_bytes": (_SZ, [_I64]),
     "fph_reduce": (_I32, [_I64, _P, _P, _I64, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _P,
                           _SZ, _P, _I32, _P]),
+    "fph_reduce_serial": (_I32, [_I64, _P, _P, _I64, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P,
+                                 _P, _P, _I64, _P, _SZ, _P, _P]),
     "fph_diagram": (_I32, [_I64, _P, _P, _P, _P, _P, _I64, _P, _P, _I64, _I32, _P, _P, _P, _I64,
                            _P, _P, _P, _PSZ, _P]),
     "fph_clear_pivots": (_I32, [_I64, _P, _P, _I64, _I64, _P, _P]),
@@ -59,6 +61,8 @@ SIGNATURES = {
                                _I64, _P, _P, _PSZ, _P]),
     "fph_knn_threshold_batch": (_I32, [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I32, _F64, _I32,
                                        _P, _P, _P, _P, _PSZ, _P]),
+    "fph_enclosing_radius_batch": (_I32, [_P, _P, _P, _P, _P, _I64, _I64, _I32, _P, _P, _PSZ,
+                                          _P]),
     "fph_edge_screen_batch": (_I32, [_P, _P, _P, _P, _P, _P, _I64, _I64, _P, _P, _I64, _P, _P]),
     "fph_edge_sort_batch": (_I32, [_P, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _PSZ, _P]),
     "fph_h0_diagram_batch": (_I32, [_P, _P, _I64, _P, _P, _I64, _I64, _P, _P, _PSZ, _P]),

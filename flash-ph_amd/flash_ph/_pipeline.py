@@ -25,6 +25,10 @@ _EDGE_CAP_PER_POINT = 64
 # Reduction pool (32-bit words): max(this, 8 x cofacet entries).
 _POOL_MIN_WORDS = 1 << 20
 _MAX_ROUNDS = 1 << 22
+# Entry cap of the reference's reduction store (_numba_utils.py:11).
+_REF_MAX_POOL = 100_000_000
+# When a list, every reduction attempt appends (pool words, status words).
+REDUCE_LOG = None
 
 
 class StageTimer:
@@ -196,8 +200,12 @@ def build_mst(cx: Complex, want_labels: bool = False) -> None:
     cx.info["n_mst"] = nm
 
 
-def build_h1(cx: Complex, max_triangles: int = MAX_TRIANGLES) -> None:
-    """Stages (d), (c), (e) for H1 and the H1 diagram."""
+def build_h1(cx: Complex, max_triangles: int = MAX_TRIANGLES, semantics: str = "exact") -> None:
+    """Stages (d), (c), (e) for H1 and the H1 diagram.
+
+    semantics: "exact" (parallel reduction; the pairs of any complete
+    reduction) or "reference" (the reference's capacity-limited serial
+    reduction, identical to its output even where its store overflows)."""
     lib = load()
     dev, n, E = cx.device, cx.n, cx.E
     s = stream_of(dev)
@@ -224,6 +232,7 @@ def build_h1(cx: Complex, max_triangles: int = MAX_TRIANGLES) -> None:
         tmax = max(tcs) if tcs else 0
         segs = {"cols": cx.edge_off, "edges": cx.edge_off,
                 "ess_asc": (tc == 0).to(torch.uint8), "cols_host": host[2 + nc:]}
+        ecs = [b - a for a, b in zip(segs["cols_host"][:-1], segs["cols_host"][1:])]
     if not int(cx.info["converged"].item()):
         raise _lib.FlashPHError("Boruvka did not converge")
     if tmax > max_triangles:
@@ -247,20 +256,43 @@ def build_h1(cx: Complex, max_triangles: int = MAX_TRIANGLES) -> None:
             (ptr(cx.edge_i), ptr(cx.edge_j), E, T, ptr(cx.adj_ptr), ptr(cx.adj_idx),
              ptr(cx.adj_rank), ptr(cx.mst_mask), ptr(cx.tri_off), ptr(cx.tri_w),
              ptr(cx.cof_off), C, ptr(cx.cof_data)), (s,), dev)
+    if semantics == "reference":
+        # the reference's store counts 3 cofacet entries per triangle and
+        # all E edges as columns (reduce_h1_gpu.py:523-534)
+        ref_sizes = ([(3 * T, E)] if segs is None else
+                     [(3 * t, e) for t, e in zip(tcs, ecs)])
+    elif semantics == "exact":
+        ref_sizes = None
+    else:
+        raise ValueError(f"unknown reduction semantics {semantics!r}")
     cx.kind1, cx.pivot1, h1, stats = _reduce_and_diagram(
         K=E, col_base=0, off=cx.cof_off, data=cx.cof_data, skip=cx.mst_mask, row_cmf=cx.tri_cmf,
         row_base=E, nrows=T, C=C, col_edge=None, row_edge=cx.tri_e, edge_filt=cx.edge_filt,
-        E=E, ess_ascending=(T == 0), device=dev, segs=segs)
+        E=E, ess_ascending=(T == 0), device=dev, segs=segs, ref_sizes=ref_sizes)
     cx.h1 = h1 if segs is not None else h1[0]
     cx.info["h1_stats"] = stats
 
 
+def reference_pool_caps(sizes) -> list:
+    """Store capacity of the reference's reducer per segment, from
+    (total cofacet entries, columns) (reduce_h1_gpu.py:286-289)."""
+    return [min(max(2 * cof, 16 * k, 100_000), _REF_MAX_POOL) for cof, k in sizes]
+
+
+def _round32(x: int) -> int:
+    return (x + 31) // 32 * 32
+
+
 def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, C, col_edge,
-                        row_edge, edge_filt, E, ess_ascending, device, segs=None):
+                        row_edge, edge_filt, E, ess_ascending, device, segs=None,
+                        ref_sizes=None):
     """Apparent pairs, residual reduction and the diagram of one dimension.
 
     ``segs`` (batches) = dict(cols=, edges=, ess_asc= device tensors,
     cols_host= list): the columns split into independent point clouds.
+    ``ref_sizes``: None for the parallel exact reduction, or per segment
+    (reference cofacet entries, reference columns) to run the reference's
+    capacity-limited serial reduction (fph_reduce_serial).
     Returns (kind, pivot, [diagram per segment], stats)."""
     lib = load()
     s = stream_of(device)
@@ -272,14 +304,26 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
     own = _empty(torch.int64, device, max(nrows, 1))
     pivot = _empty(torch.int32, device, max(K, 1))
     res_list = _empty(torch.int32, device, max(K, 1))
-    # info32: [0..4] reduction status, [5] residual count, [8..8+nseg) rows
+    # info32: [0..5] reduction status, [6] residual count, [8..8+nseg) rows
     info32 = torch.zeros(8 + nseg, dtype=torch.int32, device=device)
     status_p = info32.data_ptr()
-    rescnt_p = info32.data_ptr() + 20
+    rescnt_p = info32.data_ptr() + 24
     counts_p = info32.data_ptr() + 32
     scratch_bytes = lib.fph_reduce_scratch_bytes(K)
-    # apparent columns in place (C + K words) + line-rounded reduced columns
-    pool_words = C + K + 64 + max(_POOL_MIN_WORDS, 64 * K + 4 * C)
+    if ref_sizes is None:
+        # apparent columns in place (C + K words) + line-rounded reduced columns
+        pool_words = C + K + 64 + max(_POOL_MIN_WORDS, 64 * K + 4 * C)
+    else:
+        caps = reference_pool_caps(ref_sizes)
+        base = _round32(C + K) + 32
+        seg_pool = []
+        for cap, (_, kc) in zip(caps, ref_sizes):
+            seg_pool += [base, cap]
+            base += _round32(cap + kc + 64)
+        pool_start = base
+        seg_pool_t = torch.tensor(seg_pool, dtype=torch.int64).to(device)
+        # + two growing off-chip buffers per segment
+        pool_words = pool_start + max(_POOL_MIN_WORDS, nseg * 2 * 16384)
     out = _empty(torch.float32, device, max(K, 1), 2)
     while True:
         check(lib.fph_apparent(K, col_base, ptr(off), ptr(data), ptr(skip), ptr(row_cmf),
@@ -288,10 +332,19 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
         pool = _empty(torch.int32, device, pool_words + 64)  # +64: speculative line reads
         scratch = _empty(torch.uint8, device, scratch_bytes)
         tok = TIMER.start("reduce", device)
-        check(lib.fph_reduce(K, ptr(off), ptr(data), row_base, nrows, ptr(seg_cols), nseg,
-                             ptr(kind), ptr(own), ptr(pivot), ptr(res_list), rescnt_p,
-                             ptr(pool), pool_words, ptr(scratch), scratch_bytes, status_p,
-                             _MAX_ROUNDS, s), "fph_reduce")
+        if ref_sizes is None:
+            check(lib.fph_reduce(K, ptr(off), ptr(data), row_base, nrows, ptr(seg_cols), nseg,
+                                 ptr(kind), ptr(own), ptr(pivot), ptr(res_list), rescnt_p,
+                                 ptr(pool), pool_words, ptr(scratch), scratch_bytes, status_p,
+                                 _MAX_ROUNDS, s), "fph_reduce")
+        else:
+            if pool_words > 0xFFFFFFFF:
+                raise _lib.FlashPHError("reduction store exceeds 2^32 words")
+            check(lib.fph_reduce_serial(K, ptr(off), ptr(data), row_base, nrows, ptr(seg_cols),
+                                        nseg, ptr(seg_pool_t), pool_start, ptr(kind), ptr(own),
+                                        ptr(pivot), ptr(res_list), rescnt_p, ptr(pool),
+                                        pool_words, ptr(scratch), scratch_bytes, status_p, s),
+                  "fph_reduce_serial")
         TIMER.stop(tok)
         call_ws("fph_diagram",
                 (K, ptr(kind), ptr(off), ptr(data), ptr(pivot), ptr(col_edge), row_base,
@@ -299,14 +352,16 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
                  ptr(seg_edges), ptr(seg_ess_asc), nseg, ptr(out), counts_p),
                 (s,), device)
         st = info32.tolist()
+        if REDUCE_LOG is not None:
+            REDUCE_LOG.append((pool_words, st[:8]))
         if st[0] == 0:
             break
         if st[0] == 1:
-            pool_words *= 4
+            pool_words = max(2 * pool_words, 32 * st[5] + (32 * st[5]) // 4)
             continue
         raise _lib.FlashPHError(f"fph_reduce failed with status {st[0]}")
     stats = {"rounds": st[1], "xors": st[2], "longest": st[3], "offchip_xors": st[4],
-             "residual": st[5]}
+             "pool_words": 32 * st[5], "pool_cap": pool_words, "residual": st[6]}
     counts = st[8:8 + nseg]
     starts = [0] if segs is None else segs["cols_host"][:-1]
     dgms = [out[a:a + c] for a, c in zip(starts, counts)]
@@ -354,13 +409,14 @@ def build_h2(cx: Complex) -> None:
     cx.info["h2_stats"] = stats
 
 
-def compute(pts: torch.Tensor, max_edge_length: float, max_dim: int) -> tuple[list, Complex]:
+def compute(pts: torch.Tensor, max_edge_length: float, max_dim: int,
+            semantics: str = "exact") -> tuple[list, Complex]:
     """Diagrams [H0, ..., H_max_dim] of a CUDA point cloud with n >= 2."""
     cx = build_edges(pts, max_edge_length)
     build_mst(cx)
     out = [cx.h0]
     if max_dim >= 1:
-        build_h1(cx)
+        build_h1(cx, semantics=semantics if max_dim == 1 else "exact")
         out.append(cx.h1)
     if max_dim >= 2:
         build_h2(cx)
@@ -371,6 +427,7 @@ def compute(pts: torch.Tensor, max_edge_length: float, max_dim: int) -> tuple[li
 # ----------------------------------------------------------------- batches
 
 _KNN_TILE = 256
+KNN_MAX_K = 128
 _SCREEN_TILE = 64
 # Largest cloud whose local colex index fits the 32-bit key field, so that
 # keys are unique and the per-cloud order is the key order (rips.py:36-49).
@@ -499,8 +556,12 @@ def build_mst_batch(L: BatchLayout, cx: Complex) -> None:
 
 
 def compute_batch(L: BatchLayout, thr_sq: torch.Tensor, max_dim: int,
-                  max_triangles: int = MAX_TRIANGLES) -> tuple[list, Complex]:
+                  max_triangles: int = MAX_TRIANGLES,
+                  semantics: str = "exact") -> tuple[list, Complex]:
     """Diagrams [[H0, H1?] per cloud] of a packed batch (max_dim <= 1)."""
+    if L.N == 0:
+        empty = torch.empty(0, 2, dtype=torch.float32, device=L.pts.device)
+        return [[empty] * (max_dim + 1) for _ in range(L.C)], None
     cx = build_edges_batch(L, thr_sq)
     build_mst_batch(L, cx)
     vb = L.vbase_host
@@ -509,5 +570,5 @@ def compute_batch(L: BatchLayout, thr_sq: torch.Tensor, max_dim: int,
         if not int(cx.info["converged"].item()):
             raise _lib.FlashPHError("Boruvka did not converge")
         return [[h] for h in h0s], cx
-    build_h1(cx, max_triangles)
+    build_h1(cx, max_triangles, semantics=semantics)
     return [[h0, h1] for h0, h1 in zip(h0s, cx.h1)], cx

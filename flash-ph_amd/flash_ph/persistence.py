@@ -15,6 +15,8 @@ from flash_ph import _lib, _pipeline
 
 _VERTEX_PACK_BITS = 16
 _VALID_BACKENDS = ("auto", "gpu", "giotto")
+# accepted in addition to the reference's backends
+_EXTRA_BACKENDS = ("exact",)
 
 
 def _rips_h1h2_giotto(filt, n, max_dim, max_edge_length, device):
@@ -67,7 +69,13 @@ def rips_persistence(
         H2 uses 16-bit vertex packing, requiring n < 65536.
     backend : str
         H1 reduction backend (validated when ``max_dim=1``).  ``'auto'`` and
-        ``'gpu'`` run the GPU reduction; ``'giotto'`` hands H1 to giotto-ph.
+        ``'gpu'`` reduce on the GPU with the reference's own semantics,
+        including the capacity of its column store (a reduced column that
+        does not fit is not stored, reduce_h1_gpu.py:282-289), so the pairs
+        equal the reference's in every case; ``'exact'`` runs the parallel
+        complete reduction (equal to ``'gpu'`` whenever the reference's store
+        does not overflow); ``'giotto'`` hands H1 to giotto-ph.  For
+        ``max_dim=2`` H1 and H2 are complete reductions, as giotto-ph's.
 
     Returns
     -------
@@ -114,7 +122,7 @@ def rips_persistence(
             for _ in range(max_dim)
         ]
 
-    if max_dim == 1 and backend not in _VALID_BACKENDS:
+    if max_dim == 1 and backend not in _VALID_BACKENDS + _EXTRA_BACKENDS:
         # reduce_h1_gpu.py:462-465
         raise ValueError(
             f"backend must be one of {_VALID_BACKENDS}, got {backend!r}"
@@ -134,7 +142,8 @@ def rips_persistence(
         h1 = _rips_h1h2_giotto(filt, n, 1, max_edge_length, pts.device)[0]
         out = [cx.h0, h1]
     else:
-        out, _ = _pipeline.compute(pts, max_edge_length, max_dim)
+        semantics = "exact" if backend == "exact" else "reference"
+        out, _ = _pipeline.compute(pts, max_edge_length, max_dim, semantics=semantics)
     if device.type != "cuda" or out[0].device != device:
         out = [d.to(device) for d in out]
     return out

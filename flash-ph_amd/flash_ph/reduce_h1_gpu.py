@@ -45,5 +45,6 @@ def rips_h1_gpu_native(filt, n, mst_idx, max_edge_length, device, backend='auto'
         mask[mst_idx.long()] = 1
     cx.mst_mask = mask
     cx.info["converged"] = torch.ones(1, dtype=torch.int32)
-    _pipeline.build_h1(cx)
+    # 'auto' and 'gpu': the reference's store semantics (see rips_persistence)
+    _pipeline.build_h1(cx, semantics="reference")
     return cx.h1.to(device)

@@ -4,14 +4,22 @@
 - ``enclosing_radius``: min over points of the max distance to any other
   point; simplices with a larger diameter cannot carry a non-trivial pair.
 
-Both follow the reference's tiled torch formulation (threshold.py:133-200):
-doubly tiled ``torch.cdist`` with a running top-k, then ``torch.quantile``.
+Both run on the GPU through libflashph (fph_knn_threshold_batch,
+fph_enclosing_radius_batch): one thread per query point scans its cloud
+with the edge screen's FP32 FMA-chain distances, so a threshold derived
+here is measured in the same arithmetic as the filtration it bounds.  The
+reference computes the same quantities with torch.cdist (threshold.py:133-200),
+whose GEMM-form rounding differs from the direct form in the last bits.
+For k > 128 auto_threshold keeps the reference's tiled cdist + top-k
+formulation, on the GPU.
 """
 from __future__ import annotations
 
 import numpy as np
 import torch
 from torch import Tensor
+
+from flash_ph import _lib, _pipeline
 
 
 def auto_threshold(
@@ -42,6 +50,7 @@ def auto_threshold(
     n = pts.shape[0]
     if n <= 1:
         return 0.0
+    pts = _on_gpu(pts)
     k = max(1, min(k, n - 1))
     if subsample is not None and subsample >= 2 and subsample < n:
         idx = torch.randperm(n, device=pts.device)[:subsample]
@@ -49,6 +58,10 @@ def auto_threshold(
         n = pts.shape[0]
         k = max(1, min(k, n - 1))
     percentile = max(0.0, min(100.0, percentile))
+    if k <= _pipeline.KNN_MAX_K:
+        L = _pipeline.layout_batch([pts.contiguous()], pts.device)
+        t, _ = _pipeline.knn_threshold_batch(L, k, percentile)
+        return float(t.item())
     knn_dists = _tiled_knn(pts, k)
     return float(torch.quantile(knn_dists, percentile / 100.0).item())
 
@@ -62,19 +75,35 @@ def enclosing_radius(
     n = pts.shape[0]
     if n <= 1:
         return 0.0
+    pts = _on_gpu(pts)
     if subsample is not None and subsample >= 2 and subsample < n:
         idx = torch.randperm(n, device=pts.device)[:subsample]
         pts = pts[idx]
-    return float(_tiled_enclosing_radius(pts))
+    L = _pipeline.layout_batch([pts.contiguous()], pts.device)
+    r = _enclosing_radius_batch(L)
+    return float(r.item())
+
+
+def _enclosing_radius_batch(L) -> Tensor:
+    r = torch.empty(L.C, dtype=torch.float32, device=L.pts.device)
+    _lib.call_ws("fph_enclosing_radius_batch",
+                 (_lib.ptr(L.pts), _lib.ptr(L.foff), _lib.ptr(L.dims), _lib.ptr(L.vbase),
+                  _lib.ptr(L.qtile_off), L.n_qtiles, L.C, max(1, L.max_d), _lib.ptr(r)),
+                 (_lib.stream_of(L.pts.device),), L.pts.device)
+    return r
 
 
 def _to_tensor(points) -> Tensor:
+    """float32 tensor (threshold.py:123-130)."""
     if isinstance(points, np.ndarray):
-        t = torch.from_numpy(points).float()
-        if torch.cuda.is_available():
-            t = t.cuda()
-        return t
+        points = torch.from_numpy(points)
     return points.float()
+
+
+def _on_gpu(pts: Tensor) -> Tensor:
+    """The computation runs on the GPU; CPU tensors are moved there."""
+    _lib.require_gpu()
+    return pts if pts.device.type == "cuda" else pts.cuda()
 
 
 def _tiled_knn(pts: Tensor, k: int, tile: int = 4096) -> Tensor:

@@ -170,10 +170,11 @@ size_t fph_reduce_scratch_bytes(int64_t K);
  * (reduced to zero: essential), and own[r - row_base] = (owner + 1) << 32 |
  * pool offset for every pivot row.  Columns are written to `pool`
  * (pool_cap 32-bit words, as [length, rows...]; the buffer must extend 64
- * words past pool_cap).  status (device int32[5]): [0] = 0 ok, 1 pool too
- * small (retry with a larger pool), 2 round cap reached, 3 inconsistent
- * input; [1] = rounds; [2] = column additions performed; [3] = longest
- * reduced column; [4] = additions too long for on-chip memory.
+ * words past pool_cap).  status (device int32[6]): [0] = 0 ok, 1 pool too
+ * small (retry with a pool of more than 32 * status[5] words), 2 round cap
+ * reached, 3 inconsistent input; [1] = rounds; [2] = column additions
+ * performed; [3] = longest reduced column; [4] = additions too long for
+ * on-chip memory; [5] = pool words handed out, in 32-word lines.
  * seg_cols (nseg+1 column offsets, device, may be NULL when nseg == 1)
  * splits the columns into independent segments (the point clouds of a
  * batch: no row is shared between segments); one workgroup reduces each.
@@ -185,6 +186,29 @@ int fph_reduce(int64_t K, const int32_t *col_off, const uint32_t *col_data, int6
                uint64_t *own, int32_t *pivot, const int32_t *res_list,
                const int32_t *res_count, uint32_t *pool, int64_t pool_cap, void *scratch,
                size_t scratch_bytes, int32_t *status, int max_rounds, void *stream);
+
+/*
+ * The reference's own reduction semantics, for results identical to it in
+ * every case: per segment, apparent columns enter a store of seg_pool[2s+1]
+ * entries in ascending column order, then residual columns are reduced one
+ * at a time in descending rank order; a column that does not fit in the
+ * store is not stored (later columns reaching its pivot stop there) and
+ * column additions keep at most 1048576 rows.  Where the store never
+ * fills, the pairs equal fph_reduce's.  seg_pool (device int64, 2 per
+ * segment): word offset in `pool` of the segment's store (at least
+ * capacity + segment columns + 64 words, beyond the apparent region of
+ * col_off[K] + K words) and its capacity in reference entries.  pool_start:
+ * first pool word after all stores.  status as for fph_reduce.
+ * Replaces _general_cohomology_reduce_v2 (cohomology_general.py:202-371)
+ * with the pool sizing of reduce_h1_gpu.py:282-289 and the prefill of
+ * _numba_utils.py:68-104, including their capacity limits.
+ */
+int fph_reduce_serial(int64_t K, const int32_t *col_off, const uint32_t *col_data,
+                      int64_t row_base, int64_t nrows, const int32_t *seg_cols, int64_t nseg,
+                      const int64_t *seg_pool, int64_t pool_start, const uint8_t *kind,
+                      uint64_t *own, int32_t *pivot, const int32_t *res_list,
+                      const int32_t *res_count, uint32_t *pool, int64_t pool_cap,
+                      void *scratch, size_t scratch_bytes, int32_t *status, void *stream);
 
 /* ---- diagrams --------------------------------------------------------- */
 
@@ -295,6 +319,16 @@ int fph_knn_threshold_batch(const float *pts, const int64_t *foff, const int32_t
                             int64_t C, int64_t N, int k, double percentile, int max_dim_host,
                             float *kth_dist, float *t_out, float *thr_sq_out, void *ws,
                             size_t *ws_bytes, void *stream);
+
+/*
+ * Per-cloud enclosing radius min_x max_y |x - y| (threshold.py:81-116,
+ * 179-200), with the filtration's FP32 distances; 0 for n_c <= 1.  Same
+ * layout arguments as fph_knn_threshold_batch.
+ */
+int fph_enclosing_radius_batch(const float *pts, const int64_t *foff, const int32_t *dims,
+                               const int64_t *vbase, const int64_t *qtile_off,
+                               int64_t n_qtiles, int64_t C, int max_dim_host, float *r_out,
+                               void *ws, size_t *ws_bytes, void *stream);
 
 /*
  * Screen every cloud against its own FP32 threshold thr_sq[c] (strict <),

@@ -27,6 +27,9 @@
  *   tetrahedra                    _numba_utils.py:186-284
  *   tetrahedron order             kernels/apparent_pair_h2_kernel.py:52-58,118
  *   H1 -> H2 clearing cascade     cohomology_general.py:171-173
+ *   k-th NN distance (threshold)  threshold.py:133-176 (distance in the FMA
+ *                                 form above, not cdist's GEMM form)
+ *   enclosing radius              threshold.py:179-200 (same distances)
  *
  * For max_dim=2 the reference hands H1/H2 to giotto-ph (persistence.py:197-206),
  * which is not available here.  H2 is restated with the same cohomology
@@ -324,8 +327,8 @@ typedef struct {
 } ReduceOut;
 
 static int64_t xor_merge(const int32_t *a, int64_t na, const int32_t *b, int64_t nb,
-                         int32_t *out, int64_t *trunc) {
-    int64_t ia = 0, ib = 0, no = 0, lim = ORC_MAX_COL;
+                         int32_t *out, int64_t *trunc, int64_t lim) {
+    int64_t ia = 0, ib = 0, no = 0;
     while (ia < na && ib < nb) {
         if (no >= lim) { (*trunc)++; break; }
         if (a[ia] < b[ib]) out[no++] = a[ia++];
@@ -349,7 +352,7 @@ static int64_t xor_merge(const int32_t *a, int64_t na, const int32_t *b, int64_t
 static void reduce_columns(int64_t K, const int32_t *col_rank, const int32_t *off,
                            const int32_t *data, const uint8_t *skip,
                            const float *r2f, int64_t total, uint8_t *apparent_out,
-                           ReduceOut *ro) {
+                           ReduceOut *ro, int exact) {
     memset(ro, 0, sizeof(*ro));
     int32_t *cmf = xmalloc((size_t)nz(total) * 4);
     for (int64_t r = 0; r < total; r++) cmf[r] = -1;
@@ -363,12 +366,24 @@ static void reduce_columns(int64_t K, const int32_t *col_rank, const int32_t *of
     if (K * 16 > pool_cap) pool_cap = K * 16;
     if (100000 > pool_cap) pool_cap = 100000;
     if (pool_cap > ORC_MAX_POOL) pool_cap = ORC_MAX_POOL;
+    /* exact: a store without capacity limit (grown on demand) and no column
+     * truncation -- the complete reduction (the pairs giotto-ph returns). */
+    int64_t store_cap = pool_cap;
     int64_t pool_ptr = 0, num_piv = 0;
     int64_t *piv_of = xmalloc((size_t)nz(total) * 8);   /* rank -> stored idx */
     for (int64_t r = 0; r < total; r++) piv_of[r] = -1;
     int64_t *ps = xmalloc((size_t)nz(K) * 8);
     int64_t *pl = xmalloc((size_t)nz(K) * 8);
     int32_t *pool = xmalloc((size_t)pool_cap * 4);
+#define ORC_STORE_FITS(cn_) (exact || pool_ptr + (cn_) <= pool_cap)
+#define ORC_STORE_GROW(cn_)                                                   \
+    do {                                                                      \
+        while (pool_ptr + (cn_) > store_cap) {                                \
+            store_cap *= 2;                                                   \
+            pool = realloc(pool, (size_t)store_cap * 4);                      \
+            if (!pool) abort();                                               \
+        }                                                                     \
+    } while (0)
 
     ro->pb = xmalloc((size_t)nz(K) * 4);
     ro->pd = xmalloc((size_t)nz(K) * 4);
@@ -398,7 +413,8 @@ static void reduce_columns(int64_t K, const int32_t *col_rank, const int32_t *of
         if (!apparent_out[k]) continue;
         int64_t cn = off[k + 1] - off[k];
         if (cn == 0) continue;
-        if (pool_ptr + cn <= pool_cap && num_piv < K) {
+        if (ORC_STORE_FITS(cn) && num_piv < K) {
+            ORC_STORE_GROW(cn);
             piv_of[data[off[k]]] = num_piv;
             ps[num_piv] = pool_ptr;
             pl[num_piv] = cn;
@@ -421,7 +437,9 @@ static void reduce_columns(int64_t K, const int32_t *col_rank, const int32_t *of
     int64_t maxrow = 16;
     for (int64_t k = 0; k < K; k++)
         if (off[k + 1] - off[k] > maxrow) maxrow = off[k + 1] - off[k];
+    int64_t lim = exact ? INT64_MAX : ORC_MAX_COL;
     int64_t wcap = ORC_MAX_COL;
+    if (exact && total + 1 > wcap) wcap = total + 1;
     int32_t *cur = xmalloc((size_t)wcap * 4), *tmp = xmalloc((size_t)wcap * 4);
     int32_t *carry = xmalloc((size_t)wcap * 4);
     for (int64_t ki = 0; ki < K; ki++) {
@@ -432,20 +450,21 @@ static void reduce_columns(int64_t K, const int32_t *col_rank, const int32_t *of
         if (cn == 0) { ro->eb[ro->ness++] = r_e; continue; }
         ro->residual++;
         /* safety-net apparent check is dead here: cmf matches ext_skip */
-        if (cn > ORC_MAX_COL) { cn = ORC_MAX_COL; ro->truncations++; }
+        if (cn > lim) { cn = lim; ro->truncations++; }
         memcpy(cur, data + off[idx], (size_t)cn * 4);
         while (cn > 0) {
             int64_t pi = piv_of[cur[0]];
             if (pi == -1) break;
             memcpy(carry, pool + ps[pi], (size_t)pl[pi] * 4);
-            cn = xor_merge(cur, cn, carry, pl[pi], tmp, &ro->truncations);
+            cn = xor_merge(cur, cn, carry, pl[pi], tmp, &ro->truncations, lim);
             memcpy(cur, tmp, (size_t)cn * 4);
             ro->xors++;
         }
         if (cn > 0) {
             int32_t piv = cur[0];
             ro->allpiv[ro->nallpiv++] = piv;
-            if (pool_ptr + cn <= pool_cap && num_piv < K) {
+            if (ORC_STORE_FITS(cn) && num_piv < K) {
+                ORC_STORE_GROW(cn);
                 piv_of[piv] = num_piv;
                 ps[num_piv] = pool_ptr;
                 pl[num_piv] = cn;
@@ -467,6 +486,8 @@ static void reduce_columns(int64_t K, const int32_t *col_rank, const int32_t *of
     (void)maxrow;
     free(cur); free(tmp); free(carry); free(order); free(ext_skip);
     free(cmf); free(piv_of); free(ps); free(pl); free(pool);
+#undef ORC_STORE_FITS
+#undef ORC_STORE_GROW
 }
 
 /* Diagram rows: pairs (in list order) ++ essentials (in list order), then a
@@ -513,7 +534,13 @@ static int csr_lookup(const int32_t *ptr, const int32_t *idx, const int32_t *ran
 /* Full pipeline.                                                          */
 /* ---------------------------------------------------------------------- */
 
-OrcResult *orc_run(const float *pts, int64_t n, int d, float thr_sq, int max_dim) {
+/* semantics 0: the reference -- H1 for max_dim <= 1 by its capacity-limited
+ * store (reduce_h1_gpu.py:438-604); for max_dim = 2 the reference hands H1
+ * and H2 to giotto-ph (persistence.py:197-206), whose complete reduction is
+ * restated by the exact store.  semantics 1: exact everywhere. */
+OrcResult *orc_run(const float *pts, int64_t n, int d, float thr_sq, int max_dim,
+                   int semantics) {
+    const int exact1 = semantics == 1 || max_dim >= 2;
     OrcResult *r = calloc(1, sizeof(OrcResult));
     if (!r) abort();
     for (int k = 0; k < A_COUNT; k++) arr_set(r, k, xmalloc(1), 0, 4);
@@ -699,7 +726,7 @@ OrcResult *orc_run(const float *pts, int64_t n, int d, float thr_sq, int max_dim
     for (int64_t e = 0; e < E; e++) colr[e] = (int32_t)e;
     uint8_t *app1 = xmalloc((size_t)E);
     ReduceOut ro1;
-    reduce_columns(E, colr, coff, cdat, mst_mask, r2f, total1, app1, &ro1);
+    reduce_columns(E, colr, coff, cdat, mst_mask, r2f, total1, app1, &ro1, exact1);
     arr_set(r, A_APPARENT1, app1, E, 1);
     r->stats[S_RES1] = ro1.residual;
     r->stats[S_XOR1] = ro1.xors;
@@ -842,7 +869,7 @@ OrcResult *orc_run(const float *pts, int64_t n, int d, float thr_sq, int max_dim
     /* CSR rows are indexed by local triangle rank; reorder data accordingly */
     uint8_t *app2 = xmalloc((size_t)T);
     ReduceOut ro2;
-    reduce_columns(T, col2, c2off, c2dat, skip2, r2f2, total2, app2, &ro2);
+    reduce_columns(T, col2, c2off, c2dat, skip2, r2f2, total2, app2, &ro2, 1);
     arr_set(r, A_APPARENT2, app2, T, 1);
     r->stats[S_RES2] = ro2.residual;
     r->stats[S_XOR2] = ro2.xors;
@@ -894,3 +921,48 @@ float orc_dist_sq(const float *pts, int d, int64_t i, int64_t j) {
 }
 
 float orc_fmaf(float a, float b, float c) { return fmaf(a, b, c); }
+
+/*
+ * k-th nearest-neighbour squared distance of every point (threshold.py:
+ * 133-176: per-row top-k over the distance matrix with the diagonal
+ * masked, k-th = largest of the k smallest).  k must lie in [1, n-1].
+ * Distances are the filtration's FMA chain.  A sorted insertion into a
+ * list of the k smallest values, O(n^2 (d + k)).
+ */
+int orc_knn_kth(const float *pts, int64_t n, int d, int k, float *out) {
+    if (n < 2 || k < 1 || k > n - 1) return -1;
+    float *top = (float *)xmalloc(nz(k) * sizeof(float));
+    for (int64_t q = 0; q < n; q++) {
+        int m = 0; /* entries in top, ascending */
+        for (int64_t j = 0; j < n; j++) {
+            if (j == q) continue;
+            const float v = dist_sq_fma(pts, d, q, j);
+            if (m == k && !(v < top[k - 1])) continue;
+            int at = m < k ? m : k - 1;
+            while (at > 0 && top[at - 1] > v) {
+                top[at] = top[at - 1];
+                at--;
+            }
+            top[at] = v;
+            if (m < k) m++;
+        }
+        out[q] = top[k - 1];
+    }
+    free(top);
+    return 0;
+}
+
+/* Largest squared distance from each point to any other (threshold.py:
+ * 179-200 before its min over points); FMA-chain distances. */
+int orc_far_dsq(const float *pts, int64_t n, int d, float *out) {
+    for (int64_t q = 0; q < n; q++) {
+        float best = 0.0f;
+        for (int64_t j = 0; j < n; j++) {
+            if (j == q) continue;
+            const float v = dist_sq_fma(pts, d, q, j);
+            if (v > best) best = v;
+        }
+        out[q] = best;
+    }
+    return 0;
+}

@@ -61,7 +61,7 @@ def _load():
         lib = ctypes.CDLL(_LIB_PATH)
         lib.orc_run.restype = ctypes.c_void_p
         lib.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
-                                ctypes.c_float, ctypes.c_int]
+                                ctypes.c_float, ctypes.c_int, ctypes.c_int]
         lib.orc_len.restype = ctypes.c_int64
         lib.orc_len.argtypes = [ctypes.c_void_p, ctypes.c_int]
         lib.orc_get.restype = ctypes.c_int
@@ -72,6 +72,12 @@ def _load():
         lib.orc_dist_sq.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64,
                                     ctypes.c_int64]
         lib.orc_fmaf.restype = ctypes.c_float
+        lib.orc_far_dsq.restype = ctypes.c_int
+        lib.orc_far_dsq.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                                    ctypes.c_void_p]
+        lib.orc_knn_kth.restype = ctypes.c_int
+        lib.orc_knn_kth.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_void_p]
         lib.orc_fmaf.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float]
         _lib = lib
     return _lib
@@ -82,7 +88,10 @@ def threshold_sq_f32(max_edge_length: float) -> np.float32:
     return np.float32(float(max_edge_length) * float(max_edge_length))
 
 
-def run(points, max_edge_length: float, max_dim: int = 1) -> dict:
+def run(points, max_edge_length: float, max_dim: int = 1, exact: bool = False) -> dict:
+    """exact=False: the reference's results (its capacity-limited H1 store
+    for max_dim <= 1; the complete reduction giotto-ph performs for
+    max_dim = 2).  exact=True: the complete reduction everywhere."""
     lib = _load()
     pts = np.ascontiguousarray(np.asarray(points, dtype=np.float32))
     if pts.ndim != 2:
@@ -96,7 +105,7 @@ def run(points, max_edge_length: float, max_dim: int = 1) -> dict:
         out["stats"] = {k: 0 for k in STATS}
         return out
     h = lib.orc_run(pts.ctypes.data, n, d, float(threshold_sq_f32(max_edge_length)),
-                    int(max_dim))
+                    int(max_dim), 1 if exact else 0)
     try:
         for idx, name in enumerate(NAMES):
             ln = lib.orc_len(h, idx)
@@ -126,3 +135,61 @@ def dist_sq(points, i: int, j: int) -> np.float32:
 
 def fmaf(a, b, c) -> np.float32:
     return np.float32(_load().orc_fmaf(float(a), float(b), float(c)))
+
+
+def knn_kth_dsq(points, k: int) -> np.ndarray:
+    """k-th nearest-neighbour squared distance per point (FMA-chain form)."""
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float32))
+    n, d = pts.shape
+    out = np.empty(n, dtype=np.float32)
+    if _load().orc_knn_kth(pts.ctypes.data, n, d, int(k), out.ctypes.data) != 0:
+        raise ValueError("knn_kth_dsq: need 1 <= k <= n-1")
+    return out
+
+
+def sqrt_f32(x) -> np.ndarray:
+    """Correctly rounded FP32 square root (sqrt in double, then rounded)."""
+    return np.sqrt(np.asarray(x, dtype=np.float32).astype(np.float64)).astype(np.float32)
+
+
+def quantile_f32(values, q: float) -> np.float32:
+    """torch.quantile(values_f32, q), linear interpolation, restated in FP32
+    (ATen quantile: rank = q * (n - 1) in the input dtype, lerp between the
+    order statistics below and above, with torch's lerp:
+    a + w (b - a) for |w| < 0.5, else b - (b - a)(1 - w))."""
+    v = np.sort(np.asarray(values, dtype=np.float32))
+    n = v.size
+    qf = np.float32(q)
+    rank = np.float32(qf * np.float32(n - 1))
+    lo = int(np.floor(rank))
+    hi = int(np.ceil(rank))
+    w = np.float32(rank - np.float32(lo))
+    a, b = v[lo], v[hi]
+    if abs(w) < np.float32(0.5):
+        return np.float32(a + np.float32(w * np.float32(b - a)))
+    return np.float32(b - np.float32(np.float32(b - a) * np.float32(np.float32(1.0) - w)))
+
+
+def auto_threshold(points, k: int = 20, percentile: float = 95.0) -> float:
+    """Restated auto_threshold (threshold.py:20-78) on the FMA-form distances:
+    k clamped to [1, n-1], percentile to [0, 100]; 0.0 for n <= 1."""
+    pts = np.asarray(points, dtype=np.float32)
+    n = pts.shape[0]
+    if n <= 1:
+        return 0.0
+    k = max(1, min(int(k), n - 1))
+    p = max(0.0, min(100.0, float(percentile)))
+    kth = sqrt_f32(knn_kth_dsq(pts, k))
+    return float(quantile_f32(kth, p / 100.0))
+
+
+def enclosing_radius(points) -> float:
+    """Restated enclosing_radius (threshold.py:81-116, 179-200) on the
+    FMA-form distances: sqrt(min_x max_y dsq); 0.0 for n <= 1."""
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float32))
+    n, d = pts.shape
+    if n <= 1:
+        return 0.0
+    far = np.empty(n, dtype=np.float32)
+    _load().orc_far_dsq(pts.ctypes.data, n, d, far.ctypes.data)
+    return float(sqrt_f32(far.min()))

@@ -82,3 +82,19 @@ MEDIUM = [
     ("clifford_1000_noisy", lambda: clifford(1000, 3, 0.02), 0.45, 2),
     ("randn_d10_n10000_cfg2", lambda: randn(10000, 10, 0), 1.6, 2),
 ]
+
+
+def batch_uneven(C, seed, nmin=200, nmax=5000, dims=(4, 10, 50), noise=0.05):
+    """Clouds of log-uniform size in [nmin, nmax] and dimension drawn from
+    ``dims``; points uniform on the unit (d-1)-sphere plus N(0, noise^2)."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(nmin), np.log(nmax), C)).astype(np.int64)
+    ns = np.clip(ns, nmin, nmax)
+    ds = rng.choice(np.asarray(dims), C)
+    clouds = []
+    for n, d in zip(ns, ds):
+        x = rng.standard_normal((int(n), int(d)))
+        x /= np.linalg.norm(x, axis=1, keepdims=True)
+        x = x + rng.normal(0, noise, x.shape)
+        clouds.append(x.astype(np.float32))
+    return clouds

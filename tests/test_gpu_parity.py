@@ -28,13 +28,13 @@ def _cpu(t):
     return t.detach().cpu().numpy()
 
 
-def _check_case(pts, t, max_dim):
+def _check_case(pts, t, max_dim, semantics="reference"):
     import oracle
     from flash_ph import _pipeline
 
-    ref = oracle.run(pts, t, max_dim)
+    ref = oracle.run(pts, t, max_dim, exact=(semantics == "exact"))
     x = torch.from_numpy(pts).cuda()
-    dgms, cx = _pipeline.compute(x, t, max_dim)
+    dgms, cx = _pipeline.compute(x, t, max_dim, semantics=semantics)
     torch.cuda.synchronize()
     E = cx.E
     # stage (a): edges and CSR
@@ -103,3 +103,42 @@ def test_stage_parity_small(name, make, t, max_dim):
 def test_stage_parity_medium(name, make, t, max_dim):
     print(f"[gpu] parity {name}", flush=True)
     _check_case(make(), t, max_dim)
+
+
+H1_CASES = [c for c in cases.SMALL + cases.MEDIUM if c[3] == 1]
+
+
+@pytest.mark.parametrize("name,make,t,max_dim", H1_CASES, ids=[c[0] for c in H1_CASES])
+def test_exact_semantics_h1(name, make, t, max_dim):
+    """The parallel complete reduction (backend='exact') against the oracle's
+    complete reduction."""
+    print(f"[gpu] exact semantics {name}", flush=True)
+    _check_case(make(), t, 1, semantics="exact")
+
+
+def _store_overflow_clouds():
+    """Clouds of the uneven-batch workload whose reference store overflows
+    (the oracle reports dropped columns): n=1184 d=10 and n=1539 d=50."""
+    clouds = cases.batch_uneven(256, seed=0)
+    return [clouds[140], clouds[225]]
+
+
+@pytest.mark.parametrize("idx", [0, 1])
+def test_store_overflow_reference_and_exact(idx):
+    """Where the reference's capacity-limited store drops columns, backend
+    'auto' reproduces the reference bit for bit and backend 'exact' the
+    complete reduction."""
+    import oracle
+    from flash_ph import rips_persistence
+    x = _store_overflow_clouds()[idx]
+    t = oracle.auto_threshold(x)
+    print(f"[gpu] store overflow case {idx} n={x.shape[0]} d={x.shape[1]}", flush=True)
+    ref = oracle.run(x, t, 1)
+    assert ref["stats"]["pool_drops"] > 0
+    got = rips_persistence(torch.from_numpy(x).cuda(), t, max_dim=1)
+    _eq("h0", _cpu(got[0]), ref["h0"])
+    _eq("h1 reference", _cpu(got[1]), ref["h1"])
+    ex = oracle.run(x, t, 1, exact=True)
+    assert ex["stats"]["pool_drops"] == 0
+    got_ex = rips_persistence(torch.from_numpy(x).cuda(), t, max_dim=1, backend="exact")
+    _eq("h1 exact", _cpu(got_ex[1]), ex["h1"])

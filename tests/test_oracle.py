@@ -239,3 +239,118 @@ def test_row_order_rule():
     ess = h1[np.isinf(h1[:, 1])]
     fin = h1[np.isfinite(h1[:, 1])]
     assert ess.shape[0] + fin.shape[0] == h1.shape[0]
+
+
+# ---------------------------------------------------------------- kNN threshold
+
+def test_knn_oracle_vs_exact_brute():
+    """k-th NN squared distance against exact-rational FMA chains and a full
+    sort (no insertion list), on tiny clouds with ties."""
+    for seed, (n, d) in enumerate([(9, 2), (12, 3), (7, 5)]):
+        pts = cases.randn(n, d, 100 + seed)
+        pts[3] = pts[1]  # a duplicate point: distance 0 counts
+        D = brute.all_dsq(pts)
+        for k in range(1, n):
+            got = oracle.knn_kth_dsq(pts, k)
+            for q in range(n):
+                ds = sorted(D[q, j] for j in range(n) if j != q)
+                assert got[q] == ds[k - 1], (seed, k, q)
+
+
+def test_knn_oracle_vs_torch_cdist():
+    """The reference's cdist + topk k-th distance agrees to cdist rounding."""
+    import torch
+    x = cases.randn(400, 6, 5)
+    for k in (1, 20):
+        ours = oracle.sqrt_f32(oracle.knn_kth_dsq(x, k))
+        t = torch.from_numpy(x).double()
+        dm = torch.cdist(t, t)
+        dm.fill_diagonal_(float("inf"))
+        ref = dm.topk(k, dim=1, largest=False).values[:, -1].numpy()
+        assert np.allclose(ours, ref, rtol=1e-6, atol=0)
+
+
+def test_quantile_matches_torch():
+    """quantile_f32 is torch.quantile (the call the reference makes,
+    threshold.py:77) on FP32 input, bit for bit."""
+    import torch
+    rng = np.random.default_rng(0)
+    for n in (1, 2, 3, 10, 999, 1000, 4097):
+        v = rng.gamma(2.0, 1.0, n).astype(np.float32)
+        for q in (0.0, 0.05, 0.25, 0.5, 0.9, 0.95, 0.999, 1.0, 1 / 3):
+            ref = torch.quantile(torch.from_numpy(v), q).item()
+            assert oracle.quantile_f32(v, q) == np.float32(ref), (n, q)
+
+
+def test_auto_threshold_oracle_edge_cases():
+    assert oracle.auto_threshold(np.zeros((0, 3), np.float32)) == 0.0
+    assert oracle.auto_threshold(np.zeros((1, 3), np.float32)) == 0.0
+    # k clamps to n-1; percentile clamps to [0, 100]
+    x = cases.randn(5, 2, 1)
+    assert oracle.auto_threshold(x, k=50) == oracle.auto_threshold(x, k=4)
+    assert oracle.auto_threshold(x, percentile=500) == oracle.auto_threshold(x, percentile=100)
+    # a regular polygon: every point's k-th neighbour is at the same distance
+    th = np.arange(12) * (2 * np.pi / 12)
+    poly = np.column_stack([np.cos(th), np.sin(th)]).astype(np.float32)
+    t = oracle.auto_threshold(poly, k=2, percentile=50)
+    assert abs(t - 2 * np.sin(np.pi / 12)) < 1e-6
+
+
+def test_enclosing_radius_oracle():
+    """Against exact-rational distances, and against float64 cdist."""
+    import torch
+    x = cases.randn(11, 3, 4)
+    D = brute.all_dsq(x)
+    np.fill_diagonal(D, 0.0)
+    ref = float(np.sqrt(np.float64(D.max(axis=1).min())).astype(np.float32))
+    assert oracle.enclosing_radius(x) == ref
+    y = cases.randn(300, 7, 5)
+    t = torch.from_numpy(y).double()
+    r64 = torch.cdist(t, t).max(dim=1).values.min().item()
+    assert abs(oracle.enclosing_radius(y) - r64) <= 1e-6 * r64
+    assert oracle.enclosing_radius(np.zeros((1, 2), np.float32)) == 0.0
+
+
+# ---------------------------------------------- reduction store semantics
+
+@pytest.mark.parametrize("seed", range(6))
+def test_exact_mode_matches_brute_homology(seed):
+    """The complete reduction (exact=True) is pinned by brute-force homology
+    like the reference mode."""
+    rng = np.random.default_rng(2000 + seed)
+    n = int(rng.integers(8, 16))
+    pts = rng.standard_normal((n, 3)).astype(np.float32)
+    t = float(rng.uniform(1.0, 2.5))
+    r = oracle.run(pts, t, 1, exact=True)
+    ref = brute.brute_diagrams(pts, oracle.threshold_sq_f32(t), 1)
+    for k in range(2):
+        assert _bits_equal(brute.multiset(r["diagrams"][k]), brute.multiset(ref[k])), f"H{k}"
+
+
+def test_store_overflow_changes_only_reference_mode():
+    """A cloud of the uneven-batch workload (n=1184, d=10, auto threshold)
+    overflows the reference's store (reduce_h1_gpu.py:286-289): the
+    reference mode drops columns, the exact mode does not, and where
+    nothing is dropped the two modes agree bit for bit."""
+    x = cases.batch_uneven(256, seed=0)[140]
+    t = oracle.auto_threshold(x)
+    ref = oracle.run(x, t, 1)
+    ex = oracle.run(x, t, 1, exact=True)
+    assert ref["stats"]["pool_drops"] > 0 and ex["stats"]["pool_drops"] == 0
+    assert ref["stats"]["residual1"] == ex["stats"]["residual1"]
+    # H0 and the apparent pairs do not depend on the store
+    assert np.array_equal(ref["h0"], ex["h0"])
+    assert np.array_equal(ref["apparent1"], ex["apparent1"])
+    y = cases.randn(200, 4, 5)
+    a, b = oracle.run(y, 1.0, 1), oracle.run(y, 1.0, 1, exact=True)
+    assert a["stats"]["pool_drops"] == 0
+    assert np.array_equal(a["h1"].view(np.uint32), b["h1"].view(np.uint32))
+
+
+def test_max_dim2_h1_is_complete_reduction():
+    """For max_dim=2 the reference's H1 comes from giotto-ph (a complete
+    reduction): the oracle's H1 then equals its exact-mode H1."""
+    x = cases.randn(120, 3, 9)
+    a = oracle.run(x, 1.1, 2)
+    b = oracle.run(x, 1.1, 1, exact=True)
+    assert np.array_equal(a["h1"].view(np.uint32), b["h1"].view(np.uint32))
