@@ -167,21 +167,18 @@ __device__ int wave_xor_long(const unsigned* A, int na, const unsigned* B, int n
         j1 = nb;
     }
     constexpr unsigned kEnd = 0xFFFFFFFFu;  // above every row rank
+    // branch-free merge steps: the smaller head survives unless the heads
+    // are equal; each list advances when its head was <= the other's
     int cnt = 0;
     {
         int ia = i0, ib = j0;
         unsigned av = ia < i1 ? A[ia] : kEnd, bv = ib < j1 ? B[ib] : kEnd;
         while (av != kEnd || bv != kEnd) {
-            if (av < bv) {
-                cnt++;
-                av = ++ia < i1 ? A[ia] : kEnd;
-            } else if (bv < av) {
-                cnt++;
-                bv = ++ib < j1 ? B[ib] : kEnd;
-            } else {
-                av = ++ia < i1 ? A[ia] : kEnd;
-                bv = ++ib < j1 ? B[ib] : kEnd;
-            }
+            cnt += av != bv;
+            ia += av <= bv;
+            ib += bv <= av;
+            av = ia < i1 ? A[ia] : kEnd;
+            bv = ib < j1 ? B[ib] : kEnd;
         }
     }
     int incl = cnt;
@@ -197,16 +194,11 @@ __device__ int wave_xor_long(const unsigned* A, int na, const unsigned* B, int n
         int ia = i0, ib = j0;
         unsigned av = ia < i1 ? A[ia] : kEnd, bv = ib < j1 ? B[ib] : kEnd;
         while ((av != kEnd || bv != kEnd) && at < len) {
-            if (av < bv) {
-                C[at++] = av;
-                av = ++ia < i1 ? A[ia] : kEnd;
-            } else if (bv < av) {
-                C[at++] = bv;
-                bv = ++ib < j1 ? B[ib] : kEnd;
-            } else {
-                av = ++ia < i1 ? A[ia] : kEnd;
-                bv = ++ib < j1 ? B[ib] : kEnd;
-            }
+            if (av != bv) C[at++] = min(av, bv);
+            ia += av <= bv;
+            ib += bv <= av;
+            av = ia < i1 ? A[ia] : kEnd;
+            bv = ib < j1 ? B[ib] : kEnd;
         }
     }
     wave_sync();
@@ -580,47 +572,191 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_rounds(
 // capacity in reference entries; a stored column takes len + 1 words here.
 constexpr int kSerialMaxCol = 1048576;
 
-__global__ __launch_bounds__(FPH_WAVE) void k_reduce_serial(
+// One workgroup of kSerThreads per segment.  Every column addition is done
+// by the whole workgroup: the merged order of the two sorted lists is cut
+// into kSerThreads pieces along the merge path (ties: working column first;
+// a piece never separates equal rows), each thread counts the survivors of
+// its piece, a workgroup prefix sum places them, and each thread writes its
+// piece.  The working column lives in LDS (two halves, ping-pong) while it
+// fits, the owner column is staged into LDS while it fits, and longer lists
+// are merged in two reusable off-chip buffers.
+constexpr int kSerThreads = 256;
+constexpr int kSerWaves = kSerThreads / FPH_WAVE;
+constexpr int kSerHalf = 12288;   // words per LDS half of the working column
+constexpr int kSerStage = 8192;   // words of LDS staging for the owner column
+constexpr size_t kSerLds = (2 * kSerHalf + kSerStage) * 4;
+
+struct SerShared {
+    int split_i[kSerThreads + 1];
+    int split_j[kSerThreads + 1];
+    int wave_sum[kSerWaves];
+    int flag;
+    int first_bad;
+    unsigned long long big_off[2];
+    unsigned long long next_v;  // owner word of the result's pivot (0: none)
+    int next_nb;                // that owner's length
+};
+
+// Exclusive prefix sum of v over the workgroup; *total = sum.  Contains
+// barriers: call from uniform control flow.
+__device__ int block_excl_scan(int v, SerShared& sh, int* total) {
+    const int lane = fph_lane(), wid = threadIdx.x / FPH_WAVE;
+    int incl = v;
+    for (int o = 1; o < FPH_WAVE; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    if (lane == FPH_WAVE - 1) sh.wave_sum[wid] = incl;
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int w = 0; w < kSerWaves; w++) {
+        const int x = sh.wave_sum[w];
+        if (w < wid) before += x;
+        all += x;
+    }
+    __syncthreads();
+    *total = all;
+    return before + incl - v;
+}
+
+// C = A xor B (sorted, duplicate-free; any sizes), keeping at most `limit`
+// rows (a prefix); C must not alias A or B.  Returns the length.  The
+// owner of the result's pivot is looked up while the result is written:
+// sh.next_v / sh.next_nb (valid when the result is not empty).
+__device__ int block_xor(const unsigned* A, int na, const unsigned* B, int nb, unsigned* C,
+                         int limit, SerShared& sh, const RedArgs& a) {
+    const int t = threadIdx.x;
+    const int total = na + nb;
+    const int per = (total + kSerThreads - 1) / kSerThreads;
+    const int k = min(t * per, total);
+    int lo = max(0, k - nb), hi = min(k, na);
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (A[mid] <= B[k - mid - 1]) lo = mid + 1;
+        else hi = mid;
+    }
+    int i0 = lo;
+    const int j0 = k - lo;
+    if (i0 > 0 && j0 < nb && A[i0 - 1] == B[j0]) i0--;
+    sh.split_i[t] = i0;
+    sh.split_j[t] = j0;
+    if (t == 0) {
+        sh.split_i[kSerThreads] = na;
+        sh.split_j[kSerThreads] = nb;
+    }
+    __syncthreads();
+    const int i1 = sh.split_i[t + 1], j1 = sh.split_j[t + 1];
+    // Branch-free merge steps (the lanes of a wave take different branches
+    // otherwise): both heads are compared, the smaller one survives unless
+    // they are equal, and each list advances when its head was <= the other.
+    constexpr unsigned kEnd = 0xFFFFFFFFu;
+    int cnt = 0;
+    unsigned first = kEnd;  // this piece's first survivor
+    {
+        int ia = i0, ib = j0;
+        unsigned av = ia < i1 ? A[ia] : kEnd, bv = ib < j1 ? B[ib] : kEnd;
+        while (av != kEnd || bv != kEnd) {
+            const unsigned m = min(av, bv);
+            const bool keep = av != bv;
+            first = (keep && cnt == 0) ? m : first;
+            cnt += keep;
+            ia += av <= bv;
+            ib += bv <= av;
+            av = ia < i1 ? A[ia] : kEnd;
+            bv = ib < j1 ? B[ib] : kEnd;
+        }
+    }
+    // speculative: only the piece holding the result's first row keeps it
+    const unsigned long long pv = first != kEnd ? a.own[(long long)first - a.row_base] : 0ull;
+    int all;
+    int at = block_excl_scan(cnt, sh, &all);
+    const int len = min(all, limit);
+    const bool lead = cnt > 0 && at == 0;
+    int pnb = 0;
+    if (lead && pv != kNoOwner) pnb = (int)a.pool[pv & 0xFFFFFFFFull];
+    if (at < len) {
+        int ia = i0, ib = j0;
+        unsigned av = ia < i1 ? A[ia] : kEnd, bv = ib < j1 ? B[ib] : kEnd;
+        while ((av != kEnd || bv != kEnd) && at < len) {
+            const unsigned m = min(av, bv);
+            if (av != bv) C[at++] = m;
+            ia += av <= bv;
+            ib += bv <= av;
+            av = ia < i1 ? A[ia] : kEnd;
+            bv = ib < j1 ? B[ib] : kEnd;
+        }
+    }
+    if (lead) {
+        sh.next_v = pv;
+        sh.next_nb = pnb;
+    }
+    __threadfence_block();
+    __syncthreads();
+    return len;
+}
+
+__device__ __forceinline__ void block_copy(const unsigned* src, int n, unsigned* dst) {
+    for (int i = threadIdx.x; i < n; i += kSerThreads) dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(kSerThreads) void k_reduce_serial(
     RedArgs a, const unsigned* __restrict__ order, const int* __restrict__ res_count,
     const int* __restrict__ seg_cols, const long long* __restrict__ seg_pool) {
     extern __shared__ unsigned smem[];
-    unsigned* arena = smem;
-    const int lane = fph_lane();
-    const unsigned long long lt = fph_lanemask_lt();
+    __shared__ SerShared sh;
+    unsigned* half0 = smem;
+    unsigned* stage = smem + 2 * kSerHalf;
+    const int t = threadIdx.x;
     const int seg = blockIdx.x;
     const int c0 = seg_cols ? seg_cols[seg] : 0;
     const int c1 = seg_cols ? seg_cols[seg + 1] : (int)a.K;
     const long long base = seg_pool[2 * seg];
     const long long cap = seg_pool[2 * seg + 1];
-    long long used = 0;    // reference entries in the store
-    long long stored = 0;  // words written at base
+    long long used = 0;    // reference entries in the store (uniform)
+    long long stored = 0;  // words written at base (uniform)
     // 1. apparent columns, ascending: store while pool_ptr + cn <= pool_cap
-    for (int cb = c0; cb < c1; cb += FPH_WAVE) {
-        const int c = cb + lane;
+    for (int cb = c0; cb < c1; cb += kSerThreads) {
+        const int c = cb + t;
         const bool app = c < c1 && a.kind[c] == 2;
         const int cn = app ? a.off[c + 1] - a.off[c] : 0;
-        int incl = cn;
-        for (int o = 1; o < FPH_WAVE; o <<= 1) {
-            const int y = __shfl_up(incl, o);
-            if (lane >= o) incl += y;
-        }
-        int p = 0;
-        while (p < FPH_WAVE) {
-            const int excl_p = p == 0 ? 0 : __shfl(incl, p - 1);
-            const bool fits = used + (incl - excl_p) <= cap;
-            const unsigned long long bad = __ballot(lane >= p && cn > 0 && !fits);
-            const int f = bad ? __builtin_ctzll(bad) : FPH_WAVE;
-            if (lane >= p && lane < f && cn > 0) {
+        int chunk;
+        const int excl = block_excl_scan(cn, sh, &chunk);
+        const int incl = excl + cn;
+        if (used + chunk <= cap) {  // common case: the whole chunk fits
+            if (app && cn > 0) {
                 const long long lo = a.off[c];
                 a.own[(long long)a.data[lo] - a.row_base] =
                     ((unsigned long long)(c + 1) << 32) | (unsigned long long)(lo + c);
             }
-            if (f > p) used += __shfl(incl, f - 1) - excl_p;
+            used += chunk;
+            continue;
+        }
+        // some column fails: walk the chunk, storing maximal fitting runs
+        int p = 0;
+        while (p < kSerThreads) {
+            if (t == 0) sh.first_bad = kSerThreads;
+            __syncthreads();
+            // excl of thread p: entries of threads [0, p) in this chunk
+            if (t == p) sh.flag = excl;
+            __syncthreads();
+            const int excl_p = sh.flag;
+            if (t >= p && cn > 0 && used + (incl - excl_p) > cap) atomicMin(&sh.first_bad, t);
+            __syncthreads();
+            const int f = sh.first_bad;
+            if (t >= p && t < f && cn > 0) {
+                const long long lo = a.off[c];
+                a.own[(long long)a.data[lo] - a.row_base] =
+                    ((unsigned long long)(c + 1) << 32) | (unsigned long long)(lo + c);
+            }
+            if (t == f - 1) sh.flag = incl;
+            __syncthreads();
+            if (f > p) used += sh.flag - excl_p;
+            __syncthreads();
             p = f + 1;  // column f (if any) is dropped
         }
     }
-    (void)lt;
-    wave_sync();
+    __threadfence_block();
+    __syncthreads();
     // 2. residual columns of the segment, descending rank
     const int R = *res_count;
     int lo = 0, hi = R;
@@ -635,51 +771,118 @@ __global__ __launch_bounds__(FPH_WAVE) void k_reduce_serial(
         if ((int)order[mid] >= c0) lo2 = mid + 1;
         else hi2 = mid;
     }
-    Alloc al = make_alloc(4096);
-    int nxor = 0, longest = 0;
+    unsigned long long big_off[2] = {0, 0};
+    long long big_cap[2] = {0, 0};
+    int nxor = 0, longest = 0, offchip = 0;
     for (int r = lo; r < lo2; r++) {
         const int c = (int)order[r];
         const int lo_c = a.off[c];
         const int n0 = min(a.off[c + 1] - lo_c, kSerialMaxCol);
-        Work w;
-        w.len = n0;
-        if (n0 <= kHalfB) {
-            wave_copy(a.data + lo_c, n0, arena);
-            w.ptr = arena;
-            w.half = 0;
+        const unsigned* cur;
+        int where;  // 0 / 1: LDS half, 2 / 3: off-chip buffer, -1: CSR (read-only)
+        if (n0 <= kSerHalf) {
+            block_copy(a.data + lo_c, n0, half0);
+            cur = half0;
+            where = 0;
         } else {
-            w.ptr = a.data + lo_c;
-            w.half = -1;
+            cur = a.data + lo_c;
+            where = -1;
         }
-        while (w.len > 0) {
-            const unsigned long long v = a.own[(long long)w.ptr[0] - a.row_base];
+        int len = n0;
+        __syncthreads();
+        bool have_next = false;
+        unsigned long long next_v = 0;
+        int next_nb = 0;
+        while (len > 0) {
+            const unsigned p_old = cur[0];
+            unsigned long long v;
+            int nb;
+            if (have_next) {
+                v = next_v;
+                nb = next_nb;
+            } else {
+                v = a.own[(long long)p_old - a.row_base];
+                nb = v != kNoOwner ? (int)a.pool[v & 0xFFFFFFFFull] : 0;
+            }
             if (v == kNoOwner) break;
-            if (!xor_step<kHalfB, kStageB>(w, a.pool + (v & 0xFFFFFFFFull), arena, a, al,
-                                           kSerialMaxCol))
-                return;
+            const unsigned* ob = a.pool + (v & 0xFFFFFFFFull);
+            const unsigned* B = ob + 1;
+            if (nb <= kSerStage) {
+                block_copy(ob + 1, nb, stage);
+                B = stage;
+            }
+            const int need = len + nb;
+            unsigned* out;
+            int out_where;
+            if ((where == 0 || where == 1) && need <= kSerHalf) {
+                out_where = 1 - where;
+                out = smem + out_where * kSerHalf;
+            } else {
+                offchip++;
+                const int bi = where == 2 ? 1 : 0;
+                if (big_cap[bi] < need) {
+                    long long ncap = 16384;
+                    while (ncap < need) ncap *= 2;
+                    if (t == 0) {
+                        const unsigned long long at = atomicAdd(a.pool_ptr, (unsigned long long)ncap);
+                        sh.big_off[bi] = at;
+                        sh.flag = (at + ncap > a.pool_cap) ? 1 : 0;
+                        if (sh.flag) atomicExch(&a.status[0], 1);
+                    }
+                    __syncthreads();
+                    if (sh.flag) return;  // uniform: every thread read the same flag
+                    big_off[bi] = sh.big_off[bi];
+                    big_cap[bi] = ncap;
+                }
+                out = a.pool + big_off[bi];
+                out_where = 2 + bi;
+            }
+            __syncthreads();  // staged owner visible
+            int nl = block_xor(cur, len, B, nb, out, kSerialMaxCol, sh, a);
+            next_v = sh.next_v;
+            next_nb = sh.next_nb;
+            have_next = true;
+            if (out_where >= 2 && nl <= kSerHalf) {  // back on chip
+                const int h = (where == 0 || where == 1) ? 1 - where : 0;
+                block_copy(out, nl, smem + h * kSerHalf);
+                __syncthreads();
+                out = smem + h * kSerHalf;
+                out_where = h;
+            }
+            cur = out;
+            where = out_where;
+            len = nl;
             nxor++;
+            if (len > 0 && !(cur[0] > p_old)) {
+                if (t == 0) atomicExch(&a.status[0], 3);
+                return;
+            }
         }
-        if (w.len == 0) {
-            if (lane == 0) a.pivot[c] = -1;
+        if (len == 0) {
+            if (t == 0) a.pivot[c] = -1;
             continue;
         }
-        const unsigned piv = w.ptr[0];
-        if (lane == 0) a.pivot[c] = (int)piv;
-        longest = max(longest, w.len);
-        if (used + w.len <= cap) {
+        const unsigned piv = cur[0];
+        if (t == 0) a.pivot[c] = (int)piv;
+        longest = max(longest, len);
+        if (used + len <= cap) {
             const long long at = base + stored;
-            if (lane == 0) a.pool[at] = (unsigned)w.len;
-            wave_copy(w.ptr, w.len, a.pool + at + 1);
-            if (lane == 0)
+            if (t == 0) a.pool[at] = (unsigned)len;
+            block_copy(cur, len, a.pool + at + 1);
+            __threadfence_block();
+            __syncthreads();
+            if (t == 0)
                 a.own[(long long)piv - a.row_base] =
                     ((unsigned long long)(c + 1) << 32) | (unsigned long long)at;
-            used += w.len;
-            stored += w.len + 1;
+            used += len;
+            stored += len + 1;
         }
-        wave_sync();
+        __threadfence_block();
+        __syncthreads();
     }
-    if (lane == 0) {
+    if (t == 0) {
         if (nxor) atomicAdd(&a.status[2], nxor);
+        if (offchip) atomicAdd(&a.status[4], offchip);
         if (longest) atomicMax(&a.status[3], longest);
         atomicMax(&a.status[1], 1);
     }
@@ -785,9 +988,11 @@ extern "C" int fph_reduce_serial(int64_t K, const int32_t* col_off, const uint32
         FPH_LAUNCH_CHECK();
         FPH_CHECK_HIP(rocprim::radix_sort_keys_desc(sort_tmp, sort_sz, keys, order, (unsigned)K,
                                                     0, 32, s));
-        const size_t lds = kArenaB * 4;
-        k_reduce_serial<<<(unsigned)nseg, FPH_WAVE, lds, s>>>(a, order, res_count, seg_cols,
-                                                             (const long long*)seg_pool);
+        FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_reduce_serial,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)kSerLds));
+        k_reduce_serial<<<(unsigned)nseg, kSerThreads, kSerLds, s>>>(
+            a, order, res_count, seg_cols, (const long long*)seg_pool);
         FPH_LAUNCH_CHECK();
     }
     k_reduce_fini<<<1, 64, 0, s>>>(status, a.pool_ptr);

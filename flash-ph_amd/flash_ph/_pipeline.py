@@ -322,8 +322,9 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
             base += _round32(cap + kc + 64)
         pool_start = base
         seg_pool_t = torch.tensor(seg_pool, dtype=torch.int64).to(device)
-        # + two growing off-chip buffers per segment
-        pool_words = pool_start + max(_POOL_MIN_WORDS, nseg * 2 * 16384)
+        # + two growing off-chip buffers per segment (used only by columns
+        # longer than the on-chip halves)
+        pool_words = pool_start + max(_POOL_MIN_WORDS, nseg * 262144)
     out = _empty(torch.float32, device, max(K, 1), 2)
     while True:
         check(lib.fph_apparent(K, col_base, ptr(off), ptr(data), ptr(skip), ptr(row_cmf),
