@@ -83,7 +83,7 @@ __global__ void k_diag_write(const unsigned long long* __restrict__ keys, long l
                              const int* __restrict__ row_edge, const float* __restrict__ edge_filt,
                              int ess_asc_all, const int* __restrict__ seg_cols,
                              const unsigned char* __restrict__ seg_ess_asc, int nseg,
-                             float* __restrict__ out) {
+                             float* __restrict__ out, int* __restrict__ out_edges) {
     long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
     const int s = segment_of(seg_cols, nseg, k);
@@ -95,9 +95,13 @@ __global__ void k_diag_write(const unsigned long long* __restrict__ keys, long l
     const unsigned ord = (unsigned)(key & 0x7FFFFFFFull);
     const long long c = (group == 0 || (group == 2 && asc)) ? (long long)ord : 0x7FFFFFFFLL - ord;
     const int be = col_edge ? col_edge[c] : (int)c;
+    const int de = group == 2 ? -1 : row_edge[(long long)pivot[c] - row_base];
     out[2 * k] = edge_filt[be];
-    out[2 * k + 1] = group == 2 ? __builtin_inff()
-                                : edge_filt[row_edge[(long long)pivot[c] - row_base]];
+    out[2 * k + 1] = de < 0 ? __builtin_inff() : edge_filt[de];
+    if (out_edges) {
+        out_edges[2 * k] = be;
+        out_edges[2 * k + 1] = de;
+    }
 }
 
 __global__ void k_zero_counts(int* p, int n) {
@@ -113,7 +117,8 @@ extern "C" int fph_diagram(int64_t K, const uint8_t* kind, const int32_t* col_of
                            const float* edge_filt, int64_t E, int ess_ascending,
                            const int32_t* seg_cols, const int32_t* seg_edges,
                            const uint8_t* seg_ess_asc, int64_t nseg, float* out,
-                           int32_t* counts, void* ws, size_t* ws_bytes, void* stream) {
+                           int32_t* out_edges, int32_t* counts, void* ws, size_t* ws_bytes,
+                           void* stream) {
     hipStream_t s = (hipStream_t)stream;
     if (K < 0 || !ws_bytes || !counts || nseg < 1) return FPH_ERR_ARG;
     if (nseg > 1 && !seg_cols) return FPH_ERR_ARG;
@@ -154,7 +159,7 @@ extern "C" int fph_diagram(int64_t K, const uint8_t* kind, const int32_t* col_of
     }
     k_diag_write<<<fph_blocks(K, 256), 256, 0, s>>>(sorted, K, counts, pivot, col_edge, row_base,
                                                    row_edge, edge_filt, ess_ascending, seg_cols,
-                                                   seg_ess_asc, (int)nseg, out);
+                                                   seg_ess_asc, (int)nseg, out, out_edges);
     FPH_LAUNCH_CHECK();
     return FPH_OK;
 }

@@ -1,0 +1,242 @@
+// Stage (a) on the matrix cores, for wide points (d >= 32).
+//
+// |x - y|^2 = |x|^2 + |y|^2 - 2 x.y, and the Gram products x.y of 128 x 128
+// point tiles run on v_mfma_f32_32x32x16_bf16 with every coordinate split
+// into two bf16 parts, x = xh + xl (+ r):
+//     x.y ~ xh.yh + xh.yl + xl.yh        (FP32 accumulation).
+// The approximation only screens: a pair is a candidate when
+//     approx < t^2 + a (|x|^2 + |y|^2) + b,
+// with a and b bounding every error term (below), and each candidate is
+// recomputed in the reference's direct FP32 form (fma(diff, diff, acc) over
+// k = 0..d-1, the same chain as fph_edge_screen) and kept iff that value
+// is < t^2.  So the output -- pairs, dsq bits -- is exactly fph_edge_screen's.
+//
+// Error bound (u = 2^-24, bf16 rounding to nearest even, unit 2^-8):
+//   |x - xh| <= 2^-8 |x|;  x - xh is exact in FP32 (Sterbenz);
+//   |(x - xh) - xl| <= 2^-16 |x|, so x = xh + xl + r with |r| <= 2^-16 |x|.
+//   Dropped terms per coordinate (xl yl, (xh + xl) ry, rx (yh + yl), rx ry)
+//   <= 3.1 * 2^-16 |x_k| |y_k|; bf16 x bf16 products are exact in FP32 and
+//   the 3 dpad of them accumulate with error <= gamma(3 dpad) * 1.02 sum
+//   |x_k y_k| whatever the accumulation order.  With sum |x_k y_k| <=
+//   |x| |y| <= (|x|^2 + |y|^2) / 2:  |2 (x.y - G)| <= cG (|x|^2 + |y|^2),
+//   cG = 3.1 * 2^-16 + 1.02 gamma(3 dpad).
+//   The FP32 norms add gamma(d) (|x|^2 + |y|^2), the two FP32 operations of
+//   approx 4u (|x|^2 + |y|^2), and the direct form differs from the true
+//   distance by <= gamma(d) dsq ~ gamma(d) t^2 near the threshold:
+//   a = 1.25 (cG + gamma(d) + 4u),  b = 1.1 gamma(d) t^2.
+#include "fph_common.h"
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+constexpr int kMT = 128;       // points per tile side
+constexpr int kThreadsM = 256; // 4 waves, each a 64 x 64 quarter of the tile
+
+inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+__device__ __forceinline__ unsigned short bf16_rne(float x) {
+    unsigned u;
+    memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ float bf16_to_f32(unsigned short h) {
+    const unsigned u = (unsigned)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+// One thread per (point, coordinate) of the padded layout: xh, xl (bf16,
+// n x dpad, zero padded) and, per point, |x|^2 as an FP32 FMA chain.
+__global__ void k_split_bf16(const float* __restrict__ pts, long long n, int d, int dpad,
+                             unsigned short* __restrict__ xh, unsigned short* __restrict__ xl) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * dpad) return;
+    const long long i = t / dpad;
+    const int k = (int)(t - i * dpad);
+    const float x = k < d ? pts[i * d + k] : 0.0f;
+    const unsigned short h = bf16_rne(x);
+    const float r = x - bf16_to_f32(h);  // exact
+    xh[t] = h;
+    xl[t] = bf16_rne(r);
+}
+
+__global__ void k_norms(const float* __restrict__ pts, long long n, int d,
+                        float* __restrict__ nrm) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float acc = 0.0f;
+    for (int k = 0; k < d; k++) {
+        const float x = pts[i * d + k];
+        acc = __builtin_fmaf(x, x, acc);
+    }
+    nrm[i] = acc;
+}
+
+__device__ __forceinline__ void tile_coords(long long t, int* I, int* J) {
+    long long j = (long long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((j + 1) * (j + 2) / 2 <= t) j++;
+    while (j * (j + 1) / 2 > t) j--;
+    *J = (int)j;
+    *I = (int)(t - j * (j + 1) / 2);
+}
+
+__global__ __launch_bounds__(kThreadsM) void k_edge_screen_mfma(
+    const float* __restrict__ pts, const bf16x8* __restrict__ xh,
+    const bf16x8* __restrict__ xl, const float* __restrict__ nrm, int n, int d, int dpad,
+    float thr, float marg_a, float marg_b, long long ntiles, unsigned long long* __restrict__ keys,
+    unsigned long long* __restrict__ vals, long long cap,
+    unsigned long long* __restrict__ count, unsigned long long* __restrict__ ncand) {
+    __shared__ float s_nr[kMT], s_nc[kMT];
+    // XCD-aware order: the 8 XCDs take blockIdx round robin; give each XCD
+    // a contiguous run of tiles (shared row blocks stay in its L2).
+    const long long nb = gridDim.x;
+    const long long per_xcd = (nb + 7) / 8;
+    long long t = (long long)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    const bool live = t < ntiles;
+    if (!live) t = 0;
+    int I, J;
+    tile_coords(t, &I, &J);
+    const int tid = threadIdx.x;
+    if (tid < kMT) {
+        const int i = min(I * kMT + tid, n - 1);
+        s_nr[tid] = nrm[i];
+    } else {
+        const int j = min(J * kMT + tid - kMT, n - 1);
+        s_nc[tid - kMT] = nrm[j];
+    }
+    __syncthreads();
+    if (!live) return;
+    const int w = tid / FPH_WAVE, lane = fph_lane();
+    const int rsub = (w >> 1) * 64, csub = (w & 1) * 64;
+    if (I == J && rsub > csub) return;  // entirely below the diagonal
+    const int row0 = I * kMT + rsub, col0 = J * kMT + csub;
+    const int vrow = dpad / 8;  // bf16x8 vectors per point
+    const int r = lane & 31, kh = lane >> 5;
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc[a][b][v] = 0.0f;
+    const long long ra0 = (long long)min(row0 + r, n - 1) * vrow;
+    const long long ra1 = (long long)min(row0 + 32 + r, n - 1) * vrow;
+    const long long cb0 = (long long)min(col0 + r, n - 1) * vrow;
+    const long long cb1 = (long long)min(col0 + 32 + r, n - 1) * vrow;
+    for (int ks = 0; ks < dpad / 16; ks++) {
+        const int o = ks * 2 + kh;
+        const bf16x8 ah0 = xh[ra0 + o], al0 = xl[ra0 + o];
+        const bf16x8 ah1 = xh[ra1 + o], al1 = xl[ra1 + o];
+        const bf16x8 bh0 = xh[cb0 + o], bl0 = xl[cb0 + o];
+        const bf16x8 bh1 = xh[cb1 + o], bl1 = xl[cb1 + o];
+        acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bh0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bh1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bh0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bh1, acc[1][1], 0, 0, 0);
+        acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bl0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bl1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bl0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bl1, acc[1][1], 0, 0, 0);
+        acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al0, bh0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al0, bh1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al1, bh0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al1, bh1, acc[1][1], 0, 0, 0);
+    }
+    // Epilogue: accumulator slot v of lane l is row 8 (v / 4) + 4 (l / 32)
+    // + v % 4, column l % 32 of its 32 x 32 block.
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int li = rsub + 32 * a + 8 * (v >> 2) + 4 * kh + (v & 3);
+                const int lj = csub + 32 * b + r;
+                const int i = I * kMT + li, j = J * kMT + lj;
+                const float ni = s_nr[li], nj = s_nc[lj];
+                const float approx = (ni + nj) - 2.0f * acc[a][b][v];
+                const bool cand = i < j && j < n &&
+                                  approx < thr + marg_a * (ni + nj) + marg_b;
+                if (cand) {  // rare: recompute in the direct form
+                    atomicAdd(ncand, 1ull);
+                    const float* xi = pts + (long long)i * d;
+                    const float* xj = pts + (long long)j * d;
+                    float e = 0.0f;
+                    for (int k = 0; k < d; k++) {
+                        const float diff = xi[k] - xj[k];
+                        e = __builtin_fmaf(diff, diff, e);
+                    }
+                    if (e < thr) {
+                        const unsigned long long at = atomicAdd(count, 1ull);
+                        if (at < (unsigned long long)cap) {
+                            unsigned bits;
+                            memcpy(&bits, &e, 4);
+                            const unsigned long long uj = (unsigned long long)j;
+                            const unsigned long long colex = uj * (uj - 1) / 2 + (unsigned)i;
+                            keys[at] = ((unsigned long long)bits << 32) | (colex & 0xFFFFFFFFull);
+                            vals[at] = (uj << 32) | (unsigned long long)i;
+                        }
+                    }
+                }
+            }
+}
+
+__global__ void k_zero2(unsigned long long* a, unsigned long long* b) {
+    if (threadIdx.x == 0) {
+        a[0] = 0ull;
+        b[0] = 0ull;
+    }
+}
+
+double gamma_n(int m) {
+    const double u = 1.0 / 16777216.0;
+    return m * u / (1.0 - m * u);
+}
+
+}  // namespace
+
+extern "C" int fph_edge_screen_mfma(const float* pts, int64_t n, int d, float thr_sq,
+                                    uint64_t* keys, uint64_t* vals, int64_t cap,
+                                    uint64_t* count_dev, uint64_t* cand_dev, void* ws,
+                                    size_t* ws_bytes, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (n < 0 || d < 1 || cap < 0 || !count_dev || !cand_dev || !ws_bytes) return FPH_ERR_ARG;
+    if (n > 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
+    const int dpad = (d + 15) / 16 * 16;
+    const size_t half = align_up(2 * (size_t)std::max<int64_t>(n, 1) * dpad);
+    const size_t need = 2 * half + align_up(4 * (size_t)std::max<int64_t>(n, 1));
+    if (!ws) { *ws_bytes = need; return FPH_OK; }
+    if (*ws_bytes < need) return FPH_ERR_ARG;
+    char* w = (char*)ws;
+    unsigned short* xh = (unsigned short*)w;
+    unsigned short* xl = (unsigned short*)(w + half);
+    float* nrm = (float*)(w + 2 * half);
+    k_zero2<<<1, 64, 0, s>>>((unsigned long long*)count_dev, (unsigned long long*)cand_dev);
+    FPH_LAUNCH_CHECK();
+    if (n < 2) return FPH_OK;
+    k_split_bf16<<<fph_blocks(n * dpad, 256), 256, 0, s>>>(pts, n, d, dpad, xh, xl);
+    FPH_LAUNCH_CHECK();
+    k_norms<<<fph_blocks(n, 256), 256, 0, s>>>(pts, n, d, nrm);
+    FPH_LAUNCH_CHECK();
+    const double cg = 3.1 * std::ldexp(1.0, -16) + 1.02 * gamma_n(3 * dpad);
+    const double u = std::ldexp(1.0, -24);
+    const float marg_a = (float)(1.25 * (cg + gamma_n(d) + 4.0 * u));
+    const float marg_b = (float)(1.1 * gamma_n(d) * (double)thr_sq);
+    const long long nts = (n + kMT - 1) / kMT;
+    const long long ntiles = nts * (nts + 1) / 2;
+    if (ntiles > 0x7FFFFFF0LL) return FPH_ERR_LIMIT;
+    // 8 x ceil(ntiles / 8) blocks: the XCD remap is a bijection onto
+    // [0, 8 ceil(ntiles / 8)); blocks past ntiles exit.
+    const long long nblk = (ntiles + 7) / 8 * 8;
+    k_edge_screen_mfma<<<(unsigned)nblk, kThreadsM, 0, s>>>(
+        pts, (const bf16x8*)xh, (const bf16x8*)xl, nrm, (int)n, d, dpad, thr_sq, marg_a, marg_b,
+        ntiles, (unsigned long long*)keys, (unsigned long long*)vals, cap,
+        (unsigned long long*)count_dev, (unsigned long long*)cand_dev);
+    FPH_LAUNCH_CHECK();
+    return FPH_OK;
+}

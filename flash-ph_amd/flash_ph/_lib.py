@@ -33,6 +33,7 @@ _PSZ = ctypes.POINTER(ctypes.c_size_t)
 SIGNATURES = {
     "fph_version": (ctypes.c_char_p, []),
     "fph_edge_screen": (_I32, [_P, _I64, _I32, _F32, _P, _P, _I64, _P, _P]),
+    "fph_edge_screen_mfma": (_I32, [_P, _I64, _I32, _F32, _P, _P, _I64, _P, _P, _P, _PSZ, _P]),
     "fph_edge_sort": (_I32, [_P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _PSZ, _P]),
     "fph_vertex_csr": (_I32, [_P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _PSZ, _P]),
     "fph_boruvka_rounds": (_I32, [_I64]),
@@ -50,7 +51,7 @@ SIGNATURES = {
     "fph_reduce_serial": (_I32, [_I64, _P, _P, _I64, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P,
                                  _P, _P, _I64, _P, _SZ, _P, _P]),
     "fph_diagram": (_I32, [_I64, _P, _P, _P, _P, _P, _I64, _P, _P, _I64, _I32, _P, _P, _P, _I64,
-                           _P, _P, _P, _PSZ, _P]),
+                           _P, _P, _P, _P, _PSZ, _P]),
     "fph_clear_pivots": (_I32, [_I64, _P, _P, _I64, _I64, _P, _P]),
     "fph_tet_count": (_I32, [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _PSZ, _P]),
     "fph_tet_emit": (_I32, [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
@@ -63,6 +64,8 @@ SIGNATURES = {
                                        _P, _P, _P, _P, _PSZ, _P]),
     "fph_enclosing_radius_batch": (_I32, [_P, _P, _P, _P, _P, _I64, _I64, _I32, _P, _P, _PSZ,
                                           _P]),
+    "fph_backward_scatter": (_I32, [_P, _P, _P, _P, _I64, _I64, _P, _I64, _P, _P, _P, _P, _P,
+                                    _I64, _P, _P, _PSZ, _P]),
     "fph_edge_screen_batch": (_I32, [_P, _P, _P, _P, _P, _P, _I64, _I64, _P, _P, _I64, _P, _P]),
     "fph_edge_sort_batch": (_I32, [_P, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _PSZ, _P]),
     "fph_h0_diagram_batch": (_I32, [_P, _P, _I64, _P, _P, _I64, _I64, _P, _P, _PSZ, _P]),

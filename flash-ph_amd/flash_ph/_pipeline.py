@@ -71,6 +71,22 @@ class StageTimer:
 TIMER = StageTimer()
 
 
+# Screening kernel: "auto" picks the split-bf16 matrix-core screen for wide
+# points (d >= 32) and large clouds, the direct FP32 screen otherwise; both
+# give identical edges.  "direct" / "mfma" force one.
+SCREEN = "auto"
+_MFMA_MIN_D = 32
+_MFMA_MIN_N = 4096
+
+
+def use_mfma_screen(n: int, d: int) -> bool:
+    if SCREEN == "mfma":
+        return True
+    if SCREEN == "direct":
+        return False
+    return d >= _MFMA_MIN_D and n >= _MFMA_MIN_N
+
+
 def threshold_sq_f32(max_edge_length: float) -> float:
     """t*t in Python double rounded to FP32, as the reference's Triton scalar
     argument (rips.py:125, edge_kernel.py:125)."""
@@ -130,6 +146,10 @@ class Complex:
     # batches: per-cloud edge ranges (device, C+1) and vertex offsets (host)
     edge_off: torch.Tensor | None = None
     vbase_host: list | None = None
+    # gradients: per diagram row, the birth and death edges (-1: none)
+    want_edges: bool = False
+    h1_edges: object = None
+    h2_edges: object = None
 
 
 def build_edges(pts: torch.Tensor, max_edge_length: float) -> Complex:
@@ -141,15 +161,23 @@ def build_edges(pts: torch.Tensor, max_edge_length: float) -> Complex:
     cx = Complex(n=n, device=dev)
     thr = threshold_sq_f32(max_edge_length)
     cap = max(_EDGE_CAP_MIN, _EDGE_CAP_PER_POINT * n)
-    cnt = _empty(torch.int64, dev, 1)
+    cnt = _empty(torch.int64, dev, 2)
+    mfma = use_mfma_screen(n, d)
     while True:
         keys = _empty(torch.int64, dev, cap)
         vals = _empty(torch.int64, dev, cap)
         tok = TIMER.start("edge_screen", dev)
-        check(lib.fph_edge_screen(ptr(pts), n, d, thr, ptr(keys), ptr(vals), cap, ptr(cnt), s),
-              "fph_edge_screen")
+        if mfma:
+            call_ws("fph_edge_screen_mfma",
+                    (ptr(pts), n, d, thr, ptr(keys), ptr(vals), cap, ptr(cnt),
+                     cnt.data_ptr() + 8), (s,), dev)
+        else:
+            check(lib.fph_edge_screen(ptr(pts), n, d, thr, ptr(keys), ptr(vals), cap, ptr(cnt),
+                                      s), "fph_edge_screen")
         TIMER.stop(tok)
-        E = int(cnt.item())
+        E = int(cnt[0].item())
+        cx.info["screen"] = "mfma" if mfma else "direct"
+        cx.info["screen_candidates"] = cnt[1] if mfma else None
         if E <= cap:
             break
         cap = E
@@ -268,8 +296,12 @@ def build_h1(cx: Complex, max_triangles: int = MAX_TRIANGLES, semantics: str = "
     cx.kind1, cx.pivot1, h1, stats = _reduce_and_diagram(
         K=E, col_base=0, off=cx.cof_off, data=cx.cof_data, skip=cx.mst_mask, row_cmf=cx.tri_cmf,
         row_base=E, nrows=T, C=C, col_edge=None, row_edge=cx.tri_e, edge_filt=cx.edge_filt,
-        E=E, ess_ascending=(T == 0), device=dev, segs=segs, ref_sizes=ref_sizes)
+        E=E, ess_ascending=(T == 0), device=dev, segs=segs, ref_sizes=ref_sizes,
+        want_edges=cx.want_edges)
     cx.h1 = h1 if segs is not None else h1[0]
+    if cx.want_edges:
+        re = stats.pop("row_edges")
+        cx.h1_edges = re if segs is not None else re[0]
     cx.info["h1_stats"] = stats
 
 
@@ -285,7 +317,7 @@ def _round32(x: int) -> int:
 
 def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, C, col_edge,
                         row_edge, edge_filt, E, ess_ascending, device, segs=None,
-                        ref_sizes=None):
+                        ref_sizes=None, want_edges=False):
     """Apparent pairs, residual reduction and the diagram of one dimension.
 
     ``segs`` (batches) = dict(cols=, edges=, ess_asc= device tensors,
@@ -326,6 +358,7 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
         # longer than the on-chip halves)
         pool_words = pool_start + max(_POOL_MIN_WORDS, nseg * 262144)
     out = _empty(torch.float32, device, max(K, 1), 2)
+    out_edges = _empty(torch.int32, device, max(K, 1), 2) if want_edges else None
     while True:
         check(lib.fph_apparent(K, col_base, ptr(off), ptr(data), ptr(skip), ptr(row_cmf),
                                row_base, nrows, ptr(kind), ptr(own), ptr(pivot), ptr(res_list),
@@ -350,7 +383,7 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
         call_ws("fph_diagram",
                 (K, ptr(kind), ptr(off), ptr(data), ptr(pivot), ptr(col_edge), row_base,
                  ptr(row_edge), ptr(edge_filt), E, int(bool(ess_ascending)), ptr(seg_cols),
-                 ptr(seg_edges), ptr(seg_ess_asc), nseg, ptr(out), counts_p),
+                 ptr(seg_edges), ptr(seg_ess_asc), nseg, ptr(out), ptr(out_edges), counts_p),
                 (s,), device)
         st = info32.tolist()
         if REDUCE_LOG is not None:
@@ -366,6 +399,8 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
     counts = st[8:8 + nseg]
     starts = [0] if segs is None else segs["cols_host"][:-1]
     dgms = [out[a:a + c] for a, c in zip(starts, counts)]
+    if want_edges:
+        stats["row_edges"] = [out_edges[a:a + c] for a, c in zip(starts, counts)]
     return kind, pivot, dgms, stats
 
 
@@ -405,15 +440,19 @@ def build_h2(cx: Complex) -> None:
     cx.kind2, cx.pivot2, h2, stats = _reduce_and_diagram(
         K=T, col_base=E, off=cx.cof2_off, data=cx.cof2_data, skip=cx.cleared,
         row_cmf=cx.tet_cmf, row_base=E + T, nrows=Q, C=C2, col_edge=cx.tri_e,
-        row_edge=cx.tet_e, edge_filt=cx.edge_filt, E=E, ess_ascending=False, device=dev)
+        row_edge=cx.tet_e, edge_filt=cx.edge_filt, E=E, ess_ascending=False, device=dev,
+        want_edges=cx.want_edges)
     cx.h2 = h2[0]
+    if cx.want_edges:
+        cx.h2_edges = stats.pop("row_edges")[0]
     cx.info["h2_stats"] = stats
 
 
 def compute(pts: torch.Tensor, max_edge_length: float, max_dim: int,
-            semantics: str = "exact") -> tuple[list, Complex]:
+            semantics: str = "exact", want_edges: bool = False) -> tuple[list, Complex]:
     """Diagrams [H0, ..., H_max_dim] of a CUDA point cloud with n >= 2."""
     cx = build_edges(pts, max_edge_length)
+    cx.want_edges = want_edges
     build_mst(cx)
     out = [cx.h0]
     if max_dim >= 1:
@@ -557,13 +596,14 @@ def build_mst_batch(L: BatchLayout, cx: Complex) -> None:
 
 
 def compute_batch(L: BatchLayout, thr_sq: torch.Tensor, max_dim: int,
-                  max_triangles: int = MAX_TRIANGLES,
-                  semantics: str = "exact") -> tuple[list, Complex]:
+                  max_triangles: int = MAX_TRIANGLES, semantics: str = "exact",
+                  want_edges: bool = False) -> tuple[list, Complex]:
     """Diagrams [[H0, H1?] per cloud] of a packed batch (max_dim <= 1)."""
     if L.N == 0:
         empty = torch.empty(0, 2, dtype=torch.float32, device=L.pts.device)
         return [[empty] * (max_dim + 1) for _ in range(L.C)], None
     cx = build_edges_batch(L, thr_sq)
+    cx.want_edges = want_edges
     build_mst_batch(L, cx)
     vb = L.vbase_host
     h0s = [cx.h0[vb[c]:vb[c + 1]] for c in range(L.C)]

@@ -127,7 +127,11 @@ def rips_persistence_batch(
         thr_sq = torch.tensor([_pipeline.threshold_sq_f32(t) for t in ts], dtype=torch.float32,
                               device=dev)
     semantics = "exact" if backend == "exact" else "reference"
-    dgms, _ = _pipeline.compute_batch(L, thr_sq, max_dim, semantics=semantics)
+    if torch.is_grad_enabled() and any(getattr(c, "requires_grad", False) for c in clouds):
+        from flash_ph.autograd import rips_persistence_batch_diff
+        dgms = rips_persistence_batch_diff(pts, thr_sq, max_dim, semantics)
+    else:
+        dgms, _ = _pipeline.compute_batch(L, thr_sq, max_dim, semantics=semantics)
     if out_dev != dev:
         dgms = [[d.to(out_dev) for d in per] for per in dgms]
     return dgms

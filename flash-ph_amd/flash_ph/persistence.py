@@ -82,7 +82,9 @@ def rips_persistence(
     list[Tensor]
         [H0, H1, ...], each (K, 2) float32 on the input's device: H0 has one
         (0, inf) row per connected component; H1/H2 include essential
-        (birth, inf) rows; rows sorted stably by birth.
+        (birth, inf) rows; rows sorted stably by birth.  When ``points``
+        requires grad, the diagrams carry gradients back to the points
+        (each finite value is an edge length; see flash_ph.autograd).
 
     Reference: persistence.py:68-206.
     """
@@ -132,6 +134,12 @@ def rips_persistence(
     pts = points.float().contiguous()
     if pts.device.type != "cuda":
         pts = pts.cuda()
+
+    if torch.is_grad_enabled() and points.requires_grad and backend != "giotto":
+        from flash_ph.autograd import rips_persistence_diff
+        semantics = "exact" if backend == "exact" else "reference"
+        out = rips_persistence_diff(pts, max_edge_length, max_dim, semantics)
+        return out if device.type == "cuda" else [d.to(device) for d in out]
 
     if max_dim == 1 and backend == "giotto":
         from flash_ph.rips import RipsFiltration

@@ -52,6 +52,22 @@ int fph_edge_screen(const float *pts, int64_t n, int d, float thr_sq, uint64_t *
                     uint64_t *vals, int64_t cap, uint64_t *count_dev, void *stream);
 
 /*
+ * fph_edge_screen for wide points on the matrix cores: Gram products of
+ * 128 x 128 point tiles with split-bf16 operands (x = xh + xl) on
+ * v_mfma_f32_32x32x16_bf16 with FP32 accumulation screen every pair against
+ * t^2 plus a rigorous error margin; candidates are recomputed in the direct
+ * FP32 form and kept iff < thr_sq.  Same outputs as fph_edge_screen (same
+ * pairs, same dsq bits, unordered); *cand_dev = candidates recomputed
+ * (>= edges; the difference is the margin band).  Workspace: the bf16
+ * splits and norms (4 n ceil16(d) + 4 n bytes).
+ * Replaces kernels/edge_kernel.py:23-224 (the reference computes the direct
+ * form for every pair).
+ */
+int fph_edge_screen_mfma(const float *pts, int64_t n, int d, float thr_sq, uint64_t *keys,
+                         uint64_t *vals, int64_t cap, uint64_t *count_dev, uint64_t *cand_dev,
+                         void *ws, size_t *ws_bytes, void *stream);
+
+/*
  * Sort the E screened pairs by key (ties, possible only for n > 92682, in
  * the reference's emission order) and unpack them into edge_i, edge_j,
  * edge_dsq and edge_filt = sqrt(edge_dsq) (correctly rounded; may be NULL).
@@ -221,7 +237,10 @@ int fph_reduce_serial(int64_t K, const int32_t *col_off, const uint32_t *col_dat
  * capacity.  Segments (seg_cols: nseg+1 column offsets, seg_edges: nseg+1
  * edge offsets, seg_ess_asc: per-segment flag; all NULL for one segment
  * spanning [0, K) and edges [0, E)) are the clouds of a batch: segment s
- * writes counts[s] rows at out[seg_cols[s] ...].
+ * writes counts[s] rows at out[seg_cols[s] ...].  out_edges (K x 2 int32,
+ * may be NULL): per row the edge whose length is the birth and the one
+ * whose length is the death (-1 for an infinite death) -- the diameter
+ * edges of the critical simplices, for gradients.
  * Replaces reduce_h1_gpu.py:495-502, 574-604 (and persistence.py:202-205
  * for H2).
  */
@@ -229,8 +248,8 @@ int fph_diagram(int64_t K, const uint8_t *kind, const int32_t *col_off,
                 const uint32_t *col_data, const int32_t *pivot, const int32_t *col_edge,
                 int64_t row_base, const int32_t *row_edge, const float *edge_filt, int64_t E,
                 int ess_ascending, const int32_t *seg_cols, const int32_t *seg_edges,
-                const uint8_t *seg_ess_asc, int64_t nseg, float *out, int32_t *counts,
-                void *ws, size_t *ws_bytes, void *stream);
+                const uint8_t *seg_ess_asc, int64_t nseg, float *out, int32_t *out_edges,
+                int32_t *counts, void *ws, size_t *ws_bytes, void *stream);
 
 /* ---- H2: clearing cascade, tetrahedra, cofacet columns ---------------- */
 
@@ -362,6 +381,26 @@ int fph_edge_sort_batch(const uint64_t *keys_in, const uint64_t *vals_in, int64_
 int fph_h0_diagram_batch(const uint8_t *mst_mask, const float *edge_dsq, int64_t E,
                          const int32_t *edge_off, const int64_t *vbase, int64_t C, int64_t N,
                          float *h0, void *ws, size_t *ws_bytes, void *stream);
+
+/* ---- gradients -------------------------------------------------------- */
+
+/*
+ * Gradient of a loss on diagram entries with respect to the points (same
+ * batch layout as fph_edge_screen_batch; one cloud: C = 1, foff = {0, n d},
+ * vbase = {0, n}).  Entry m has upstream gradient contrib_val[m] and sits on
+ * edge contrib_edge[m] (-1: none, e.g. an H0 birth or an infinite death),
+ * as given by fph_diagram's out_edges and the H0 forest edges.  With
+ * w_e = (sum of the entries on e) / edge_filt[e]:
+ *   grad_pts[v] = sum over the CSR edges e = (v, u) of w_e (x_v - x_u).
+ * Sums run in a fixed order (deterministic).  grad_pts has the layout of
+ * pts.  No counterpart in the reference (which has no gradient path).
+ */
+int fph_backward_scatter(const float *pts, const int64_t *foff, const int32_t *dims,
+                         const int64_t *vbase, int64_t C, int64_t N, const float *edge_filt,
+                         int64_t E, const int32_t *adj_ptr, const int32_t *adj_idx,
+                         const int32_t *adj_rank, const int32_t *contrib_edge,
+                         const float *contrib_val, int64_t M, float *grad_pts, void *ws,
+                         size_t *ws_bytes, void *stream);
 
 #ifdef __cplusplus
 }

@@ -98,3 +98,8 @@ def batch_uneven(C, seed, nmin=200, nmax=5000, dims=(4, 10, 50), noise=0.05):
         x = x + rng.normal(0, noise, x.shape)
         clouds.append(x.astype(np.float32))
     return clouds
+
+
+def clifford_batch(C, n, seed, noise=0.02):
+    """C Clifford-torus clouds (cos t, sin t, cos p, sin p)/sqrt(2) + noise."""
+    return [clifford(n, seed + c, noise) for c in range(C)]
