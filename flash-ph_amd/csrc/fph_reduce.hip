@@ -581,6 +581,7 @@ constexpr int kSerialMaxCol = 1048576;
 // fits, the owner column is staged into LDS while it fits, and longer lists
 // are merged in two reusable off-chip buffers.
 constexpr int kSerThreads = 256;
+constexpr int kSegPoolWords = 5;  // per segment: store base, capacity, row0, rows, bitmap
 constexpr int kSerWaves = kSerThreads / FPH_WAVE;
 constexpr int kSerHalf = 12288;   // words per LDS half of the working column
 constexpr int kSerStage = 8192;   // words of LDS staging for the owner column
@@ -699,9 +700,164 @@ __device__ __forceinline__ void block_copy(const unsigned* src, int n, unsigned*
     for (int i = threadIdx.x; i < n; i += kSerThreads) dst[i] = src[i];
 }
 
+// ---- working column as a bitmap --------------------------------------
+//
+// The segment's rows [row_lo, row_lo + rows) as a bitmap (in LDS when it
+// fits next to its summary, else in a zeroed global region) plus a summary
+// bitmap in LDS (bit w set iff bitmap word w is non-zero).  Adding a pooled
+// column (sorted rows) flips one bit per row with independent atomics; the
+// pivot (lowest set bit; it only increases) is found through the summary;
+// at the end the column is read out in ascending order through the summary
+// and the bits are cleared.  Same additions in the same order as the list
+// form, so the same results.
+constexpr int kBmLdsWords = 2 * kSerHalf + kSerStage;  // dynamic LDS words
+
+template <bool G>
+__device__ __forceinline__ unsigned bm_load(const unsigned* p) {
+    if constexpr (G) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *p;
+}
+
+template <bool G>
+__device__ __forceinline__ void bm_zero(unsigned* p) {
+    if constexpr (G) __hip_atomic_store(p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = 0u;
+}
+
+// Flip local row x; returns +1 if the bit was set by it, -1 if cleared.
+__device__ __forceinline__ int bm_flip(unsigned* bits, unsigned* summ, int x) {
+    const int w = x >> 5;
+    const unsigned m = 1u << (x & 31);
+    const unsigned old = atomicXor(&bits[w], m);
+    if (old == 0u || old == m) atomicXor(&summ[w >> 5], 1u << (w & 31));
+    return (old & m) ? -1 : 1;
+}
+
+template <bool G>
+__device__ __forceinline__ void serial_bitmap(RedArgs& a, SerShared& sh, const unsigned* __restrict__ order,
+                              int lo, int lo2, long long row_lo, int rows, unsigned* bits,
+                              unsigned* summ, long long base, long long cap, long long& used,
+                              long long& stored, int& nxor, int& longest) {
+    const int t = threadIdx.x;
+    const int S = ((rows + 31) / 32 + 31) / 32;  // summary words
+    for (int r = lo; r < lo2; r++) {
+        const int c = (int)order[r];
+        const int lo_c = a.off[c];
+        const int n0 = min(a.off[c + 1] - lo_c, kSerialMaxCol);
+        int bad = 0;
+        for (int e = t; e < n0; e += kSerThreads) {
+            const long long x = (long long)a.data[lo_c + e] - row_lo;
+            if (x < 0 || x >= rows) bad = 1;
+            else bm_flip(bits, summ, (int)x);
+        }
+        int len = n0;
+        int p = (int)((long long)a.data[lo_c] - row_lo);
+        int phigh = (int)((long long)a.data[lo_c + n0 - 1] - row_lo);
+        int nbad;
+        (void)block_excl_scan(bad, sh, &nbad);  // also the barrier after the flips
+        if (nbad) {
+            if (t == 0) atomicExch(&a.status[0], 3);
+            return;
+        }
+        while (true) {
+            const unsigned long long v = a.own[row_lo + p - a.row_base];
+            if (v == kNoOwner) break;
+            const unsigned* ob = a.pool + (v & 0xFFFFFFFFull);
+            const int nb = (int)ob[0];
+            phigh = max(phigh, (int)((long long)ob[nb] - row_lo));
+            int delta = 0;
+            bad = 0;
+            for (int e = t; e < nb; e += kSerThreads) {
+                const long long x = (long long)ob[1 + e] - row_lo;
+                if (x < 0 || x >= rows) bad = 1;
+                else delta += bm_flip(bits, summ, (int)x);
+            }
+            int dsum;
+            (void)block_excl_scan(delta + (bad << 24), sh, &dsum);
+            if (dsum >= (1 << 23) || dsum <= -(1 << 23)) {  // a row outside the segment
+                if (t == 0) atomicExch(&a.status[0], 3);
+                return;
+            }
+            len += dsum;
+            nxor++;
+            if (len > kSerialMaxCol) {  // the reference would truncate: list form
+                if (t == 0) atomicExch(&a.status[0], 4);
+                return;
+            }
+            if (len == 0) break;
+            // lowest set bit, at or above the word of the old pivot
+            const int ws = p >> 5;
+            int found = -1;
+            for (int sw0 = ws >> 5; sw0 < S && found < 0; sw0 += kSerThreads) {
+                if (t == 0) sh.first_bad = 0x7FFFFFFF;
+                __syncthreads();
+                const int sw = sw0 + t;
+                if (sw < S) {
+                    unsigned val = summ[sw];
+                    if (sw == (ws >> 5)) val &= ~0u << (ws & 31);
+                    if (val) atomicMin(&sh.first_bad, sw * 32 + __builtin_ctz(val));
+                }
+                __syncthreads();
+                found = sh.first_bad == 0x7FFFFFFF ? -1 : sh.first_bad;
+                __syncthreads();
+            }
+            const unsigned bw = found >= 0 ? bm_load<G>(&bits[found]) : 0u;
+            const int np = found >= 0 && bw ? found * 32 + __builtin_ctz(bw) : -1;
+            if (!(np > p)) {  // pivot must increase
+                if (t == 0) atomicExch(&a.status[0], 3);
+                return;
+            }
+            p = np;
+        }
+        if (len == 0) {
+            if (t == 0) a.pivot[c] = -1;
+            __syncthreads();
+            continue;
+        }
+        if (t == 0) a.pivot[c] = (int)(row_lo + p);
+        longest = max(longest, len);
+        const bool keep = used + len <= cap;
+        const long long at = base + stored;
+        // read out (when kept) and clear, summary words [p / 1024, phigh / 1024]
+        int written = 0;
+        for (int sw0 = p >> 10; sw0 <= (phigh >> 10); sw0 += kSerThreads) {
+            const int sw = sw0 + t;
+            const unsigned val = (sw <= (phigh >> 10) && sw < S) ? summ[sw] : 0u;
+            int cnt = 0;
+            for (unsigned m = val; m; m &= m - 1) cnt += __popc(bm_load<G>(&bits[sw * 32 + __builtin_ctz(m)]));
+            int tot;
+            int off = block_excl_scan(cnt, sh, &tot) + written;
+            for (unsigned m = val; m; m &= m - 1) {
+                const int w = sw * 32 + __builtin_ctz(m);
+                unsigned bw = bm_load<G>(&bits[w]);
+                if (keep)
+                    for (; bw; bw &= bw - 1)
+                        a.pool[at + 1 + off++] = (unsigned)(row_lo + w * 32 + __builtin_ctz(bw));
+                bm_zero<G>(&bits[w]);
+            }
+            if (val) summ[sw] = 0u;
+            written += tot;
+        }
+        if (keep) {
+            if (t == 0) a.pool[at] = (unsigned)len;
+            __threadfence_block();
+            __syncthreads();
+            if (t == 0)
+                a.own[row_lo + p - a.row_base] =
+                    ((unsigned long long)(c + 1) << 32) | (unsigned long long)at;
+            used += len;
+            stored += len + 1;
+        }
+        if (written != len && t == 0) atomicExch(&a.status[0], 3);
+        __threadfence_block();
+        __syncthreads();
+        if (written != len) return;
+    }
+}
+
 __global__ __launch_bounds__(kSerThreads) void k_reduce_serial(
     RedArgs a, const unsigned* __restrict__ order, const int* __restrict__ res_count,
-    const int* __restrict__ seg_cols, const long long* __restrict__ seg_pool) {
+    const int* __restrict__ seg_cols, const long long* __restrict__ seg_pool, int variant) {
     extern __shared__ unsigned smem[];
     __shared__ SerShared sh;
     unsigned* half0 = smem;
@@ -710,8 +866,9 @@ __global__ __launch_bounds__(kSerThreads) void k_reduce_serial(
     const int seg = blockIdx.x;
     const int c0 = seg_cols ? seg_cols[seg] : 0;
     const int c1 = seg_cols ? seg_cols[seg + 1] : (int)a.K;
-    const long long base = seg_pool[2 * seg];
-    const long long cap = seg_pool[2 * seg + 1];
+    const long long* sp = seg_pool + kSegPoolWords * seg;
+    const long long base = sp[0];
+    const long long cap = sp[1];
     long long used = 0;    // reference entries in the store (uniform)
     long long stored = 0;  // words written at base (uniform)
     // 1. apparent columns, ascending: store while pool_ptr + cn <= pool_cap
@@ -771,9 +928,26 @@ __global__ __launch_bounds__(kSerThreads) void k_reduce_serial(
         if ((int)order[mid] >= c0) lo2 = mid + 1;
         else hi2 = mid;
     }
+    int nxor = 0, longest = 0, offchip = 0;
+    // bitmap form: sp[2] = first row (global rank), sp[3] = rows, sp[4] =
+    // word offset of a zeroed global bitmap in the pool, or -1 for LDS
+    const int rows = (int)sp[3];
+    if (variant != 1 && rows > 0) {
+        const int W = (rows + 31) / 32, S = (W + 31) / 32;
+        for (int i = t; i < S; i += kSerThreads) smem[i] = 0u;
+        if (sp[4] < 0)
+            for (int i = t; i < W; i += kSerThreads) smem[S + i] = 0u;
+        __syncthreads();
+        if (sp[4] < 0)
+            serial_bitmap<false>(a, sh, order, lo, lo2, sp[2], rows, smem + S, smem, base, cap,
+                                 used, stored, nxor, longest);
+        else
+            serial_bitmap<true>(a, sh, order, lo, lo2, sp[2], rows, a.pool + sp[4], smem, base,
+                                cap, used, stored, nxor, longest);
+        lo = lo2;  // done (or aborted with a status)
+    }
     unsigned long long big_off[2] = {0, 0};
     long long big_cap[2] = {0, 0};
-    int nxor = 0, longest = 0, offchip = 0;
     for (int r = lo; r < lo2; r++) {
         const int c = (int)order[r];
         const int lo_c = a.off[c];
@@ -949,7 +1123,8 @@ extern "C" int fph_reduce_serial(int64_t K, const int32_t* col_off, const uint32
                                  const uint8_t* kind, uint64_t* own, int32_t* pivot,
                                  const int32_t* res_list, const int32_t* res_count,
                                  uint32_t* pool, int64_t pool_cap, void* scratch,
-                                 size_t scratch_bytes, int32_t* status, void* stream) {
+                                 size_t scratch_bytes, int32_t* status, int variant,
+                                 void* stream) {
     hipStream_t s = (hipStream_t)stream;
     if (K < 0 || nrows < 0 || pool_cap < 0 || !scratch || !status || nseg < 1 || !seg_pool)
         return FPH_ERR_ARG;
@@ -992,7 +1167,7 @@ extern "C" int fph_reduce_serial(int64_t K, const int32_t* col_off, const uint32
                                           hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)kSerLds));
         k_reduce_serial<<<(unsigned)nseg, kSerThreads, kSerLds, s>>>(
-            a, order, res_count, seg_cols, (const long long*)seg_pool);
+            a, order, res_count, seg_cols, (const long long*)seg_pool, variant);
         FPH_LAUNCH_CHECK();
     }
     k_reduce_fini<<<1, 64, 0, s>>>(status, a.pool_ptr);

@@ -27,6 +27,11 @@ _POOL_MIN_WORDS = 1 << 20
 _MAX_ROUNDS = 1 << 22
 # Entry cap of the reference's reduction store (_numba_utils.py:11).
 _REF_MAX_POOL = 100_000_000
+# Dynamic LDS words of the serial reducer (working-column bitmap + summary).
+_BITMAP_LDS_WORDS = 32768
+# Working column of the serial reducer: 0 = bitmap (list form only where a
+# column would pass the reference's MAX_COL), 1 = sorted list.
+SERIAL_VARIANT = 0
 # When a list, every reduction attempt appends (pool words, status words).
 REDUCE_LOG = None
 
@@ -287,8 +292,8 @@ def build_h1(cx: Complex, max_triangles: int = MAX_TRIANGLES, semantics: str = "
     if semantics == "reference":
         # the reference's store counts 3 cofacet entries per triangle and
         # all E edges as columns (reduce_h1_gpu.py:523-534)
-        ref_sizes = ([(3 * T, E)] if segs is None else
-                     [(3 * t, e) for t, e in zip(tcs, ecs)])
+        ref_sizes = ([(3 * T, E, T)] if segs is None else
+                     [(3 * t, e, t) for t, e in zip(tcs, ecs)])
     elif semantics == "exact":
         ref_sizes = None
     else:
@@ -346,17 +351,27 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
         # apparent columns in place (C + K words) + line-rounded reduced columns
         pool_words = C + K + 64 + max(_POOL_MIN_WORDS, 64 * K + 4 * C)
     else:
-        caps = reference_pool_caps(ref_sizes)
+        caps = reference_pool_caps([(cof, kc) for cof, kc, _ in ref_sizes])
         base = _round32(C + K) + 32
-        seg_pool = []
-        for cap, (_, kc) in zip(caps, ref_sizes):
-            seg_pool += [base, cap]
+        seg_pool, bitmaps = [], []
+        row0 = row_base
+        for cap, (_, kc, rows) in zip(caps, ref_sizes):
+            seg_pool += [base, cap, row0, rows, -1]
             base += _round32(cap + kc + 64)
+            w = (rows + 31) // 32
+            if w + (w + 31) // 32 > _BITMAP_LDS_WORDS:
+                bitmaps.append((len(seg_pool) - 1, w))
+            row0 += rows
+        bm_lo = base
+        for slot, w in bitmaps:       # global working-column bitmaps
+            seg_pool[slot] = base
+            base += _round32(w)
+        bm_hi = base
         pool_start = base
         seg_pool_t = torch.tensor(seg_pool, dtype=torch.int64).to(device)
-        # + two growing off-chip buffers per segment (used only by columns
-        # longer than the on-chip halves)
+        # + two growing off-chip buffers per segment (list form only)
         pool_words = pool_start + max(_POOL_MIN_WORDS, nseg * 262144)
+        variant = SERIAL_VARIANT
     out = _empty(torch.float32, device, max(K, 1), 2)
     out_edges = _empty(torch.int32, device, max(K, 1), 2) if want_edges else None
     while True:
@@ -374,11 +389,13 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
         else:
             if pool_words > 0xFFFFFFFF:
                 raise _lib.FlashPHError("reduction store exceeds 2^32 words")
+            if bm_hi > bm_lo:
+                pool[bm_lo:bm_hi].zero_()
             check(lib.fph_reduce_serial(K, ptr(off), ptr(data), row_base, nrows, ptr(seg_cols),
                                         nseg, ptr(seg_pool_t), pool_start, ptr(kind), ptr(own),
                                         ptr(pivot), ptr(res_list), rescnt_p, ptr(pool),
-                                        pool_words, ptr(scratch), scratch_bytes, status_p, s),
-                  "fph_reduce_serial")
+                                        pool_words, ptr(scratch), scratch_bytes, status_p,
+                                        variant, s), "fph_reduce_serial")
         TIMER.stop(tok)
         call_ws("fph_diagram",
                 (K, ptr(kind), ptr(off), ptr(data), ptr(pivot), ptr(col_edge), row_base,
@@ -392,6 +409,9 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
             break
         if st[0] == 1:
             pool_words = max(2 * pool_words, 32 * st[5] + (32 * st[5]) // 4)
+            continue
+        if st[0] == 4 and ref_sizes is not None and variant == 0:
+            variant = 1   # a column would pass the reference's MAX_COL: list form
             continue
         raise _lib.FlashPHError(f"fph_reduce failed with status {st[0]}")
     stats = {"rounds": st[1], "xors": st[2], "longest": st[3], "offchip_xors": st[4],

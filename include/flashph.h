@@ -210,11 +210,17 @@ int fph_reduce(int64_t K, const int32_t *col_off, const uint32_t *col_data, int6
  * at a time in descending rank order; a column that does not fit in the
  * store is not stored (later columns reaching its pivot stop there) and
  * column additions keep at most 1048576 rows.  Where the store never
- * fills, the pairs equal fph_reduce's.  seg_pool (device int64, 2 per
- * segment): word offset in `pool` of the segment's store (at least
+ * fills, the pairs equal fph_reduce's.  seg_pool (device int64, 5 per
+ * segment): [0] word offset in `pool` of the segment's store (at least
  * capacity + segment columns + 64 words, beyond the apparent region of
- * col_off[K] + K words) and its capacity in reference entries.  pool_start:
- * first pool word after all stores.  status as for fph_reduce.
+ * col_off[K] + K words), [1] its capacity in reference entries, [2] global
+ * rank of the segment's first row, [3] its number of rows, [4] word offset
+ * of a zeroed bitmap of ceil(rows / 32) words in `pool` for the working
+ * column, or -1 when ceil(rows/32) + ceil(rows/1024) <= 32768 (kept in LDS).
+ * pool_start: first pool word after the stores and bitmaps.  variant: 0 =
+ * working column as a bitmap (status[0] = 4 when a column would exceed
+ * 1048576 rows: rerun with 1), 1 = as a sorted list.  status as for
+ * fph_reduce.
  * Replaces _general_cohomology_reduce_v2 (cohomology_general.py:202-371)
  * with the pool sizing of reduce_h1_gpu.py:282-289 and the prefill of
  * _numba_utils.py:68-104, including their capacity limits.
@@ -224,7 +230,8 @@ int fph_reduce_serial(int64_t K, const int32_t *col_off, const uint32_t *col_dat
                       const int64_t *seg_pool, int64_t pool_start, const uint8_t *kind,
                       uint64_t *own, int32_t *pivot, const int32_t *res_list,
                       const int32_t *res_count, uint32_t *pool, int64_t pool_cap,
-                      void *scratch, size_t scratch_bytes, int32_t *status, void *stream);
+                      void *scratch, size_t scratch_bytes, int32_t *status, int variant,
+                      void *stream);
 
 /* ---- diagrams --------------------------------------------------------- */
 

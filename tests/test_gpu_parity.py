@@ -142,3 +142,24 @@ def test_store_overflow_reference_and_exact(idx):
     assert ex["stats"]["pool_drops"] == 0
     got_ex = rips_persistence(torch.from_numpy(x).cuda(), t, max_dim=1, backend="exact")
     _eq("h1 exact", _cpu(got_ex[1]), ex["h1"])
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_serial_reducer_variants(variant):
+    """Both forms of the working column (bitmap, sorted list) of the
+    reference-semantics reducer give the reference's diagrams, on clouds
+    with and without store overflow."""
+    import oracle
+    from flash_ph import _pipeline
+    print(f"[gpu] serial variant {variant}", flush=True)
+    old = _pipeline.SERIAL_VARIANT
+    _pipeline.SERIAL_VARIANT = variant
+    try:
+        for x, t in [(_store_overflow_clouds()[0], None), (cases.randn(400, 3, 2), 1.0),
+                     (cases.clifford(600, 3, 0.02), 0.6)]:
+            t = oracle.auto_threshold(x) if t is None else t
+            ref = oracle.run(x, t, 1)
+            dg, cx = _pipeline.compute(torch.from_numpy(x).cuda(), t, 1, semantics="reference")
+            _eq("h1", _cpu(dg[1]), ref["h1"])
+    finally:
+        _pipeline.SERIAL_VARIANT = old
