@@ -10,9 +10,14 @@ log-uniform in [200, 5000], d in {4, 10, 50}, unit sphere + N(0, 0.05^2),
 per-cloud threshold = 95th percentile of 20-NN distances, H0+H1): one
 rips_persistence_batch call per step, beside the plain per-cloud loop
 (auto_threshold + rips_persistence per cloud), and checks that both give
-bit-identical diagrams.  Prints one JSON line.
+bit-identical diagrams.  "precision_n200k_d50" times the split-bf16
+matrix-core screen beside the direct FP32 screen at n=200,000, d=50, t=6.0
+(H0+H1), with the margin band and the FP64 error of the distances;
+"autograd_clifford_64x2000" times forward + backward of total H1
+persistence over 64 Clifford-torus clouds and checks the gradient against
+finite differences.  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch] [--no-extra]
 """
 from __future__ import annotations
 
@@ -28,6 +33,7 @@ sys.path.insert(0, os.path.join(ROOT, "flash-ph_amd"))
 
 N, D, T_EDGE, SEED = 10_000, 10, 1.6, 0
 PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 
 
@@ -110,13 +116,151 @@ def bench_batch(steps: int) -> dict:
     }
 
 
+def traffic_from_profile(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (FETCH_SIZE / WRITE_SIZE per dispatch, each scaled by its reading for a
+    plain copy of known size); None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "rips_n10k_d10_h0h2_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    fetch = write = fcal = wcal = None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 4 and f[1] == kernel and f[0] in ("FETCH_SIZE", "WRITE_SIZE"):
+            if f[0] == "FETCH_SIZE":
+                fetch = float(f[3])
+            else:
+                write = float(f[3])
+        if len(f) >= 7 and f[0] == "calibration":
+            cal = 262144.0 / float(f[6])   # a 256 MiB copy moves 262144 KiB each way
+            if f[1] == "FETCH_SIZE":
+                fcal = cal
+            else:
+                wcal = cal
+    if None in (fetch, write, fcal, wcal):
+        return None
+    return int(round(1024.0 * (fetch * fcal + write * wcal)))
+
+
+def bench_cfg4() -> dict:
+    """n=200,000, d=50 N(0,1), t=6.0, H0+H1: the split-bf16 MFMA screen
+    beside the direct FP32 screen (identical edges), the margin band, the
+    FP64 error of the FP32 distances, and the whole H0+H1 call."""
+    import numpy as np
+    import torch
+    from flash_ph import _pipeline, rips_persistence
+    n, d, t = 200_000, 50, 6.0
+    x = torch.from_numpy(np.random.default_rng(0).standard_normal((n, d)).astype(np.float32)).cuda()
+    out = {"workload": f"n={n}, d={d}, N(0,1) float32 seed 0, t={t}, H0+H1"}
+    for mode in ("mfma", "direct"):
+        _pipeline.SCREEN = mode
+        _pipeline.build_edges(x, t)                           # warm-up
+        _pipeline.TIMER.enable(True)
+        reps = 3 if mode == "mfma" else 1
+        for _ in range(reps):
+            cx = _pipeline.build_edges(x, t)
+        ms = _pipeline.TIMER.summary()["edge_screen"][1]
+        _pipeline.TIMER.enable(False)
+        out[f"screen_ms_{mode}"] = round(ms, 3)
+        if mode == "mfma":
+            E = cx.E
+            cand = int(cx.info["screen_candidates"].item())
+            flops = 3.0 * d * float(n) * n                    # 3 products x 2d x n^2/2
+            out["E"] = E
+            out["candidates"] = cand
+            out["margin_band"] = cand - E
+            out["roofline_mfma"] = {"bound": "mfma", "kernel": "k_edge_screen_mfma (split-bf16)",
+                                    "achieved": round(flops / (ms * 1e-3) / 1e12, 2),
+                                    "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                                    "frac": round(flops / (ms * 1e-3) / 1e12 / PEAK_BF16_TFLOPS, 4)}
+            i, j = cx.edge_i[:E].long(), cx.edge_j[:E].long()
+            x64 = x.double()
+            d64 = ((x64[i] - x64[j]) ** 2).sum(1)
+            d32 = cx.edge_dsq[:E]
+            ulp = (torch.nextafter(d32, torch.full_like(d32, float("inf"))) - d32).double()
+            out["fp64_err_max_ulp"] = round(((d32.double() - d64).abs() / ulp).max().item(), 3)
+            del i, j, x64, d64
+    _pipeline.SCREEN = "auto"
+    rips_persistence(x, t, max_dim=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dg = rips_persistence(x, t, max_dim=1)
+    torch.cuda.synchronize()
+    out["h0h1_ms"] = round(1000.0 * (time.perf_counter() - t0), 2)
+    out["rows"] = [int(v.shape[0]) for v in dg]
+    return out
+
+
+def bench_cfg5() -> dict:
+    """64 Clifford-torus clouds (n=2000, d=4, noise 0.02, t=0.6), loss =
+    total finite H1 persistence: forward + hand-written backward, and the
+    gradient against central differences (h=1e-3, FP64) of the same loss on
+    the recorded critical edges."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cases
+    from flash_ph import rips_persistence_batch, _pipeline
+    from flash_ph.autograd import _h0_edges
+    clouds = cases.clifford_batch(64, 2000, seed=0)
+
+    def step():
+        xs = [torch.from_numpy(c).cuda().requires_grad_(True) for c in clouds]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dg = rips_persistence_batch(xs, 0.6, max_dim=1)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss = 0
+        for per in dg:
+            fin = torch.isfinite(per[1][:, 1])
+            loss = loss + (per[1][fin, 1] - per[1][fin, 0]).sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return xs, float(loss), (t1 - t0) * 1e3, (t2 - t1) * 1e3
+
+    step()
+    xs, loss, fwd, bwd = step()
+    # central differences on the critical edges of cloud 0, float64
+    x0 = clouds[0]
+    _, cx = _pipeline.compute(torch.from_numpy(x0).cuda(), 0.6, 1, semantics="reference",
+                              want_edges=True)
+    re = cx.h1_edges
+    fin = re[:, 1] >= 0
+    bi, de = re[fin, 0].long(), re[fin, 1].long()
+    ei, ej = cx.edge_i.long(), cx.edge_j.long()
+
+    def loss64(z):
+        ln = lambda e: torch.linalg.norm(z[ei[e]] - z[ej[e]], dim=1)
+        return float((ln(de) - ln(bi)).sum())
+
+    g = xs[0].grad.double().cpu().numpy()
+    rng = np.random.default_rng(0)
+    worst = 0.0
+    for _ in range(24):
+        i, k = int(rng.integers(2000)), int(rng.integers(4))
+        zp = torch.from_numpy(x0).double().cuda()
+        zm = zp.clone()
+        zp[i, k] += 1e-3
+        zm[i, k] -= 1e-3
+        fd = (loss64(zp) - loss64(zm)) / 2e-3
+        worst = max(worst, abs(fd - g[i, k]) / max(1.0, abs(fd)))
+    return {"workload": "64 clouds, Clifford torus n=2000 d=4 + N(0,0.02^2), seed 0, t=0.6, "
+                        "loss = total finite H1 persistence",
+            "fwd_ms": round(fwd, 2), "bwd_ms": round(bwd, 2), "loss": round(loss, 4),
+            "fd_check": {"h": 1e-3, "dtype": "f64", "coords": 24,
+                         "max_rel_err": float(f"{worst:.3e}")}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-cloud batch leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip the n=200K and autograd legs")
     ap.add_argument("--batch-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -142,17 +286,21 @@ def main():
     torch.cuda.synchronize()
     ms = 1000.0 * (time.perf_counter() - t0) / args.steps
 
-    # Live per-kernel timing of the screening kernel (HIP events on the
-    # launch stream) over a separate pass of the same steps.
+    # Live per-launch timing (HIP events on the launch stream) over a
+    # separate pass of the same steps: the screen and the reductions.
     _pipeline.TIMER.enable(True)
     for _ in range(min(args.steps, 20)):
         rips_persistence(pts, T_EDGE, max_dim=2)
     summ = _pipeline.TIMER.summary()
     _pipeline.TIMER.enable(False)
-    launches, screen_ms = summ["edge_screen"]
+    _, screen_ms = summ["edge_screen"]
     flops = 3.0 * D * N * (N - 1) / 2.0          # sub + FMA per dimension per pair
     achieved = flops / (screen_ms * 1e-3) / 1e12
-    reduce_ms = summ.get("reduce", (0, 0.0))[1]
+    red_launches, reduce_ms = summ.get("reduce", (0, 0.0))
+    # entries merged per reduction launch (H1 and H2), 4 bytes each
+    _, cx = _pipeline.compute(pts, T_EDGE, 2)
+    merged = (cx.info["h1_stats"]["merged"] + cx.info["h2_stats"]["merged"]) / 2.0
+    red_gbs = 4.0 * merged / (reduce_ms * 1e-3) / 1e9
 
     out = {
         "metric": "ms/diagram n=10K d=10 H0-H2; batched clouds/s vs plain path, bit-exact",
@@ -170,19 +318,33 @@ def main():
         "config": {"workload": f"plain, n={N}, d={D}, t={T_EDGE} (strict <), H0+H1+H2",
                    "rows": [int(d.shape[0]) for d in dg]},
         "roofline": {
+            "bound": "hbm",
+            "kernel": "fph_reduce launch (k_reduce_rounds dominant; latency-bound Z/2 "
+                      "column additions)",
+            "achieved": round(red_gbs, 3),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(red_gbs / PEAK_HBM_GBS, 6),
+            "traffic": traffic_from_profile("k_reduce_rounds"),
+            "kernel_ms": round(reduce_ms, 4),
+            "bytes_per_launch": int(4 * merged),
+        },
+        "roofline_screen": {
             "bound": "mfma",
             "kernel": "k_edge_screen (FP32 vector FMA chain)",
             "achieved": round(achieved, 3),
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
-            "traffic": None,
             "kernel_ms": round(screen_ms, 5),
         },
         "stage_ms": {"edge_screen": round(screen_ms, 4), "reduce_h1_h2_mean": round(reduce_ms, 4)},
     }
     if not args.no_batch:
         out["batch"] = bench_batch(args.batch_steps)
+    if not args.no_extra:
+        out["precision_n200k_d50"] = bench_cfg4()
+        out["autograd_clifford_64x2000"] = bench_cfg5()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pts_np)
     print(json.dumps(out), flush=True)

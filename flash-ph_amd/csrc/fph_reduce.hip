@@ -64,6 +64,12 @@ struct RedArgs {
     int* status;  // [0] error, [1] rounds, [2] XORs, [3] longest column, [4] off-chip XORs
 };
 
+// status[6..7]: 64-bit count of entries merged (na + nb + result length
+// per column addition), for the algorithmic traffic of the reduction.
+__device__ __forceinline__ void count_merged(const RedArgs& a, long long n) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(a.status + 6), (unsigned long long)n);
+}
+
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -348,6 +354,7 @@ __device__ bool xor_step(Work& w, const unsigned* ob, unsigned* arena, const Red
         out = arena + h * H;
         out_half = h;
     }
+    if (fph_lane() == 0) count_merged(a, (long long)na + nb + len);
     w.ptr = out;
     w.half = out_half;
     w.len = len;
@@ -778,6 +785,7 @@ __device__ __forceinline__ void serial_bitmap(RedArgs& a, SerShared& sh, const u
                 if (t == 0) atomicExch(&a.status[0], 3);
                 return;
             }
+            if (t == 0) count_merged(a, 2LL * len + nb + dsum);
             len += dsum;
             nxor++;
             if (len > kSerialMaxCol) {  // the reference would truncate: list form
@@ -1023,6 +1031,7 @@ __global__ __launch_bounds__(kSerThreads) void k_reduce_serial(
                 out = smem + h * kSerHalf;
                 out_where = h;
             }
+            if (t == 0) count_merged(a, (long long)len + nb + nl);
             cur = out;
             where = out_where;
             len = nl;
@@ -1085,6 +1094,8 @@ __global__ void k_reduce_init(int* __restrict__ status, unsigned long long* __re
         status[3] = 0;
         status[4] = 0;
         status[5] = 0;
+        status[6] = 0;
+        status[7] = 0;
         // apparent region: off[K] + K words, one line of padding (speculative
         // reads of the last apparent column), then line-aligned dynamic space
         pool_ptr[0] = ((unsigned long long)off[K] + K + kLine - 1) / kLine * kLine + kLine;

@@ -341,11 +341,11 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
     own = _empty(torch.int64, device, max(nrows, 1))
     pivot = _empty(torch.int32, device, max(K, 1))
     res_list = _empty(torch.int32, device, max(K, 1))
-    # info32: [0..5] reduction status, [6] residual count, [8..8+nseg) rows
-    info32 = torch.zeros(8 + nseg, dtype=torch.int32, device=device)
+    # info32: [0..8) reduction status, [8] residual count, [12..12+nseg) rows
+    info32 = torch.zeros(12 + nseg, dtype=torch.int32, device=device)
     status_p = info32.data_ptr()
-    rescnt_p = info32.data_ptr() + 24
-    counts_p = info32.data_ptr() + 32
+    rescnt_p = info32.data_ptr() + 32
+    counts_p = info32.data_ptr() + 48
     scratch_bytes = lib.fph_reduce_scratch_bytes(K)
     if ref_sizes is None:
         # apparent columns in place (C + K words) + line-rounded reduced columns
@@ -404,7 +404,7 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
                 (s,), device)
         st = info32.tolist()
         if REDUCE_LOG is not None:
-            REDUCE_LOG.append((pool_words, st[:8]))
+            REDUCE_LOG.append((pool_words, st[:9]))
         if st[0] == 0:
             break
         if st[0] == 1:
@@ -414,9 +414,11 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
             variant = 1   # a column would pass the reference's MAX_COL: list form
             continue
         raise _lib.FlashPHError(f"fph_reduce failed with status {st[0]}")
+    merged = (st[6] & 0xFFFFFFFF) | ((st[7] & 0xFFFFFFFF) << 32)
     stats = {"rounds": st[1], "xors": st[2], "longest": st[3], "offchip_xors": st[4],
-             "pool_words": 32 * st[5], "pool_cap": pool_words, "residual": st[6]}
-    counts = st[8:8 + nseg]
+             "pool_words": 32 * st[5], "pool_cap": pool_words, "residual": st[8],
+             "merged": merged}
+    counts = st[12:12 + nseg]
     starts = [0] if segs is None else segs["cols_host"][:-1]
     dgms = [out[a:a + c] for a, c in zip(starts, counts)]
     if want_edges:

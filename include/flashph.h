@@ -186,11 +186,13 @@ size_t fph_reduce_scratch_bytes(int64_t K);
  * (reduced to zero: essential), and own[r - row_base] = (owner + 1) << 32 |
  * pool offset for every pivot row.  Columns are written to `pool`
  * (pool_cap 32-bit words, as [length, rows...]; the buffer must extend 64
- * words past pool_cap).  status (device int32[6]): [0] = 0 ok, 1 pool too
- * small (retry with a pool of more than 32 * status[5] words), 2 round cap
- * reached, 3 inconsistent input; [1] = rounds; [2] = column additions
- * performed; [3] = longest reduced column; [4] = additions too long for
- * on-chip memory; [5] = pool words handed out, in 32-word lines.
+ * words past pool_cap).  status (device int32[8], 8-byte aligned): [0] = 0
+ * ok, 1 pool too small (retry with a pool of more than 32 * status[5]
+ * words), 2 round cap reached, 3 inconsistent input; [1] = rounds; [2] =
+ * column additions performed; [3] = longest reduced column; [4] = additions
+ * too long for on-chip memory; [5] = pool words handed out, in 32-word
+ * lines; [6..7] = uint64 entries merged (sum over additions of both input
+ * lengths and the result length).
  * seg_cols (nseg+1 column offsets, device, may be NULL when nseg == 1)
  * splits the columns into independent segments (the point clouds of a
  * batch: no row is shared between segments); one workgroup reduces each.
