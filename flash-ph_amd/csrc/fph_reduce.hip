@@ -445,6 +445,10 @@ __global__ __launch_bounds__(kPhaseAWaves * FPH_WAVE) void k_reduce_chain(
 
 // Phase B work item: reduce column c against current owners, then claim.
 // Returns the displaced column (>= 0), -1 if none, -2 on error.
+// kAgent: claimers run in several workgroups (on different CUs), so the
+// publish -> claim and claim -> read orders need device (agent) scope, and a
+// reader must acquire before reading an owner's pooled content.
+template <bool kAgent>
 __device__ int reduce_claim(const RedArgs& a, int c, unsigned* arena, int* nxor, Alloc& al) {
     Work w;
     work_load(w, a.pool + a.desc[c], arena, kHalfB);
@@ -455,22 +459,28 @@ __device__ int reduce_claim(const RedArgs& a, int c, unsigned* arena, int* nxor,
         }
         const unsigned p = w.ptr[0];
         unsigned long long v = load_own(a, p);
+        if (kAgent && v != kNoOwner) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         if ((int)(v >> 32) - 1 <= c) {
             const long long at = publish(w, a, al);
             if (at < 0) return -2;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            // desc / pivot before the claim: once the claim lands, a higher
+            // column may displace c at once and another wavefront may take c
+            // up (queue mode); a lost claim rewrites them later
+            if (fph_lane() == 0) {
+                a.desc[c] = (unsigned)at;
+                a.pivot[c] = (int)p;
+            }
+            if (kAgent) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             const unsigned long long mine =
                 ((unsigned long long)(c + 1) << 32) | (unsigned long long)at;
             unsigned long long old = 0;
             if (fph_lane() == 0) old = atomicMax(&a.own[(long long)p - a.row_base], mine);
             old = __shfl(old, 0);
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            if (kAgent) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             const int oc = (int)(old >> 32) - 1;
             if (oc < c) {
-                if (fph_lane() == 0) {
-                    a.desc[c] = (unsigned)at;
-                    a.pivot[c] = (int)p;
-                }
                 note_len(a, w.len);
                 return oc;  // displaced owner (or -1)
             }
@@ -484,15 +494,25 @@ __device__ int reduce_claim(const RedArgs& a, int c, unsigned* arena, int* nxor,
 
 // Phase B: one workgroup per segment of columns (a point cloud of a batch;
 // the whole complex otherwise).  Segments never share rows, so they reduce
-// independently.
+// independently.  The 16 wavefronts take columns from a queue: first the
+// segment's residual columns in descending rank, then the owners displaced
+// by claims, in the order of displacement (a ring of (sequence << 32 |
+// column) words in the segment's column range: at most as many columns are
+// pending as the segment has, so a slot is never reused while unread).  A
+// wavefront with nothing to take waits for its slot and leaves when no
+// column is pending; the waits are bounded (status 5).  All communication is
+// inside one workgroup, so workgroup-scope ordering suffices.
 __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_rounds(
     RedArgs a, const unsigned* __restrict__ order, const int* __restrict__ res_count,
-    const int* __restrict__ seg_cols, int* __restrict__ wl0, int* __restrict__ wl1,
-    int max_rounds) {
+    const int* __restrict__ seg_cols, unsigned long long* __restrict__ ring,
+    long long spin_limit) {
     extern __shared__ unsigned smem[];
-    __shared__ int s_head, s_cnt, s_next_cnt, s_abort, s_lo;
+    __shared__ int s_lo, s_cnt, s_abort;
+    __shared__ unsigned s_head, s_dhead, s_dtail;
+    __shared__ int s_pending;
     const int tid = threadIdx.x;
     const int wid = tid / FPH_WAVE;
+    const int lane = fph_lane();
     unsigned* arena = smem + wid * kArenaB;
     const int c0 = seg_cols ? seg_cols[blockIdx.x] : 0;
     const int c1 = seg_cols ? seg_cols[blockIdx.x + 1] : (int)a.K;
@@ -515,53 +535,88 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_rounds(
         s_lo = lo;
         s_cnt = lo2 - lo;
         s_abort = a.status[0];
+        s_head = 0;
+        s_dhead = 0;
+        s_dtail = 0;
+        s_pending = lo2 - lo;
     }
     __syncthreads();
     if (s_abort || s_cnt == 0) return;
-    int rounds = 0, nxor = 0;
+    const unsigned cap = (unsigned)(c1 - c0);  // >= pending columns
+    unsigned long long* q = ring + c0;
+    int nxor = 0;
     Alloc al = make_alloc(4096);
-    // Round 0 walks every residual column of the segment in descending rank
-    // (the order the reference processes them); later rounds the columns
-    // displaced before.  Worklists live in the segment's own column range.
-    const int* wl = (const int*)order + s_lo;
-    int* w0 = wl0 + c0;
-    int* w1 = wl1 + c0;
-    int* wn = w0;
     while (true) {
-        const int n = s_cnt;
-        if (n == 0 || s_abort) break;
-        if (rounds >= max_rounds) {
-            if (tid == 0) atomicExch(&a.status[0], 2);
-            break;
+        if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+        unsigned k = 0;
+        if (lane == 0) k = atomicAdd(&s_head, 1u);
+        k = __shfl(k, 0);
+        int c = -1;
+        if (k < (unsigned)s_cnt) {
+            c = (int)order[s_lo + k];
+        } else {
+            unsigned slot = 0;
+            if (lane == 0) slot = atomicAdd(&s_dhead, 1u);
+            slot = __shfl(slot, 0);
+            long long spins = 0;
+            while (true) {
+                unsigned long long v = 0;
+                int pend = 0, ab = 0;
+                if (lane == 0) {
+                    v = __hip_atomic_load(&q[slot % cap], __ATOMIC_ACQUIRE,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+                    pend = __hip_atomic_load(&s_pending, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+                    ab = __hip_atomic_load(&s_abort, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                v = __shfl(v, 0);
+                pend = __shfl(pend, 0);
+                ab = __shfl(ab, 0);
+                if ((unsigned)(v >> 32) == slot) {
+                    c = (int)(v & 0xFFFFFFFFull);
+                    break;
+                }
+                if (pend == 0 || ab) break;
+                if (++spins > spin_limit) {
+                    if (lane == 0) {
+                        atomicExch(&a.status[0], 5);
+                        atomicExch(&s_abort, 1);
+                    }
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (c < 0) break;
         }
-        if (tid == 0) {
-            s_head = 0;
-            s_next_cnt = 0;
-        }
-        __syncthreads();
-        while (true) {
-            int k = 0;
-            if (fph_lane() == 0) k = atomicAdd(&s_head, 1);
-            k = __shfl(k, 0);
-            if (k >= n || s_abort) break;
-            const int c = wl[k];
-            if (a.pivot[c] < 0) continue;  // zero after phase A
-            const int d = reduce_claim(a, c, arena, &nxor, al);
+        int d = -1;
+        if (a.pivot[c] >= 0) {  // zero after phase A otherwise
+            d = reduce_claim<false>(a, c, arena, &nxor, al);
             if (d == -2) {
-                if (fph_lane() == 0) s_abort = 1;
+                if (lane == 0) {
+                    atomicExch(&s_abort, 1);
+                    atomicSub(&s_pending, 1);
+                }
                 break;
             }
-            if (d >= 0 && fph_lane() == 0) wn[atomicAdd(&s_next_cnt, 1)] = d;
         }
-        __syncthreads();
-        if (tid == 0) s_cnt = s_next_cnt;
-        __syncthreads();
-        wl = wn;
-        wn = (wn == w0) ? w1 : w0;
-        rounds++;
+        if (lane == 0) {
+            if (d >= 0) {
+                atomicAdd(&s_pending, 1);
+                const unsigned t = atomicAdd(&s_dtail, 1u);
+                __hip_atomic_store(&q[t % cap], ((unsigned long long)t << 32) | (unsigned)d,
+                                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            atomicSub(&s_pending, 1);
+        }
     }
-    if (tid == 0) atomicMax(&a.status[1], rounds);
-    if (fph_lane() == 0 && nxor) atomicAdd(&a.status[2], nxor);
+    if (lane == 0 && nxor) atomicAdd(&a.status[2], nxor);
+    if (tid == 0) atomicMax(&a.status[1], 1);
+}
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, long long n, unsigned long long v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // ---------------------------------------------------------------------
@@ -1124,9 +1179,11 @@ size_t sort_bytes(int64_t K) {
 
 extern "C" size_t fph_reduce_scratch_bytes(int64_t K) {
     const size_t k4 = align_up(4 * (size_t)std::max<int64_t>(K, 1));
-    // desc, two worklists, sort keys in/out, sort temp, pool counter
+    // desc, two worklists (the displaced ring), sort keys in/out, sort
+    // temp, pool counter
     return 5 * k4 + sort_bytes(K) + 256;
 }
+
 
 extern "C" int fph_reduce_serial(int64_t K, const int32_t* col_off, const uint32_t* col_data,
                                  int64_t row_base, int64_t nrows, const int32_t* seg_cols,
@@ -1209,7 +1266,6 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
     a.pivot = pivot;
     a.desc = (unsigned*)w;
     int* wl0 = (int*)(w + k4);
-    int* wl1 = (int*)(w + 2 * k4);
     unsigned* keys = (unsigned*)(w + 3 * k4);
     unsigned* order = (unsigned*)(w + 4 * k4);
     void* sort_tmp = w + 5 * k4;
@@ -1234,11 +1290,15 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
     FPH_LAUNCH_CHECK();
     FPH_CHECK_HIP(rocprim::radix_sort_keys_desc(sort_tmp, sort_sz, keys, order, (unsigned)K, 0,
                                                 32, s));
+    // ring of displaced owners: K 64-bit words over the two worklist areas
+    unsigned long long* ring = (unsigned long long*)wl0;
+    k_fill_u64<<<fph_blocks(K, 256), 256, 0, s>>>(ring, K, ~0ull);
+    FPH_LAUNCH_CHECK();
     const size_t lds_b = kPhaseBWaves * kArenaB * 4;
     FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_reduce_rounds,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
     k_reduce_rounds<<<(unsigned)nseg, kPhaseBWaves * FPH_WAVE, lds_b, s>>>(
-        a, order, res_count, seg_cols, wl0, wl1, max_rounds);
+        a, order, res_count, seg_cols, ring, max_rounds > 0 ? (long long)max_rounds : 1LL << 22);
     FPH_LAUNCH_CHECK();
     k_reduce_fini<<<1, 64, 0, s>>>(status, a.pool_ptr);
     FPH_LAUNCH_CHECK();
