@@ -558,10 +558,17 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_rounds(
             unsigned slot = 0;
             if (lane == 0) slot = atomicAdd(&s_dhead, 1u);
             slot = __shfl(slot, 0);
-            long long spins = 0;
+            // bounded wait: give up (status 5) after spin_limit wall-clock
+            // ticks without progress (no change of the pending count or
+            // the ring tail) -- other wavefronts may legitimately work on a
+            // long chain meanwhile
+            unsigned long long t_last = wall_clock64();
+            int last_pend = -1;
+            unsigned last_tail = 0xFFFFFFFFu;
             while (true) {
                 unsigned long long v = 0;
                 int pend = 0, ab = 0;
+                unsigned tail = 0;
                 if (lane == 0) {
                     v = __hip_atomic_load(&q[slot % cap], __ATOMIC_ACQUIRE,
                                           __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -569,16 +576,25 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_rounds(
                                              __HIP_MEMORY_SCOPE_WORKGROUP);
                     ab = __hip_atomic_load(&s_abort, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+                    tail = __hip_atomic_load(&s_dtail, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
                 v = __shfl(v, 0);
                 pend = __shfl(pend, 0);
                 ab = __shfl(ab, 0);
+                tail = __shfl(tail, 0);
                 if ((unsigned)(v >> 32) == slot) {
                     c = (int)(v & 0xFFFFFFFFull);
                     break;
                 }
                 if (pend == 0 || ab) break;
-                if (++spins > spin_limit) {
+                const unsigned long long now = wall_clock64();
+                if (pend != last_pend || tail != last_tail) {
+                    last_pend = pend;
+                    last_tail = tail;
+                    t_last = now;
+                }
+                if (now - t_last > (unsigned long long)spin_limit) {
                     if (lane == 0) {
                         atomicExch(&a.status[0], 5);
                         atomicExch(&s_abort, 1);
@@ -658,6 +674,8 @@ struct SerShared {
     unsigned long long big_off[2];
     unsigned long long next_v;  // owner word of the result's pivot (0: none)
     int next_nb;                // that owner's length
+    int red[2][kSerWaves][3];   // bitmap form: per-wave (sum, bad, max), by parity
+    int first[2];               // bitmap form: lowest set word x 32 + bit, by parity
 };
 
 // Exclusive prefix sum of v over the workgroup; *total = sum.  Contains
@@ -821,22 +839,46 @@ __device__ __forceinline__ void serial_bitmap(RedArgs& a, SerShared& sh, const u
             if (t == 0) atomicExch(&a.status[0], 3);
             return;
         }
+        // Each addition: flips, one barrier for the combined (sum, bad, max)
+        // reduction, one for the lowest-set-bit search.  LDS slots
+        // alternate by parity, so no barrier is needed to reuse them.
+        int par = 0;
+        const int wid = t / FPH_WAVE, lane = fph_lane();
         while (true) {
             const unsigned long long v = a.own[row_lo + p - a.row_base];
             if (v == kNoOwner) break;
             const unsigned* ob = a.pool + (v & 0xFFFFFFFFull);
             const int nb = (int)ob[0];
-            phigh = max(phigh, (int)((long long)ob[nb] - row_lo));
-            int delta = 0;
+            if (t == 0) sh.first[par] = 0x7FFFFFFF;
+            int delta = 0, lmax = 0;
             bad = 0;
             for (int e = t; e < nb; e += kSerThreads) {
                 const long long x = (long long)ob[1 + e] - row_lo;
-                if (x < 0 || x >= rows) bad = 1;
-                else delta += bm_flip(bits, summ, (int)x);
+                if (x < 0 || x >= rows) {
+                    bad = 1;
+                } else {
+                    delta += bm_flip(bits, summ, (int)x);
+                    lmax = max(lmax, (int)x);
+                }
             }
-            int dsum;
-            (void)block_excl_scan(delta + (bad << 24), sh, &dsum);
-            if (dsum >= (1 << 23) || dsum <= -(1 << 23)) {  // a row outside the segment
+            for (int o = 32; o > 0; o >>= 1) {
+                delta += __shfl_xor(delta, o);
+                bad |= __shfl_xor(bad, o);
+                lmax = max(lmax, __shfl_xor(lmax, o));
+            }
+            if (lane == 0) {
+                sh.red[par][wid][0] = delta;
+                sh.red[par][wid][1] = bad;
+                sh.red[par][wid][2] = lmax;
+            }
+            __syncthreads();
+            int dsum = 0, nbad = 0;
+            for (int w2 = 0; w2 < kSerWaves; w2++) {
+                dsum += sh.red[par][w2][0];
+                nbad |= sh.red[par][w2][1];
+                phigh = max(phigh, sh.red[par][w2][2]);
+            }
+            if (nbad) {  // a row outside the segment
                 if (t == 0) atomicExch(&a.status[0], 3);
                 return;
             }
@@ -848,29 +890,27 @@ __device__ __forceinline__ void serial_bitmap(RedArgs& a, SerShared& sh, const u
                 return;
             }
             if (len == 0) break;
-            // lowest set bit, at or above the word of the old pivot
+            // lowest set bit at or above the old pivot's word: one strided
+            // sweep of the summary, each thread stopping at its first hit
             const int ws = p >> 5;
-            int found = -1;
-            for (int sw0 = ws >> 5; sw0 < S && found < 0; sw0 += kSerThreads) {
-                if (t == 0) sh.first_bad = 0x7FFFFFFF;
-                __syncthreads();
-                const int sw = sw0 + t;
-                if (sw < S) {
-                    unsigned val = summ[sw];
-                    if (sw == (ws >> 5)) val &= ~0u << (ws & 31);
-                    if (val) atomicMin(&sh.first_bad, sw * 32 + __builtin_ctz(val));
+            for (int i = (ws >> 5) + t; i < S; i += kSerThreads) {
+                unsigned val = summ[i];
+                if (i == (ws >> 5)) val &= ~0u << (ws & 31);
+                if (val) {
+                    atomicMin(&sh.first[par], i * 32 + __builtin_ctz(val));
+                    break;
                 }
-                __syncthreads();
-                found = sh.first_bad == 0x7FFFFFFF ? -1 : sh.first_bad;
-                __syncthreads();
             }
-            const unsigned bw = found >= 0 ? bm_load<G>(&bits[found]) : 0u;
-            const int np = found >= 0 && bw ? found * 32 + __builtin_ctz(bw) : -1;
+            __syncthreads();
+            const int found = sh.first[par];
+            const unsigned bw = found != 0x7FFFFFFF ? bm_load<G>(&bits[found]) : 0u;
+            const int np = found != 0x7FFFFFFF && bw ? found * 32 + __builtin_ctz(bw) : -1;
             if (!(np > p)) {  // pivot must increase
                 if (t == 0) atomicExch(&a.status[0], 3);
                 return;
             }
             p = np;
+            par ^= 1;
         }
         if (len == 0) {
             if (t == 0) a.pivot[c] = -1;
@@ -1297,8 +1337,15 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
     const size_t lds_b = kPhaseBWaves * kArenaB * 4;
     FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_reduce_rounds,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
+    // no-progress bound of a waiting wavefront: max_rounds seconds (<= 0:
+    // 120 s) of the device wall clock
+    int dev = 0, khz = 0;
+    FPH_CHECK_HIP(hipGetDevice(&dev));
+    FPH_CHECK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    const long long secs = max_rounds > 0 ? (long long)max_rounds : 120LL;
+    const long long limit_ticks = secs * (long long)std::max(khz, 1) * 1000LL;
     k_reduce_rounds<<<(unsigned)nseg, kPhaseBWaves * FPH_WAVE, lds_b, s>>>(
-        a, order, res_count, seg_cols, ring, max_rounds > 0 ? (long long)max_rounds : 1LL << 22);
+        a, order, res_count, seg_cols, ring, limit_ticks);
     FPH_LAUNCH_CHECK();
     k_reduce_fini<<<1, 64, 0, s>>>(status, a.pool_ptr);
     FPH_LAUNCH_CHECK();

@@ -24,8 +24,9 @@ _EDGE_CAP_MIN = 1 << 20
 _EDGE_CAP_PER_POINT = 64
 # Reduction pool (32-bit words): max(this, 8 x cofacet entries).
 _POOL_MIN_WORDS = 1 << 20
-# Bound on the polls of a waiting wavefront in the reduction's second phase.
-_MAX_ROUNDS = 1 << 22
+# Seconds without progress after which a waiting wavefront of the
+# reduction's second phase gives up (status 5).
+_MAX_ROUNDS = 120
 # Entry cap of the reference's reduction store (_numba_utils.py:11).
 _REF_MAX_POOL = 100_000_000
 # Dynamic LDS words of the serial reducer (working-column bitmap + summary).

@@ -192,8 +192,8 @@ size_t fph_reduce_scratch_bytes(int64_t K);
  * column additions performed; [3] = longest reduced column; [4] = additions
  * too long for on-chip memory; [5] = pool words handed out, in 32-word
  * lines; [6..7] = uint64 entries merged (sum over additions of both input
- * lengths and the result length).  Status 5: a wait of the second phase
- * exceeded max_rounds polls (max_rounds <= 0: 2^22).
+ * lengths and the result length).  Status 5: a waiting wavefront of the
+ * second phase saw no progress for max_rounds seconds (<= 0: 120 s).
  * seg_cols (nseg+1 column offsets, device, may be NULL when nseg == 1)
  * splits the columns into independent segments (the point clouds of a
  * batch: no row is shared between segments); one workgroup reduces each.
