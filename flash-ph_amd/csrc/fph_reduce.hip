@@ -674,6 +674,7 @@ struct SerShared {
     unsigned long long big_off[2];
     unsigned long long next_v;  // owner word of the result's pivot (0: none)
     int next_nb;                // that owner's length
+    long long store_at;         // store offset when allocated from the pool
     int red[2][kSerWaves][3];   // bitmap form: per-wave (sum, bad, max), by parity
     int first[2];               // bitmap form: lowest set word x 32 + bit, by parity
 };
@@ -813,17 +814,35 @@ __device__ __forceinline__ int bm_flip(unsigned* bits, unsigned* summ, int x) {
     return (old & m) ? -1 : 1;
 }
 
+// Where a kept column of len rows goes: at base + stored in the segment's
+// store, or -- base < 0, the complete reduction with no store capacity --
+// bump-allocated from the pool.  -1 on pool overflow (status 1).  Contains
+// a barrier: call from uniform control flow.
+__device__ long long serial_store_at(const RedArgs& a, SerShared& sh, long long base,
+                                     long long stored, int len) {
+    if (base >= 0) return base + stored;
+    if (threadIdx.x == 0) {
+        const unsigned long long need = ((unsigned long long)len + 1 + kLine - 1) / kLine * kLine;
+        const unsigned long long at = atomicAdd(a.pool_ptr, need);
+        const bool over = at + need > a.pool_cap || at + need > 0xFFFFFFFFull;
+        if (over) atomicExch(&a.status[0], 1);
+        sh.store_at = over ? -1 : (long long)at;
+    }
+    __syncthreads();
+    return sh.store_at;
+}
+
 template <bool G>
 __device__ __forceinline__ void serial_bitmap(RedArgs& a, SerShared& sh, const unsigned* __restrict__ order,
                               int lo, int lo2, long long row_lo, int rows, unsigned* bits,
                               unsigned* summ, long long base, long long cap, long long& used,
-                              long long& stored, int& nxor, int& longest) {
+                              long long& stored, int& nxor, int& longest, int maxcol) {
     const int t = threadIdx.x;
     const int S = ((rows + 31) / 32 + 31) / 32;  // summary words
     for (int r = lo; r < lo2; r++) {
         const int c = (int)order[r];
         const int lo_c = a.off[c];
-        const int n0 = min(a.off[c + 1] - lo_c, kSerialMaxCol);
+        const int n0 = min(a.off[c + 1] - lo_c, maxcol);
         int bad = 0;
         for (int e = t; e < n0; e += kSerThreads) {
             const long long x = (long long)a.data[lo_c + e] - row_lo;
@@ -885,7 +904,7 @@ __device__ __forceinline__ void serial_bitmap(RedArgs& a, SerShared& sh, const u
             if (t == 0) count_merged(a, 2LL * len + nb + dsum);
             len += dsum;
             nxor++;
-            if (len > kSerialMaxCol) {  // the reference would truncate: list form
+            if (len > maxcol) {  // the reference would truncate: list form
                 if (t == 0) atomicExch(&a.status[0], 4);
                 return;
             }
@@ -920,7 +939,8 @@ __device__ __forceinline__ void serial_bitmap(RedArgs& a, SerShared& sh, const u
         if (t == 0) a.pivot[c] = (int)(row_lo + p);
         longest = max(longest, len);
         const bool keep = used + len <= cap;
-        const long long at = base + stored;
+        const long long at = keep ? serial_store_at(a, sh, base, stored, len) : 0;
+        if (at < 0) return;
         // read out (when kept) and clear, summary words [p / 1024, phigh / 1024]
         int written = 0;
         for (int sw0 = p >> 10; sw0 <= (phigh >> 10); sw0 += kSerThreads) {
@@ -1035,7 +1055,10 @@ __global__ __launch_bounds__(kSerThreads) void k_reduce_serial(
     // bitmap form: sp[2] = first row (global rank), sp[3] = rows, sp[4] =
     // word offset of a zeroed global bitmap in the pool, or -1 for LDS
     const int rows = (int)sp[3];
-    if (variant != 1 && rows > 0) {
+    // variant bit 0: sorted-list working column; bit 1: the complete
+    // reduction (no store capacity, no MAX_COL truncation)
+    const int maxcol = (variant & 2) ? 0x7FFFFFFF : kSerialMaxCol;
+    if ((variant & 1) == 0 && rows > 0) {
         const int W = (rows + 31) / 32, S = (W + 31) / 32;
         for (int i = t; i < S; i += kSerThreads) smem[i] = 0u;
         if (sp[4] < 0)
@@ -1043,10 +1066,10 @@ __global__ __launch_bounds__(kSerThreads) void k_reduce_serial(
         __syncthreads();
         if (sp[4] < 0)
             serial_bitmap<false>(a, sh, order, lo, lo2, sp[2], rows, smem + S, smem, base, cap,
-                                 used, stored, nxor, longest);
+                                 used, stored, nxor, longest, maxcol);
         else
             serial_bitmap<true>(a, sh, order, lo, lo2, sp[2], rows, a.pool + sp[4], smem, base,
-                                cap, used, stored, nxor, longest);
+                                cap, used, stored, nxor, longest, maxcol);
         lo = lo2;  // done (or aborted with a status)
     }
     unsigned long long big_off[2] = {0, 0};
@@ -1054,7 +1077,7 @@ __global__ __launch_bounds__(kSerThreads) void k_reduce_serial(
     for (int r = lo; r < lo2; r++) {
         const int c = (int)order[r];
         const int lo_c = a.off[c];
-        const int n0 = min(a.off[c + 1] - lo_c, kSerialMaxCol);
+        const int n0 = min(a.off[c + 1] - lo_c, maxcol);
         const unsigned* cur;
         int where;  // 0 / 1: LDS half, 2 / 3: off-chip buffer, -1: CSR (read-only)
         if (n0 <= kSerHalf) {
@@ -1115,7 +1138,7 @@ __global__ __launch_bounds__(kSerThreads) void k_reduce_serial(
                 out_where = 2 + bi;
             }
             __syncthreads();  // staged owner visible
-            int nl = block_xor(cur, len, B, nb, out, kSerialMaxCol, sh, a);
+            int nl = block_xor(cur, len, B, nb, out, maxcol, sh, a);
             next_v = sh.next_v;
             next_nb = sh.next_nb;
             have_next = true;
@@ -1144,7 +1167,8 @@ __global__ __launch_bounds__(kSerThreads) void k_reduce_serial(
         if (t == 0) a.pivot[c] = (int)piv;
         longest = max(longest, len);
         if (used + len <= cap) {
-            const long long at = base + stored;
+            const long long at = serial_store_at(a, sh, base, stored, len);
+            if (at < 0) return;
             if (t == 0) a.pool[at] = (unsigned)len;
             block_copy(cur, len, a.pool + at + 1);
             __threadfence_block();

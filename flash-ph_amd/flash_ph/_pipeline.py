@@ -31,6 +31,12 @@ _MAX_ROUNDS = 120
 _REF_MAX_POOL = 100_000_000
 # Dynamic LDS words of the serial reducer (working-column bitmap + summary).
 _BITMAP_LDS_WORDS = 32768
+# The exact reduction of a segment with more rows than this runs in serial
+# order (bitmap working column), else in parallel (fph_reduce).  The
+# parallel scheme wins on sparse complexes with short columns (n=10K, d=10:
+# 4 ms vs ~30 ms), the serial one on dense ones (n=4918, d=50: seconds vs a
+# minute).
+EXACT_SERIAL_MIN_ROWS = 100_000
 # Working column of the serial reducer: 0 = bitmap (list form only where a
 # column would pass the reference's MAX_COL), 1 = sorted list.
 SERIAL_VARIANT = 0
@@ -291,20 +297,24 @@ def build_h1(cx: Complex, max_triangles: int = MAX_TRIANGLES, semantics: str = "
             (ptr(cx.edge_i), ptr(cx.edge_j), E, T, ptr(cx.adj_ptr), ptr(cx.adj_idx),
              ptr(cx.adj_rank), ptr(cx.mst_mask), ptr(cx.tri_off), ptr(cx.tri_w),
              ptr(cx.cof_off), C, ptr(cx.cof_data)), (s,), dev)
+    # the reference's store counts 3 cofacet entries per triangle and all E
+    # edges as columns (reduce_h1_gpu.py:523-534)
+    shapes = ([(3 * T, E, T)] if segs is None else
+              [(3 * t, e, t) for t, e in zip(tcs, ecs)])
+    serial_exact = False
     if semantics == "reference":
-        # the reference's store counts 3 cofacet entries per triangle and
-        # all E edges as columns (reduce_h1_gpu.py:523-534)
-        ref_sizes = ([(3 * T, E, T)] if segs is None else
-                     [(3 * t, e, t) for t, e in zip(tcs, ecs)])
+        ref_sizes = shapes
     elif semantics == "exact":
-        ref_sizes = None
+        # dense complexes (long columns): the serial complete reduction
+        serial_exact = max(r for _, _, r in shapes) > EXACT_SERIAL_MIN_ROWS
+        ref_sizes = shapes if serial_exact else None
     else:
         raise ValueError(f"unknown reduction semantics {semantics!r}")
     cx.kind1, cx.pivot1, h1, stats = _reduce_and_diagram(
         K=E, col_base=0, off=cx.cof_off, data=cx.cof_data, skip=cx.mst_mask, row_cmf=cx.tri_cmf,
         row_base=E, nrows=T, C=C, col_edge=None, row_edge=cx.tri_e, edge_filt=cx.edge_filt,
         E=E, ess_ascending=(T == 0), device=dev, segs=segs, ref_sizes=ref_sizes,
-        want_edges=cx.want_edges)
+        serial_exact=serial_exact, want_edges=cx.want_edges)
     cx.h1 = h1 if segs is not None else h1[0]
     if cx.want_edges:
         re = stats.pop("row_edges")
@@ -324,14 +334,15 @@ def _round32(x: int) -> int:
 
 def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, C, col_edge,
                         row_edge, edge_filt, E, ess_ascending, device, segs=None,
-                        ref_sizes=None, want_edges=False):
+                        ref_sizes=None, serial_exact=False, want_edges=False):
     """Apparent pairs, residual reduction and the diagram of one dimension.
 
     ``segs`` (batches) = dict(cols=, edges=, ess_asc= device tensors,
     cols_host= list): the columns split into independent point clouds.
     ``ref_sizes``: None for the parallel exact reduction, or per segment
-    (reference cofacet entries, reference columns) to run the reference's
-    capacity-limited serial reduction (fph_reduce_serial).
+    (reference cofacet entries, reference columns, rows) to run the serial
+    reduction (fph_reduce_serial): the reference's capacity-limited one, or
+    with ``serial_exact`` the complete one in serial order.
     Returns (kind, pivot, [diagram per segment], stats)."""
     lib = load()
     s = stream_of(device)
@@ -353,13 +364,17 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
         # apparent columns in place (C + K words) + line-rounded reduced columns
         pool_words = C + K + 64 + max(_POOL_MIN_WORDS, 64 * K + 4 * C)
     else:
-        caps = reference_pool_caps([(cof, kc) for cof, kc, _ in ref_sizes])
+        caps = ([1 << 62] * len(ref_sizes) if serial_exact else
+                reference_pool_caps([(cof, kc) for cof, kc, _ in ref_sizes]))
         base = _round32(C + K) + 32
         seg_pool, bitmaps = [], []
         row0 = row_base
         for cap, (_, kc, rows) in zip(caps, ref_sizes):
-            seg_pool += [base, cap, row0, rows, -1]
-            base += _round32(cap + kc + 64)
+            if serial_exact:      # stored columns bump-allocated from the pool
+                seg_pool += [-1, cap, row0, rows, -1]
+            else:
+                seg_pool += [base, cap, row0, rows, -1]
+                base += _round32(cap + kc + 64)
             w = (rows + 31) // 32
             if w + (w + 31) // 32 > _BITMAP_LDS_WORDS:
                 bitmaps.append((len(seg_pool) - 1, w))
@@ -371,9 +386,12 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
         bm_hi = base
         pool_start = base
         seg_pool_t = torch.tensor(seg_pool, dtype=torch.int64).to(device)
-        # + two growing off-chip buffers per segment (list form only)
+        # + two growing off-chip buffers per segment (list form only), and
+        # for the complete reduction every stored column
         pool_words = pool_start + max(_POOL_MIN_WORDS, nseg * 262144)
-        variant = SERIAL_VARIANT
+        if serial_exact:
+            pool_words += 64 * K + 4 * C
+        variant = SERIAL_VARIANT | (2 if serial_exact else 0)
     out = _empty(torch.float32, device, max(K, 1), 2)
     out_edges = _empty(torch.int32, device, max(K, 1), 2) if want_edges else None
     while True:
@@ -412,8 +430,8 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
         if st[0] == 1:
             pool_words = max(2 * pool_words, 32 * st[5] + (32 * st[5]) // 4)
             continue
-        if st[0] == 4 and ref_sizes is not None and variant == 0:
-            variant = 1   # a column would pass the reference's MAX_COL: list form
+        if st[0] == 4 and ref_sizes is not None and (variant & 1) == 0:
+            variant |= 1  # a column would pass the reference's MAX_COL: list form
             continue
         raise _lib.FlashPHError(f"fph_reduce failed with status {st[0]}")
     merged = (st[6] & 0xFFFFFFFF) | ((st[7] & 0xFFFFFFFF) << 32)
@@ -465,7 +483,8 @@ def build_h2(cx: Complex) -> None:
         K=T, col_base=E, off=cx.cof2_off, data=cx.cof2_data, skip=cx.cleared,
         row_cmf=cx.tet_cmf, row_base=E + T, nrows=Q, C=C2, col_edge=cx.tri_e,
         row_edge=cx.tet_e, edge_filt=cx.edge_filt, E=E, ess_ascending=False, device=dev,
-        want_edges=cx.want_edges)
+        ref_sizes=[(C2, T, Q)] if Q > EXACT_SERIAL_MIN_ROWS else None,
+        serial_exact=Q > EXACT_SERIAL_MIN_ROWS, want_edges=cx.want_edges)
     cx.h2 = h2[0]
     if cx.want_edges:
         cx.h2_edges = stats.pop("row_edges")[0]

@@ -220,9 +220,13 @@ int fph_reduce(int64_t K, const int32_t *col_off, const uint32_t *col_data, int6
  * rank of the segment's first row, [3] its number of rows, [4] word offset
  * of a zeroed bitmap of ceil(rows / 32) words in `pool` for the working
  * column, or -1 when ceil(rows/32) + ceil(rows/1024) <= 32768 (kept in LDS).
- * pool_start: first pool word after the stores and bitmaps.  variant: 0 =
- * working column as a bitmap (status[0] = 4 when a column would exceed
- * 1048576 rows: rerun with 1), 1 = as a sorted list.  status as for
+ * pool_start: first pool word after the stores and bitmaps.  variant bit 0:
+ * 0 = working column as a bitmap (status[0] = 4 when a column would exceed
+ * 1048576 rows: rerun with bit 0 set), 1 = as a sorted list.  Bit 1 (with
+ * seg_pool[0] = -1, capacity ignored): the complete reduction instead --
+ * every column stored (bump-allocated in the pool), no truncation; the
+ * pairs of fph_reduce, in the reference's serial order (faster than
+ * fph_reduce on dense complexes with long columns).  status as for
  * fph_reduce.
  * Replaces _general_cohomology_reduce_v2 (cohomology_general.py:202-371)
  * with the pool sizing of reduce_h1_gpu.py:282-289 and the prefill of

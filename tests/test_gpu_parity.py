@@ -163,3 +163,26 @@ def test_serial_reducer_variants(variant):
             _eq("h1", _cpu(dg[1]), ref["h1"])
     finally:
         _pipeline.SERIAL_VARIANT = old
+
+
+def test_exact_serial_and_parallel_agree():
+    """The complete reduction run in serial order (dense complexes) and in
+    parallel (sparse ones) give the oracle's complete reduction."""
+    import oracle
+    from flash_ph import _pipeline
+    print("[gpu] exact serial vs parallel", flush=True)
+    x = _store_overflow_clouds()[0]
+    t = oracle.auto_threshold(x)
+    ex = oracle.run(x, t, 1, exact=True)
+    old = _pipeline.EXACT_SERIAL_MIN_ROWS
+    try:
+        for thr in (0, 1 << 40):
+            _pipeline.EXACT_SERIAL_MIN_ROWS = thr
+            dg, cx = _pipeline.compute(torch.from_numpy(x).cuda(), t, 1, semantics="exact")
+            _eq(f"h1 exact (serial threshold {thr})", _cpu(dg[1]), ex["h1"])
+        _pipeline.EXACT_SERIAL_MIN_ROWS = 0   # H1 and H2 of max_dim=2 in serial order
+        for name, make, tt, md in cases.SMALL[:8]:
+            if md == 2:
+                _check_case(make(), tt, 2)
+    finally:
+        _pipeline.EXACT_SERIAL_MIN_ROWS = old
