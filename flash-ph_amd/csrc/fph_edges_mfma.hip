@@ -20,10 +20,12 @@
 //   |x_k y_k| whatever the accumulation order.  With sum |x_k y_k| <=
 //   |x| |y| <= (|x|^2 + |y|^2) / 2:  |2 (x.y - G)| <= cG (|x|^2 + |y|^2),
 //   cG = 3.1 * 2^-16 + 1.02 gamma(3 dpad).
-//   The FP32 norms add gamma(d) (|x|^2 + |y|^2), the two FP32 operations of
-//   approx 4u (|x|^2 + |y|^2), and the direct form differs from the true
-//   distance by <= gamma(d) dsq ~ gamma(d) t^2 near the threshold:
-//   a = 1.25 (cG + gamma(d) + 4u),  b = 1.1 gamma(d) t^2.
+//   The FP32 norms add gamma(d) (|x|^2 + |y|^2), the FP32 operations of the
+//   test below <= 6u (|x|^2 + |y|^2), and the direct form differs from the
+//   true distance by <= gamma(d) dsq ~ gamma(d) t^2 near the threshold:
+//   a = 1.25 (cG + gamma(d) + 6u),  b = 1.1 gamma(d) t^2.
+// The test is evaluated as fma(-2, G, (1-a)|x|^2 + (1-a)|y|^2) < t^2 + b
+// (the scaled norms staged once per tile): three operations per pair.
 #include "fph_common.h"
 
 namespace {
@@ -85,7 +87,7 @@ __device__ __forceinline__ void tile_coords(long long t, int* I, int* J) {
     *I = (int)(t - j * (j + 1) / 2);
 }
 
-__global__ __launch_bounds__(kThreadsM) void k_edge_screen_mfma(
+__global__ __launch_bounds__(kThreadsM, 4) void k_edge_screen_mfma(
     const float* __restrict__ pts, const bf16x8* __restrict__ xh,
     const bf16x8* __restrict__ xl, const float* __restrict__ nrm, int n, int d, int dpad,
     float thr, float marg_a, float marg_b, long long ntiles, unsigned long long* __restrict__ keys,
@@ -102,12 +104,13 @@ __global__ __launch_bounds__(kThreadsM) void k_edge_screen_mfma(
     int I, J;
     tile_coords(t, &I, &J);
     const int tid = threadIdx.x;
+    const float one_a = 1.0f - marg_a;
     if (tid < kMT) {
         const int i = min(I * kMT + tid, n - 1);
-        s_nr[tid] = nrm[i];
+        s_nr[tid] = one_a * nrm[i];
     } else {
         const int j = min(J * kMT + tid - kMT, n - 1);
-        s_nc[tid - kMT] = nrm[j];
+        s_nc[tid - kMT] = one_a * nrm[j];
     }
     __syncthreads();
     if (!live) return;
@@ -148,20 +151,23 @@ __global__ __launch_bounds__(kThreadsM) void k_edge_screen_mfma(
         acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al1, bh1, acc[1][1], 0, 0, 0);
     }
     // Epilogue: accumulator slot v of lane l is row 8 (v / 4) + 4 (l / 32)
-    // + v % 4, column l % 32 of its 32 x 32 block.
+    // + v % 4, column l % 32 of its 32 x 32 block.  Off the diagonal and
+    // inside [0, n) every pair is valid (i < j < n).
+    const float lim = thr + marg_b;
+    const bool interior = I < J && (J + 1) * kMT <= n;
+    const float cn0 = s_nc[csub + r], cn1 = s_nc[csub + 32 + r];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++)
+        for (int v = 0; v < 16; v++) {
+            const float rv = s_nr[rsub + 32 * a + 8 * (v >> 2) + 4 * kh + (v & 3)];
 #pragma unroll
-            for (int v = 0; v < 16; v++) {
+            for (int b = 0; b < 2; b++) {
+                const bool hit = __builtin_fmaf(-2.0f, acc[a][b][v], rv + (b ? cn1 : cn0)) < lim;
                 const int li = rsub + 32 * a + 8 * (v >> 2) + 4 * kh + (v & 3);
                 const int lj = csub + 32 * b + r;
                 const int i = I * kMT + li, j = J * kMT + lj;
-                const float ni = s_nr[li], nj = s_nc[lj];
-                const float approx = (ni + nj) - 2.0f * acc[a][b][v];
-                const bool cand = i < j && j < n &&
-                                  approx < thr + marg_a * (ni + nj) + marg_b;
+                const bool cand = hit && (interior || (i < j && j < n));
                 if (cand) {  // rare: recompute in the direct form
                     atomicAdd(ncand, 1ull);
                     const float* xi = pts + (long long)i * d;
@@ -184,6 +190,7 @@ __global__ __launch_bounds__(kThreadsM) void k_edge_screen_mfma(
                     }
                 }
             }
+        }
 }
 
 __global__ void k_zero2(unsigned long long* a, unsigned long long* b) {
@@ -225,7 +232,7 @@ extern "C" int fph_edge_screen_mfma(const float* pts, int64_t n, int d, float th
     FPH_LAUNCH_CHECK();
     const double cg = 3.1 * std::ldexp(1.0, -16) + 1.02 * gamma_n(3 * dpad);
     const double u = std::ldexp(1.0, -24);
-    const float marg_a = (float)(1.25 * (cg + gamma_n(d) + 4.0 * u));
+    const float marg_a = (float)(1.25 * (cg + gamma_n(d) + 6.0 * u));
     const float marg_b = (float)(1.1 * gamma_n(d) * (double)thr_sq);
     const long long nts = (n + kMT - 1) / kMT;
     const long long ntiles = nts * (nts + 1) / 2;
