@@ -18,13 +18,16 @@
 //   phase A  (every residual column at once, one wavefront each): XOR with
 //            apparent owners while the pivot has one.  Apparent columns own
 //            their pivot forever (no higher-rank column contains it).
-//   phase B  (one persistent workgroup of 16 wavefronts): rounds over a
-//            worklist in descending column rank.  A wavefront takes a column
-//            and XORs it with the current owner of its pivot until the pivot
-//            is free or owned by a lower-rank column; then it publishes the
-//            content and claims the pivot with one 64-bit atomicMax.  A lost
-//            race continues with the winner; a displaced lower-rank owner is
-//            queued for the next round.  Stops when a round displaces nobody.
+//   phase B  (one workgroup of 16 wavefronts per segment): a queue of the
+//            residual columns in descending rank, then of displaced owners.
+//            A wavefront takes a column and XORs it with the current owner
+//            of its pivot until the pivot is free or owned by a lower-rank
+//            column; then it publishes the content and claims the pivot with
+//            one 64-bit atomicMax.  A lost race continues with the winner; a
+//            displaced lower-rank owner goes back into the queue.  Ends when
+//            no column is pending.  (Across workgroups the claims would need
+//            device-scope fences, which on a multi-XCD GPU write back and
+//            invalidate the L2s on every claim: measured 20x slower.)
 //
 // A column is a sorted list of row ranks stored in a bump-allocated pool as
 // [length, rows...].  XOR of two sorted lists is a merge with cancellation
@@ -61,7 +64,7 @@ struct RedArgs {
     unsigned* pool;
     unsigned long long pool_cap;
     unsigned long long* pool_ptr;
-    int* status;  // [0] error, [1] rounds, [2] XORs, [3] longest column, [4] off-chip XORs
+    int* status;  // [0] error, [1] ran, [2] XORs, [3] longest column, [4] off-chip XORs, ...
 };
 
 // status[6..7]: 64-bit count of entries merged (na + nb + result length
@@ -502,7 +505,7 @@ __device__ int reduce_claim(const RedArgs& a, int c, unsigned* arena, int* nxor,
 // wavefront with nothing to take waits for its slot and leaves when no
 // column is pending; the waits are bounded (status 5).  All communication is
 // inside one workgroup, so workgroup-scope ordering suffices.
-__global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_rounds(
+__global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_queue(
     RedArgs a, const unsigned* __restrict__ order, const int* __restrict__ res_count,
     const int* __restrict__ seg_cols, unsigned long long* __restrict__ ring,
     long long spin_limit) {
@@ -1312,7 +1315,7 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
                           int64_t nseg, const uint8_t* kind, uint64_t* own, int32_t* pivot,
                           const int32_t* res_list, const int32_t* res_count, uint32_t* pool,
                           int64_t pool_cap, void* scratch, size_t scratch_bytes,
-                          int32_t* status, int max_rounds, void* stream) {
+                          int32_t* status, int wait_seconds, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     if (K < 0 || nrows < 0 || pool_cap < 0 || !scratch || !status || nseg < 1) return FPH_ERR_ARG;
     if (nseg > 1 && !seg_cols) return FPH_ERR_ARG;
@@ -1359,16 +1362,16 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
     k_fill_u64<<<fph_blocks(K, 256), 256, 0, s>>>(ring, K, ~0ull);
     FPH_LAUNCH_CHECK();
     const size_t lds_b = kPhaseBWaves * kArenaB * 4;
-    FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_reduce_rounds,
+    FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_reduce_queue,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
-    // no-progress bound of a waiting wavefront: max_rounds seconds (<= 0:
-    // 120 s) of the device wall clock
+    // no-progress bound of a waiting wavefront: wait_seconds (<= 0: 120 s)
+    // of the device wall clock
     int dev = 0, khz = 0;
     FPH_CHECK_HIP(hipGetDevice(&dev));
     FPH_CHECK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-    const long long secs = max_rounds > 0 ? (long long)max_rounds : 120LL;
+    const long long secs = wait_seconds > 0 ? (long long)wait_seconds : 120LL;
     const long long limit_ticks = secs * (long long)std::max(khz, 1) * 1000LL;
-    k_reduce_rounds<<<(unsigned)nseg, kPhaseBWaves * FPH_WAVE, lds_b, s>>>(
+    k_reduce_queue<<<(unsigned)nseg, kPhaseBWaves * FPH_WAVE, lds_b, s>>>(
         a, order, res_count, seg_cols, ring, limit_ticks);
     FPH_LAUNCH_CHECK();
     k_reduce_fini<<<1, 64, 0, s>>>(status, a.pool_ptr);

@@ -26,7 +26,7 @@ _EDGE_CAP_PER_POINT = 64
 _POOL_MIN_WORDS = 1 << 20
 # Seconds without progress after which a waiting wavefront of the
 # reduction's second phase gives up (status 5).
-_MAX_ROUNDS = 120
+_WAIT_SECONDS = 120
 # Entry cap of the reference's reduction store (_numba_utils.py:11).
 _REF_MAX_POOL = 100_000_000
 # Dynamic LDS words of the serial reducer (working-column bitmap + summary).
@@ -405,7 +405,7 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
             check(lib.fph_reduce(K, ptr(off), ptr(data), row_base, nrows, ptr(seg_cols), nseg,
                                  ptr(kind), ptr(own), ptr(pivot), ptr(res_list), rescnt_p,
                                  ptr(pool), pool_words, ptr(scratch), scratch_bytes, status_p,
-                                 _MAX_ROUNDS, s), "fph_reduce")
+                                 _WAIT_SECONDS, s), "fph_reduce")
         else:
             if pool_words > 0xFFFFFFFF:
                 raise _lib.FlashPHError("reduction store exceeds 2^32 words")
@@ -435,7 +435,7 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
             continue
         raise _lib.FlashPHError(f"fph_reduce failed with status {st[0]}")
     merged = (st[6] & 0xFFFFFFFF) | ((st[7] & 0xFFFFFFFF) << 32)
-    stats = {"rounds": st[1], "xors": st[2], "longest": st[3], "offchip_xors": st[4],
+    stats = {"xors": st[2], "longest": st[3], "offchip_xors": st[4],
              "pool_words": 32 * st[5], "pool_cap": pool_words, "residual": st[8],
              "merged": merged}
     counts = st[12:12 + nseg]

@@ -188,12 +188,12 @@ size_t fph_reduce_scratch_bytes(int64_t K);
  * (pool_cap 32-bit words, as [length, rows...]; the buffer must extend 64
  * words past pool_cap).  status (device int32[8], 8-byte aligned): [0] = 0
  * ok, 1 pool too small (retry with a pool of more than 32 * status[5]
- * words), 2 round cap reached, 3 inconsistent input; [1] = rounds; [2] =
- * column additions performed; [3] = longest reduced column; [4] = additions
- * too long for on-chip memory; [5] = pool words handed out, in 32-word
- * lines; [6..7] = uint64 entries merged (sum over additions of both input
- * lengths and the result length).  Status 5: a waiting wavefront of the
- * second phase saw no progress for max_rounds seconds (<= 0: 120 s).
+ * words), 3 inconsistent input, 5 a waiting wavefront of the second phase
+ * saw no progress for wait_seconds (<= 0: 120 s); [1] = 1 once the second
+ * phase ran; [2] = column additions performed; [3] = longest reduced
+ * column; [4] = additions too long for on-chip memory; [5] = pool words
+ * handed out, in 32-word lines; [6..7] = uint64 entries merged (sum over
+ * additions of both input lengths and the result length).
  * seg_cols (nseg+1 column offsets, device, may be NULL when nseg == 1)
  * splits the columns into independent segments (the point clouds of a
  * batch: no row is shared between segments); one workgroup reduces each.
@@ -204,7 +204,7 @@ int fph_reduce(int64_t K, const int32_t *col_off, const uint32_t *col_data, int6
                int64_t nrows, const int32_t *seg_cols, int64_t nseg, const uint8_t *kind,
                uint64_t *own, int32_t *pivot, const int32_t *res_list,
                const int32_t *res_count, uint32_t *pool, int64_t pool_cap, void *scratch,
-               size_t scratch_bytes, int32_t *status, int max_rounds, void *stream);
+               size_t scratch_bytes, int32_t *status, int wait_seconds, void *stream);
 
 /*
  * The reference's own reduction semantics, for results identical to it in
