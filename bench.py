@@ -264,6 +264,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-cloud batch leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the n=200K and autograd legs")
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch parameters (no autotuning pass)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -278,6 +280,12 @@ def main():
     pts_np = make_points()
     pts = torch.from_numpy(pts_np).cuda()
     torch.cuda.synchronize()
+    tuned = None
+    if not args.no_tune:
+        # the library's autotuner: times every tile / block candidate on this
+        # cloud, checks identical diagrams, installs the fastest (untimed)
+        from flash_ph import autotune
+        tuned = autotune.tune(pts, T_EDGE, max_dim=2, repeats=3)
     for _ in range(args.warmup):
         rips_persistence(pts, T_EDGE, max_dim=2)
     torch.cuda.synchronize()
@@ -349,6 +357,8 @@ def main():
             "kernel_ms": round(screen_ms, 5),
         },
         "stage_ms": {"edge_screen": round(screen_ms, 4), "reduce_h1_h2_mean": round(reduce_ms, 4)},
+        "autotune": (None if tuned is None else
+                     {k: {"value": v["value"], "ms": v["ms"]} for k, v in tuned.items()}),
     }
     if not args.no_batch:
         out["batch"] = bench_batch(args.batch_steps)

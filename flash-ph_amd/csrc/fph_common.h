@@ -21,6 +21,9 @@
         if (err_ != hipSuccess) return FPH_ERR_HIP;          \
     } while (0)
 
+// Value of autotune knob `knob` in effect for (size, d) (defined in fph_tune.hip).
+int fph_tune_value(int knob, int64_t size, int64_t d);
+
 static inline unsigned fph_blocks(int64_t n, int threads) {
     int64_t b = (n + threads - 1) / threads;
     if (b < 1) b = 1;

@@ -9,9 +9,8 @@
 
 namespace {
 
-constexpr int kTile = 64;     // points per tile side
 constexpr int kChunk = 32;    // dimensions staged in LDS per pass
-constexpr int kThreads = 256; // 16 x 16 threads, 4 x 4 pairs each
+constexpr int kThreads = 256; // 16 x 16 threads
 
 // Tile index -> (I, J) with I <= J, tiles ordered by J then I.
 __device__ __forceinline__ void tile_coords(long long t, int* I, int* J) {
@@ -27,71 +26,77 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
-// One workgroup per upper-triangular 64 x 64 tile.  Each thread owns a 4 x 4
-// block of pairs (rows ty*4+r, columns tx*4+c) and accumulates
+// One workgroup per upper-triangular TILE x TILE tile (TILE = 32, 64 or 128,
+// an autotuned launch parameter).  Each thread owns an R x R block of pairs
+// (rows ty*R+r, columns tx*R+c, R = TILE/16) and accumulates
 //     acc = fma(x_i[k] - x_j[k], x_i[k] - x_j[k], acc),   k = 0 .. d-1
-// in the reference's order (edge_kernel.py:112-123 with fused multiply-add).
-// Pairs with acc < thr_sq (strict, FP32) are appended, unordered, as
+// in the reference's order (edge_kernel.py:112-123 with fused multiply-add),
+// so the values do not depend on TILE.  Pairs with acc < thr_sq (strict,
+// FP32) are appended, unordered, as
 //     key = float_bits(acc) << 32 | (j(j-1)/2 + i) mod 2^32     (rips.py:41-49)
 //     val = j << 32 | i
 // through one atomic per wavefront.  The count keeps growing past `cap` so the
 // caller can size a retry.
+template <int TILE>
 __global__ __launch_bounds__(kThreads) void k_edge_screen(
     const float* __restrict__ pts, int n, int d, float thr_sq, long long tile0,
     unsigned long long* __restrict__ keys, unsigned long long* __restrict__ vals,
     long long cap, unsigned long long* __restrict__ count) {
-    __shared__ float4 sa[kChunk][kTile / 4];
-    __shared__ float4 sb[kChunk][kTile / 4];
+    constexpr int R = TILE / 16;
+    __shared__ __align__(16) float fa[kChunk * TILE];
+    __shared__ __align__(16) float fb[kChunk * TILE];
     int I, J;
     tile_coords(tile0 + (long long)blockIdx.x, &I, &J);
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int i0 = I * kTile, j0 = J * kTile;
+    const int i0 = I * TILE, j0 = J * TILE;
 
-    float acc[4][4];
+    float acc[R][R];
 #pragma unroll
-    for (int r = 0; r < 4; r++)
+    for (int r = 0; r < R; r++)
 #pragma unroll
-        for (int c = 0; c < 4; c++) acc[r][c] = 0.0f;
+        for (int c = 0; c < R; c++) acc[r][c] = 0.0f;
 
-    float* fa = reinterpret_cast<float*>(sa);
-    float* fb = reinterpret_cast<float*>(sb);
     for (int k0 = 0; k0 < d; k0 += kChunk) {
         const int kc = min(kChunk, d - k0);
         __syncthreads();
-        for (int idx = threadIdx.x; idx < kTile * kc; idx += kThreads) {
+        for (int idx = threadIdx.x; idx < TILE * kc; idx += kThreads) {
             const int p = idx / kc, k = idx - p * kc;
             const int gi = i0 + p, gj = j0 + p;
-            fa[k * kTile + p] = gi < n ? pts[(long long)gi * d + k0 + k] : 0.0f;
-            fb[k * kTile + p] = gj < n ? pts[(long long)gj * d + k0 + k] : 0.0f;
+            fa[k * TILE + p] = gi < n ? pts[(long long)gi * d + k0 + k] : 0.0f;
+            fb[k * TILE + p] = gj < n ? pts[(long long)gj * d + k0 + k] : 0.0f;
         }
         __syncthreads();
         for (int k = 0; k < kc; k++) {
-            const float4 a = sa[k][ty];
-            const float4 b = sb[k][tx];
-            const float av[4] = {a.x, a.y, a.z, a.w};
-            const float bv[4] = {b.x, b.y, b.z, b.w};
+            float av[R], bv[R];
 #pragma unroll
-            for (int r = 0; r < 4; r++)
+            for (int r = 0; r < R; r++) av[r] = fa[k * TILE + ty * R + r];
 #pragma unroll
-                for (int c = 0; c < 4; c++) {
+            for (int c = 0; c < R; c++) bv[c] = fb[k * TILE + tx * R + c];
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int c = 0; c < R; c++) {
                     const float diff = av[r] - bv[c];
                     acc[r][c] = __builtin_fmaf(diff, diff, acc[r][c]);
                 }
         }
     }
 
-    // Selection and wave-level compaction.
-    unsigned long long bal[16];
-    int total = 0;
+    // Selection and wave-level compaction.  Bit r*R+c of `keep` marks a kept
+    // pair (R*R <= 64); the wave's total is a lane sum, and the ballots are
+    // taken one at a time while writing, so none is kept live.
+    unsigned long long keep = 0;
 #pragma unroll
-    for (int r = 0; r < 4; r++)
+    for (int r = 0; r < R; r++)
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const int i = i0 + ty * 4 + r, j = j0 + tx * 4 + c;
+        for (int c = 0; c < R; c++) {
+            const int i = i0 + ty * R + r, j = j0 + tx * R + c;
             const bool ok = (acc[r][c] < thr_sq) && i < n && j < n && (I < J || i < j);
-            bal[r * 4 + c] = __ballot(ok);
-            total += __popcll(bal[r * 4 + c]);
+            keep |= (unsigned long long)ok << (r * R + c);
         }
+    int total = __popcll(keep);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o);
     if (total == 0) return;
     unsigned long long base = 0;
     if (fph_lane() == 0) base = atomicAdd(count, (unsigned long long)total);
@@ -99,15 +104,17 @@ __global__ __launch_bounds__(kThreads) void k_edge_screen(
     const unsigned long long lt = fph_lanemask_lt();
     unsigned long long pos = base;
 #pragma unroll
-    for (int r = 0; r < 4; r++)
+    for (int r = 0; r < R; r++)
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const unsigned long long b = bal[r * 4 + c];
-            if ((b >> fph_lane()) & 1ull) {
+        for (int c = 0; c < R; c++) {
+            const int ii = i0 + ty * R + r, jj = j0 + tx * R + c;
+            const bool ok = (keep >> (r * R + c)) & 1ull;
+            const unsigned long long b = __ballot(ok);
+            if (ok) {
                 const unsigned long long at = pos + __popcll(b & lt);
                 if (at < (unsigned long long)cap) {
-                    const unsigned long long i = (unsigned long long)(i0 + ty * 4 + r);
-                    const unsigned long long j = (unsigned long long)(j0 + tx * 4 + c);
+                    const unsigned long long i = (unsigned long long)ii;
+                    const unsigned long long j = (unsigned long long)jj;
                     unsigned bits;
                     memcpy(&bits, &acc[r][c], 4);
                     const unsigned long long colex = j * (j - 1) / 2 + i;
@@ -207,6 +214,22 @@ int bit_width_u64(unsigned long long x) {
     return b;
 }
 
+template <int TILE>
+int launch_screen(const float* pts, int64_t n, int d, float thr_sq, uint64_t* keys,
+                  uint64_t* vals, int64_t cap, uint64_t* count_dev, hipStream_t s) {
+    const long long nb = (n + TILE - 1) / TILE;
+    const long long ntiles = nb * (nb + 1) / 2;
+    const long long kMaxGrid = 1LL << 30;
+    for (long long t0 = 0; t0 < ntiles; t0 += kMaxGrid) {
+        const long long g = std::min(kMaxGrid, ntiles - t0);
+        k_edge_screen<TILE><<<(unsigned)g, kThreads, 0, s>>>(
+            pts, (int)n, d, thr_sq, t0, (unsigned long long*)keys, (unsigned long long*)vals,
+            (long long)cap, (unsigned long long*)count_dev);
+        FPH_LAUNCH_CHECK();
+    }
+    return FPH_OK;
+}
+
 }  // namespace
 
 extern "C" int fph_edge_screen(const float* pts, int64_t n, int d, float thr_sq,
@@ -218,17 +241,11 @@ extern "C" int fph_edge_screen(const float* pts, int64_t n, int d, float thr_sq,
     k_zero_u64<<<1, 64, 0, s>>>((unsigned long long*)count_dev, 1);
     FPH_LAUNCH_CHECK();
     if (n < 2) return FPH_OK;
-    const long long nb = (n + kTile - 1) / kTile;
-    const long long ntiles = nb * (nb + 1) / 2;
-    const long long kMaxGrid = 1LL << 30;
-    for (long long t0 = 0; t0 < ntiles; t0 += kMaxGrid) {
-        const long long g = std::min(kMaxGrid, ntiles - t0);
-        k_edge_screen<<<(unsigned)g, kThreads, 0, s>>>(
-            pts, (int)n, d, thr_sq, t0, (unsigned long long*)keys, (unsigned long long*)vals,
-            (long long)cap, (unsigned long long*)count_dev);
-        FPH_LAUNCH_CHECK();
+    switch (fph_tune_value(FPH_TUNE_SCREEN_TILE, n, d)) {
+        case 32: return launch_screen<32>(pts, n, d, thr_sq, keys, vals, cap, count_dev, s);
+        case 128: return launch_screen<128>(pts, n, d, thr_sq, keys, vals, cap, count_dev, s);
+        default: return launch_screen<64>(pts, n, d, thr_sq, keys, vals, cap, count_dev, s);
     }
-    return FPH_OK;
 }
 
 extern "C" int fph_edge_sort(const uint64_t* keys_in, const uint64_t* vals_in, int64_t E,

@@ -411,7 +411,8 @@ extern "C" int fph_triangle_count(const int32_t* edge_i, const int32_t* edge_j, 
     int* own = (int*)(w + tmp);
     int* cof = (int*)(w + tmp + align_up(4 * (size_t)std::max<int64_t>(E, 1)));
     if (E > 0) {
-        k_tri_count<<<fph_blocks(E, 256), 256, 0, s>>>(edge_i, edge_j, E, adj_ptr, adj_idx,
+        const int tb = fph_tune_value(FPH_TUNE_ENUM_BLOCK, E, 0);
+        k_tri_count<<<fph_blocks(E, tb), tb, 0, s>>>(edge_i, edge_j, E, adj_ptr, adj_idx,
                                                       adj_rank, mst_mask, own, cof);
         FPH_LAUNCH_CHECK();
     }
@@ -429,7 +430,8 @@ extern "C" int fph_triangle_emit(const int32_t* edge_i, const int32_t* edge_j, i
     hipStream_t s = (hipStream_t)stream;
     if (E < 0) return FPH_ERR_ARG;
     if (E == 0) return FPH_OK;
-    k_tri_emit<<<fph_blocks(E, 256), 256, 0, s>>>(edge_i, edge_j, E, adj_ptr, adj_idx, adj_rank,
+    const int tb = fph_tune_value(FPH_TUNE_ENUM_BLOCK, E, 0);
+    k_tri_emit<<<fph_blocks(E, tb), tb, 0, s>>>(edge_i, edge_j, E, adj_ptr, adj_idx, adj_rank,
                                                  mst_mask, tri_off, tri_w, tri_e, tri_ru, tri_rv,
                                                  tri_cmf);
     FPH_LAUNCH_CHECK();
@@ -451,7 +453,8 @@ extern "C" int fph_cofacets_h1(const int32_t* edge_i, const int32_t* edge_j, int
     if (*ws_bytes < need) return FPH_ERR_ARG;
     if (C == 0) return FPH_OK;
     unsigned* raw = (unsigned*)((char*)ws + tmp);
-    k_cof1_emit<<<fph_blocks(E, 256), 256, 0, s>>>(edge_i, edge_j, E, adj_ptr, adj_idx, adj_rank,
+    const int tb = fph_tune_value(FPH_TUNE_ENUM_BLOCK, E, 0);
+    k_cof1_emit<<<fph_blocks(E, tb), tb, 0, s>>>(edge_i, edge_j, E, adj_ptr, adj_idx, adj_rank,
                                                   mst_mask, tri_off, tri_w, cof_off, raw);
     FPH_LAUNCH_CHECK();
     size_t t = tmp;
@@ -476,7 +479,8 @@ extern "C" int fph_apparent(int64_t K, int64_t col_base, const int32_t* col_off,
     k_fill_i32<<<1, 64, 0, s>>>(res_count, 1, 0);
     FPH_LAUNCH_CHECK();
     if (K == 0) return FPH_OK;
-    k_apparent<<<fph_blocks(K, 256), 256, 0, s>>>(K, col_base, col_off, col_data, skip_mask,
+    const int tb = fph_tune_value(FPH_TUNE_APPARENT_BLOCK, K, 0);
+    k_apparent<<<fph_blocks(K, tb), tb, 0, s>>>(K, col_base, col_off, col_data, skip_mask,
                                                  row_cmf, row_base, kind, pivot, res_list,
                                                  res_count);
     FPH_LAUNCH_CHECK();
@@ -493,7 +497,8 @@ extern "C" int fph_clear_pivots(int64_t K, const uint8_t* kind, const int32_t* p
         FPH_LAUNCH_CHECK();
     }
     if (K == 0) return FPH_OK;
-    k_clear_pivots<<<fph_blocks(K, 256), 256, 0, s>>>(K, kind, pivot, row_base, cleared);
+    const int tb = fph_tune_value(FPH_TUNE_APPARENT_BLOCK, K, 0);
+    k_clear_pivots<<<fph_blocks(K, tb), tb, 0, s>>>(K, kind, pivot, row_base, cleared);
     FPH_LAUNCH_CHECK();
     return FPH_OK;
 }
@@ -510,7 +515,8 @@ extern "C" int fph_tet_count(int64_t E, const int32_t* adj_ptr, const int32_t* a
     if (*ws_bytes < need) return FPH_ERR_ARG;
     int* cnt = (int*)((char*)ws + tmp);
     if (E > 0) {
-        k_tet_count<<<fph_blocks(E, 256), 256, 0, s>>>(E, adj_ptr, adj_idx, adj_rank, tri_off,
+        const int tb = fph_tune_value(FPH_TUNE_ENUM_BLOCK, E, 0);
+        k_tet_count<<<fph_blocks(E, tb), tb, 0, s>>>(E, adj_ptr, adj_idx, adj_rank, tri_off,
                                                       tri_w, cnt);
         FPH_LAUNCH_CHECK();
     }
@@ -526,7 +532,8 @@ extern "C" int fph_tet_emit(int64_t E, const int32_t* edge_i, const int32_t* edg
     hipStream_t s = (hipStream_t)stream;
     if (E < 0) return FPH_ERR_ARG;
     if (E == 0) return FPH_OK;
-    k_tet_emit<<<fph_blocks(E, 256), 256, 0, s>>>(E, edge_i, edge_j, adj_ptr, adj_idx, adj_rank,
+    const int tb = fph_tune_value(FPH_TUNE_ENUM_BLOCK, E, 0);
+    k_tet_emit<<<fph_blocks(E, tb), tb, 0, s>>>(E, edge_i, edge_j, adj_ptr, adj_idx, adj_rank,
                                                  tri_off, tri_w, tri_ru, tri_rv, cleared, tet_off,
                                                  tet_key, tet_e, tet_cmf);
     FPH_LAUNCH_CHECK();
@@ -546,7 +553,8 @@ extern "C" int fph_cofacets_h2_count(int64_t T, const int32_t* edge_i, const int
     if (*ws_bytes < need) return FPH_ERR_ARG;
     int* cnt = (int*)((char*)ws + tmp);
     if (T > 0) {
-        k_cof2_count<<<fph_blocks(T, 256), 256, 0, s>>>(T, edge_i, edge_j, adj_ptr, adj_idx,
+        const int tb = fph_tune_value(FPH_TUNE_ENUM_BLOCK, T, 0);
+        k_cof2_count<<<fph_blocks(T, tb), tb, 0, s>>>(T, edge_i, edge_j, adj_ptr, adj_idx,
                                                        tri_w, tri_e, cleared, cnt);
         FPH_LAUNCH_CHECK();
     }
@@ -569,7 +577,8 @@ extern "C" int fph_cofacets_h2(int64_t T, int64_t E, int64_t Q, const int32_t* e
     if (*ws_bytes < need) return FPH_ERR_ARG;
     if (C == 0) return FPH_OK;
     unsigned* raw = (unsigned*)((char*)ws + tmp);
-    k_cof2_emit<<<fph_blocks(T, 256), 256, 0, s>>>(T, E, edge_i, edge_j, adj_ptr, adj_idx,
+    const int tb = fph_tune_value(FPH_TUNE_ENUM_BLOCK, T, 0);
+    k_cof2_emit<<<fph_blocks(T, tb), tb, 0, s>>>(T, E, edge_i, edge_j, adj_ptr, adj_idx,
                                                   adj_rank, tri_w, tri_e, cleared, tet_off,
                                                   tet_key, cof_off, raw);
     FPH_LAUNCH_CHECK();

@@ -253,10 +253,12 @@ def build_h1(cx: Complex, max_triangles: int = MAX_TRIANGLES, semantics: str = "
     cx.tri_off = _empty(torch.int32, dev, E + 1)
     cx.cof_off = _empty(torch.int32, dev, E + 1)
     totals = _empty(torch.int64, dev, 2)
+    tok = TIMER.start("enumerate", dev)
     call_ws("fph_triangle_count",
             (ptr(cx.edge_i), ptr(cx.edge_j), E, ptr(cx.adj_ptr), ptr(cx.adj_idx),
              ptr(cx.adj_rank), ptr(cx.mst_mask), ptr(cx.tri_off), ptr(cx.cof_off), ptr(totals)),
             (s,), dev)
+    TIMER.stop(tok)
     segs = None
     if cx.edge_off is None:
         T, C = (int(x) for x in totals.tolist())
@@ -288,15 +290,17 @@ def build_h1(cx: Complex, max_triangles: int = MAX_TRIANGLES, semantics: str = "
     cx.tri_ru = _empty(torch.int32, dev, max(T, 1))
     cx.tri_rv = _empty(torch.int32, dev, max(T, 1))
     cx.tri_cmf = _empty(torch.int32, dev, max(T, 1))
+    cx.cof_data = _empty(torch.int32, dev, max(C, 1))
+    tok = TIMER.start("enumerate", dev)
     check(lib.fph_triangle_emit(ptr(cx.edge_i), ptr(cx.edge_j), E, ptr(cx.adj_ptr),
                                 ptr(cx.adj_idx), ptr(cx.adj_rank), ptr(cx.mst_mask),
                                 ptr(cx.tri_off), ptr(cx.tri_w), ptr(cx.tri_e), ptr(cx.tri_ru),
                                 ptr(cx.tri_rv), ptr(cx.tri_cmf), s), "fph_triangle_emit")
-    cx.cof_data = _empty(torch.int32, dev, max(C, 1))
     call_ws("fph_cofacets_h1",
             (ptr(cx.edge_i), ptr(cx.edge_j), E, T, ptr(cx.adj_ptr), ptr(cx.adj_idx),
              ptr(cx.adj_rank), ptr(cx.mst_mask), ptr(cx.tri_off), ptr(cx.tri_w),
              ptr(cx.cof_off), C, ptr(cx.cof_data)), (s,), dev)
+    TIMER.stop(tok)
     # the reference's store counts 3 cofacet entries per triangle and all E
     # edges as columns (reduce_h1_gpu.py:523-534)
     shapes = ([(3 * T, E, T)] if segs is None else
@@ -395,9 +399,11 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
     out = _empty(torch.float32, device, max(K, 1), 2)
     out_edges = _empty(torch.int32, device, max(K, 1), 2) if want_edges else None
     while True:
+        tok = TIMER.start("apparent", device)
         check(lib.fph_apparent(K, col_base, ptr(off), ptr(data), ptr(skip), ptr(row_cmf),
                                row_base, nrows, ptr(kind), ptr(own), ptr(pivot), ptr(res_list),
                                rescnt_p, s), "fph_apparent")
+        TIMER.stop(tok)
         pool = _empty(torch.int32, device, pool_words + 64)  # +64: speculative line reads
         scratch = _empty(torch.uint8, device, scratch_bytes)
         tok = TIMER.start("reduce", device)
@@ -452,17 +458,21 @@ def build_h2(cx: Complex) -> None:
     dev, E, T = cx.device, cx.E, cx.T
     s = stream_of(dev)
     cx.cleared = _empty(torch.uint8, dev, max(T, 1))
+    tok = TIMER.start("apparent", dev)
     check(lib.fph_clear_pivots(E, ptr(cx.kind1), ptr(cx.pivot1), E, T, ptr(cx.cleared), s),
           "fph_clear_pivots")
+    TIMER.stop(tok)
     cx.tet_off = _empty(torch.int32, dev, E + 1)
     cx.cof2_off = _empty(torch.int32, dev, T + 1)
     totals = _empty(torch.int64, dev, 2)
+    tok = TIMER.start("enumerate", dev)
     call_ws("fph_tet_count",
             (E, ptr(cx.adj_ptr), ptr(cx.adj_idx), ptr(cx.adj_rank), ptr(cx.tri_off),
              ptr(cx.tri_w), ptr(cx.tet_off), totals.data_ptr()), (s,), dev)
     call_ws("fph_cofacets_h2_count",
             (T, ptr(cx.edge_i), ptr(cx.edge_j), ptr(cx.adj_ptr), ptr(cx.adj_idx), ptr(cx.tri_w),
              ptr(cx.tri_e), ptr(cx.cleared), ptr(cx.cof2_off), totals.data_ptr() + 8), (s,), dev)
+    TIMER.stop(tok)
     Q, C2 = (int(x) for x in totals.tolist())
     if E + T + Q >= 2**31 - 1:
         raise ValueError("simplex count exceeds the int32 rank range")
@@ -470,15 +480,17 @@ def build_h2(cx: Complex) -> None:
     cx.tet_key = _empty(torch.int32, dev, max(Q, 1))
     cx.tet_e = _empty(torch.int32, dev, max(Q, 1))
     cx.tet_cmf = _empty(torch.int32, dev, max(Q, 1))
+    cx.cof2_data = _empty(torch.int32, dev, max(C2, 1))
+    tok = TIMER.start("enumerate", dev)
     check(lib.fph_tet_emit(E, ptr(cx.edge_i), ptr(cx.edge_j), ptr(cx.adj_ptr), ptr(cx.adj_idx),
                            ptr(cx.adj_rank), ptr(cx.tri_off), ptr(cx.tri_w), ptr(cx.tri_ru),
                            ptr(cx.tri_rv), ptr(cx.cleared), ptr(cx.tet_off), ptr(cx.tet_key),
                            ptr(cx.tet_e), ptr(cx.tet_cmf), s), "fph_tet_emit")
-    cx.cof2_data = _empty(torch.int32, dev, max(C2, 1))
     call_ws("fph_cofacets_h2",
             (T, E, Q, ptr(cx.edge_i), ptr(cx.edge_j), ptr(cx.adj_ptr), ptr(cx.adj_idx),
              ptr(cx.adj_rank), ptr(cx.tri_w), ptr(cx.tri_e), ptr(cx.cleared), ptr(cx.tet_off),
              ptr(cx.tet_key), ptr(cx.cof2_off), C2, ptr(cx.cof2_data)), (s,), dev)
+    TIMER.stop(tok)
     cx.kind2, cx.pivot2, h2, stats = _reduce_and_diagram(
         K=T, col_base=E, off=cx.cof2_off, data=cx.cof2_data, skip=cx.cleared,
         row_cmf=cx.tet_cmf, row_base=E + T, nrows=Q, C=C2, col_edge=cx.tri_e,

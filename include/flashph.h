@@ -416,6 +416,34 @@ int fph_backward_scatter(const float *pts, const int64_t *foff, const int32_t *d
                          const float *contrib_val, int64_t M, float *grad_pts, void *ws,
                          size_t *ws_bytes, void *stream);
 
+/* ---- autotune table ---------------------------------------------------- */
+
+/*
+ * Launch parameters that change the tiling, never the results.  Values are
+ * kept per bucket (floor(log2 size), floor(log2 d)), host memory only; 0 =
+ * the default.  Keys: SCREEN_TILE by (n, d) of fph_edge_screen;
+ * ENUM_BLOCK by (E, 0) for the triangle / tetrahedron / cofacet kernels
+ * (by (T, 0) for fph_cofacets_h2_count and fph_cofacets_h2); APPARENT_BLOCK by
+ * (K, 0) for fph_apparent and fph_clear_pivots.  The reference picks its
+ * Triton BLOCK sizes as constants (kernels/edge_kernel.py:193-222,
+ * kernels/triangle_kernel.py:366-397, reduce_h1_gpu.py:159-223); this table
+ * replaces them with values a tuner measures (flash_ph.autotune).
+ */
+#define FPH_TUNE_SCREEN_TILE 0    /* points per tile side: 32, 64 (default), 128 */
+#define FPH_TUNE_ENUM_BLOCK 1     /* threads per workgroup: 64 .. 1024 (default 256) */
+#define FPH_TUNE_APPARENT_BLOCK 2 /* threads per workgroup: 64 .. 1024 (default 256) */
+#define FPH_TUNE_NKNOBS 3
+
+/* Candidate values of `knob` into out[0..cap); returns their number. */
+int fph_autotune_candidates(int knob, int32_t *out, int cap);
+/* Sets the value for the bucket of (size, d); value 0 restores the default.
+ * FPH_ERR_ARG for an unknown knob or a value not among the candidates. */
+int fph_autotune_set(int knob, int64_t size, int64_t d, int value);
+/* Value in effect for (size, d), or FPH_ERR_ARG. */
+int fph_autotune_query(int knob, int64_t size, int64_t d);
+/* Clears every bucket back to the defaults. */
+void fph_autotune_reset(void);
+
 #ifdef __cplusplus
 }
 #endif
