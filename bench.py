@@ -193,6 +193,40 @@ def bench_cfg4() -> dict:
     return out
 
 
+def bench_graph_h0(pts, reps: int = 50) -> dict:
+    """H0 only: eager rips_persistence(max_dim=0) against the same chain
+    replayed as one hipGraph (RipsH0Graph), both bit-exact, on the n=10K d=10
+    cloud (GPU-bound) and on a small n=1000 d=4 cloud (launch-bound)."""
+    import numpy as np
+    import torch
+    from flash_ph import rips_persistence
+    from flash_ph.graph import RipsH0Graph
+
+    def timeit(fn):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return 1000.0 * (time.perf_counter() - t0) / reps
+
+    small = torch.from_numpy(
+        np.random.default_rng(1).standard_normal((1000, 4), dtype=np.float32)).cuda()
+    out = {}
+    for name, x, t in (("n10000_d10_t1.6", pts, T_EDGE), ("n1000_d4_t1.0", small, 1.0)):
+        n, d = x.shape
+        g = RipsH0Graph(n, d, t)
+        same = bool(torch.equal(g(x), rips_persistence(x, t, max_dim=0)[0]))
+        eager_ms = timeit(lambda: rips_persistence(x, t, max_dim=0))
+        graph_ms = timeit(lambda: g(x))
+        out[name] = {"eager_ms": round(eager_ms, 4), "graph_ms": round(graph_ms, 4),
+                     "edge_capacity": g.cap, "E": g.E, "bit_exact_vs_eager": same}
+        g.close()
+    return out
+
+
 def bench_cfg5() -> dict:
     """64 Clifford-torus clouds (n=2000, d=4, noise 0.02, t=0.6), loss =
     total finite H1 persistence: forward + hand-written backward, and the
@@ -365,6 +399,7 @@ def main():
     if not args.no_extra:
         out["precision_n200k_d50"] = bench_cfg4()
         out["autograd_clifford_64x2000"] = bench_cfg5()
+        out["graph_h0"] = bench_graph_h0(pts)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pts_np)
     print(json.dumps(out), flush=True)

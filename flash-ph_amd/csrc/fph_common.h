@@ -24,6 +24,17 @@
 // Value of autotune knob `knob` in effect for (size, d) (defined in fph_tune.hip).
 int fph_tune_value(int knob, int64_t size, int64_t d);
 
+// Capacity forms of the edge sort + vertex CSR (fph_edges.hip) and of the H0
+// diagram (fph_mst.hip) for graph capture: sizes are the host capacity, the
+// live edge count is read on the device, and no memset node is recorded.
+int fph_edges_csr_capacity(uint64_t* keys, uint64_t* vals, const uint64_t* count_dev,
+                           int64_t cap, int64_t n, int32_t* edge_i, int32_t* edge_j,
+                           float* edge_dsq, float* edge_filt, int32_t* adj_ptr, int32_t* adj_idx,
+                           int32_t* adj_rank, float* adj_dsq, void* ws, size_t* ws_bytes,
+                           hipStream_t s);
+int fph_h0_diagram_capacity(const uint8_t* mst_mask, const float* edge_dsq, int64_t cap,
+                            int64_t n, float* h0, void* ws, size_t* ws_bytes, hipStream_t s);
+
 static inline unsigned fph_blocks(int64_t n, int threads) {
     int64_t b = (n + threads - 1) / threads;
     if (b < 1) b = 1;

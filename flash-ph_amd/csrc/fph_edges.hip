@@ -169,10 +169,17 @@ __global__ void k_edge_unpack(const unsigned long long* __restrict__ keys,
     if (filt) filt[e] = fph_sqrt_rn(f);
 }
 
+// Half-edges (i, j) and (j, i) of every edge; an unused slot of a capacity-
+// sized edge list (ei = -1) gives two all-ones keys, which sort last.
 __global__ void k_half_edges(const int* __restrict__ ei, const int* __restrict__ ej, long long E,
                              unsigned long long* __restrict__ hk, unsigned* __restrict__ hv) {
     long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
+    if (ei[e] < 0) {
+        hk[e] = hk[E + e] = ~0ull;
+        hv[e] = hv[E + e] = 0xFFFFFFFFu;
+        return;
+    }
     const unsigned long long i = (unsigned)ei[e], j = (unsigned)ej[e];
     hk[e] = (i << 32) | j;
     hv[e] = (unsigned)e;
@@ -186,6 +193,12 @@ __global__ void k_csr_unpack(const unsigned long long* __restrict__ hk,
                              int* __restrict__ adj_rank, float* __restrict__ adj_dsq) {
     long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= H) return;
+    if (hk[h] == ~0ull) {  // unused slot (capacity-sized lists), past the last row
+        adj_idx[h] = -1;
+        adj_rank[h] = -1;
+        adj_dsq[h] = 0.0f;
+        return;
+    }
     const unsigned r = hv[h];
     adj_idx[h] = (int)(hk[h] & 0xFFFFFFFFull);
     adj_rank[h] = (int)r;
@@ -213,6 +226,27 @@ int bit_width_u64(unsigned long long x) {
     while (x) { b++; x >>= 1; }
     return b;
 }
+
+// Slots [count, cap) of a screened-pair buffer get all-ones keys and values,
+// so a sort over all cap slots puts the real edges first and unpacks the
+// rest as unused (ei = ej = -1).
+__global__ void k_pad_tail(unsigned long long* __restrict__ keys,
+                           unsigned long long* __restrict__ vals,
+                           const unsigned long long* __restrict__ count, long long cap) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cap) return;
+    const unsigned long long c = *count;
+    if ((unsigned long long)i >= c) {
+        keys[i] = ~0ull;
+        vals[i] = ~0ull;
+    }
+}
+
+struct U64Less {
+    __device__ __host__ bool operator()(unsigned long long a, unsigned long long b) const {
+        return a < b;
+    }
+};
 
 template <int TILE>
 int launch_screen(const float* pts, int64_t n, int d, float thr_sq, uint64_t* keys,
@@ -349,3 +383,58 @@ extern "C" int fph_vertex_csr(const int32_t* edge_i, const int32_t* edge_j,
 }
 
 extern "C" const char* fph_version(void) { return "flashph-mi355x 0.1.0 (gfx950)"; }
+
+// ---------------------------------------------------------------------------
+// Capacity form for graph capture: every size is the capacity `cap` fixed on
+// the host, the live edge count is read on the device, and nothing records a
+// memset node or synchronises (sorts use rocprim's merge sort, whose launches
+// are kernels only).  Requires n <= 92682, where the packed keys are unique
+// and the sort order needs no emission-order tie-break.
+// ---------------------------------------------------------------------------
+
+int fph_edges_csr_capacity(uint64_t* keys, uint64_t* vals, const uint64_t* count_dev,
+                           int64_t cap, int64_t n, int32_t* edge_i, int32_t* edge_j,
+                           float* edge_dsq, float* edge_filt, int32_t* adj_ptr, int32_t* adj_idx,
+                           int32_t* adj_rank, float* adj_dsq, void* ws, size_t* ws_bytes,
+                           hipStream_t s) {
+    if (cap < 1 || n < 1 || !ws_bytes) return FPH_ERR_ARG;
+    if (n > 92682 || cap >= 0x3FFFFFFFLL) return FPH_ERR_LIMIT;
+    const size_t H = 2 * (size_t)cap;
+    size_t t_e = 0, t_h = 0;
+    using K = unsigned long long;
+    FPH_CHECK_HIP(rocprim::merge_sort(nullptr, t_e, (K*)nullptr, (K*)nullptr, (K*)nullptr,
+                                      (K*)nullptr, (size_t)cap, U64Less(), s));
+    FPH_CHECK_HIP(rocprim::merge_sort(nullptr, t_h, (K*)nullptr, (K*)nullptr, (unsigned*)nullptr,
+                                      (unsigned*)nullptr, H, U64Less(), s));
+    const size_t tmp = align_up(std::max(t_e, t_h));
+    // [sort temp | sorted keys | sorted vals | hk | hk2 | hv | hv2]
+    const size_t need = tmp + 2 * align_up(8 * (size_t)cap) + 2 * align_up(8 * H) +
+                        2 * align_up(4 * H);
+    if (!ws) { *ws_bytes = need; return FPH_OK; }
+    if (*ws_bytes < need) return FPH_ERR_ARG;
+    char* w = (char*)ws;
+    K* ks = (K*)(w + tmp);
+    K* vs = (K*)(w + tmp + align_up(8 * (size_t)cap));
+    char* w2 = w + tmp + 2 * align_up(8 * (size_t)cap);
+    K* hk = (K*)w2;
+    K* hk2 = (K*)(w2 + align_up(8 * H));
+    unsigned* hv = (unsigned*)(w2 + 2 * align_up(8 * H));
+    unsigned* hv2 = (unsigned*)(w2 + 2 * align_up(8 * H) + align_up(4 * H));
+    k_pad_tail<<<fph_blocks(cap, 256), 256, 0, s>>>((K*)keys, (K*)vals, (const K*)count_dev, cap);
+    FPH_LAUNCH_CHECK();
+    size_t t1 = tmp;
+    FPH_CHECK_HIP(rocprim::merge_sort(ws, t1, (K*)keys, ks, (K*)vals, vs, (size_t)cap, U64Less(), s));
+    k_edge_unpack<<<fph_blocks(cap, 256), 256, 0, s>>>(ks, vs, cap, edge_i, edge_j, edge_dsq,
+                                                       edge_filt);
+    FPH_LAUNCH_CHECK();
+    k_half_edges<<<fph_blocks(cap, 256), 256, 0, s>>>(edge_i, edge_j, cap, hk, hv);
+    FPH_LAUNCH_CHECK();
+    size_t t2 = tmp;
+    FPH_CHECK_HIP(rocprim::merge_sort(ws, t2, hk, hk2, hv, hv2, H, U64Less(), s));
+    k_csr_unpack<<<fph_blocks((int64_t)H, 256), 256, 0, s>>>(hk2, hv2, (long long)H, edge_dsq,
+                                                            adj_idx, adj_rank, adj_dsq);
+    FPH_LAUNCH_CHECK();
+    k_csr_ptr<<<fph_blocks(n + 1, 256), 256, 0, s>>>(hk2, (long long)H, n, adj_ptr);
+    FPH_LAUNCH_CHECK();
+    return FPH_OK;
+}

@@ -40,7 +40,9 @@ __global__ void k_bvk_find(BvkState st, const int* __restrict__ ei, const int* _
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) st.nxt[i] = (int)i;
     if (i >= E) return;
-    const int cu = st.comp[ei[i]], cv = st.comp[ej[i]];
+    const int a = ei[i];
+    if (a < 0) return;  // unused slot of a capacity-sized edge list
+    const int cu = st.comp[a], cv = st.comp[ej[i]];
     if (cu == cv) return;
     atomicMin(&st.min_e[cu], (unsigned)i);
     atomicMin(&st.min_e[cv], (unsigned)i);
@@ -137,7 +139,95 @@ __global__ void k_h0_write(const unsigned char* __restrict__ mst, const int* __r
 
 inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
+// Exclusive scan of the forest mask in three launches and no memset (for
+// graph capture): per-block counts, one-block scan of the counts, per-block
+// scan plus offset.
+constexpr int kScanBlock = 1024;
+
+__device__ __forceinline__ int block_excl_sum(int v, int* sh, int* total) {
+    // 1024 threads = 16 wavefronts: wave scans, then a scan of the wave sums
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wv] = x;
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        int t = sh[threadIdx.x];
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const int y = __shfl_up(t, o, 16);
+            if ((int)threadIdx.x >= o) t += y;
+        }
+        sh[16 + threadIdx.x] = t;
+    }
+    __syncthreads();
+    const int before = wv ? sh[16 + wv - 1] : 0;
+    if (total) *total = sh[31];
+    const int r = before + x - v;
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_mask_block_count(
+    const unsigned char* __restrict__ m, long long E, int* __restrict__ bsum) {
+    __shared__ int sh[32];
+    const long long i = (long long)blockIdx.x * kScanBlock + threadIdx.x;
+    const int v = i < E ? (int)m[i] : 0;
+    int tot;
+    block_excl_sum(v, sh, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_block_counts(int* __restrict__ bsum, int nb) {
+    __shared__ int sh[32];
+    int carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += kScanBlock) {
+        const int b = b0 + (int)threadIdx.x;
+        const int v = b < nb ? bsum[b] : 0;
+        int tot;
+        const int ex = block_excl_sum(v, sh, &tot);
+        if (b < nb) bsum[b] = carry + ex;
+        carry += tot;
+    }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_mask_block_scan(
+    const unsigned char* __restrict__ m, long long E, const int* __restrict__ bsum,
+    int* __restrict__ pos) {
+    __shared__ int sh[32];
+    const long long i = (long long)blockIdx.x * kScanBlock + threadIdx.x;
+    const int v = i < E ? (int)m[i] : 0;
+    const int ex = block_excl_sum(v, sh, nullptr);
+    if (i < E) pos[i] = bsum[blockIdx.x] + ex;
+}
+
 }  // namespace
+
+// Capacity form of fph_h0_diagram for graph capture: E = cap slots (unused
+// slots have mask 0), no memset nodes.
+int fph_h0_diagram_capacity(const uint8_t* mst_mask, const float* edge_dsq, int64_t cap,
+                            int64_t n, float* h0, void* ws, size_t* ws_bytes, hipStream_t s) {
+    if (cap < 1 || n < 1 || !ws_bytes) return FPH_ERR_ARG;
+    const long long nb = (cap + kScanBlock - 1) / kScanBlock;
+    const size_t need = align_up(4 * (size_t)nb) + align_up(4 * (size_t)cap);
+    if (!ws) { *ws_bytes = need; return FPH_OK; }
+    if (*ws_bytes < need) return FPH_ERR_ARG;
+    int* bsum = (int*)ws;
+    int* pos = (int*)((char*)ws + align_up(4 * (size_t)nb));
+    k_mask_block_count<<<(unsigned)nb, kScanBlock, 0, s>>>(mst_mask, cap, bsum);
+    k_scan_block_counts<<<1, kScanBlock, 0, s>>>(bsum, (int)nb);
+    k_mask_block_scan<<<(unsigned)nb, kScanBlock, 0, s>>>(mst_mask, cap, bsum, pos);
+    FPH_LAUNCH_CHECK();
+    const long long span = std::max<long long>(cap, n);
+    k_h0_write<<<fph_blocks(span, 256), 256, 0, s>>>(mst_mask, pos, edge_dsq, cap, n, h0, nullptr,
+                                                    nullptr);
+    FPH_LAUNCH_CHECK();
+    return FPH_OK;
+}
 
 extern "C" int fph_boruvka_rounds(int64_t n) {
     int64_t m = n < 2 ? 2 : n;

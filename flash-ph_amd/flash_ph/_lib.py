@@ -69,6 +69,12 @@ SIGNATURES = {
     "fph_edge_screen_batch": (_I32, [_P, _P, _P, _P, _P, _P, _I64, _I64, _P, _P, _I64, _P, _P]),
     "fph_edge_sort_batch": (_I32, [_P, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _PSZ, _P]),
     "fph_h0_diagram_batch": (_I32, [_P, _P, _I64, _P, _P, _I64, _I64, _P, _P, _PSZ, _P]),
+    "fph_h0_pipeline": (_I32, [_P, _I64, _I32, _F32, _I32, _I64, _P, _P, _P, _P, _P, _P, _P,
+                               _P, _P, _P, _P, _P, _P, _PSZ, _P]),
+    "fph_h0_graph_create": (_I32, [_P, _I64, _I32, _F32, _I32, _I64, _P, _P, _P, _P, _P, _P,
+                                   _P, _P, _P, _P, _P, _P, _P, _SZ, _P]),
+    "fph_graph_launch": (_I32, [_P, _P]),
+    "fph_graph_destroy": (_I32, [_P]),
     "fph_autotune_candidates": (_I32, [_I32, _P, _I32]),
     "fph_autotune_set": (_I32, [_I32, _I64, _I64, _I32]),
     "fph_autotune_query": (_I32, [_I32, _I64, _I64]),

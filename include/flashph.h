@@ -416,6 +416,44 @@ int fph_backward_scatter(const float *pts, const int64_t *foff, const int32_t *d
                          const float *contrib_val, int64_t M, float *grad_pts, void *ws,
                          size_t *ws_bytes, void *stream);
 
+/* ---- graph-captured H0 pipeline ---------------------------------------- */
+
+/*
+ * Stages (a) and (b) in capacity form, for graph capture: screen (screen 0
+ * direct, 1 split-bf16) -> sort -> vertex CSR -> Boruvka -> H0 diagram with
+ * every size fixed at `cap` edge slots on the host.  The live edge count is
+ * count_dev[0] (device uint64[2]; [1] = split-bf16 candidates); slots past
+ * it hold edge_i = edge_j = -1 and adj_idx = adj_rank = -1 beyond
+ * adj_ptr[n].  Requires 2 <= n <= 92682.  Results equal fph_edge_screen +
+ * fph_edge_sort + fph_vertex_csr + fph_boruvka_mst + fph_h0_diagram
+ * whenever count_dev[0] <= cap; otherwise the edges past cap are lost and
+ * the caller retries with cap >= count_dev[0].  Buffers: edge_* and
+ * mst_mask cap entries, adj_idx/rank/dsq 2 cap, adj_ptr n+1, h0 2n floats,
+ * converged int32[1].  Records kernels only: no memset node, no host
+ * synchronisation.  Replaces, for repeated calls on one shape, the
+ * host-driven sequence of persistence.py:140-186.
+ */
+int fph_h0_pipeline(const float *pts, int64_t n, int d, float thr_sq, int screen, int64_t cap,
+                    uint64_t *count_dev, int32_t *edge_i, int32_t *edge_j, float *edge_dsq,
+                    float *edge_filt, int32_t *adj_ptr, int32_t *adj_idx, int32_t *adj_rank,
+                    float *adj_dsq, uint8_t *mst_mask, float *h0, int32_t *converged, void *ws,
+                    size_t *ws_bytes, void *stream);
+
+/*
+ * Captures fph_h0_pipeline on these buffers (ws from its size query) into a
+ * hipGraph and instantiates it; *graph_out receives an opaque handle.  Every
+ * buffer must outlive the graph.  fph_graph_launch replays it on `stream`
+ * (refill pts first); fph_graph_destroy frees it.  No counterpart in the
+ * reference (which launches every stage from Python each call).
+ */
+int fph_h0_graph_create(const float *pts, int64_t n, int d, float thr_sq, int screen,
+                        int64_t cap, uint64_t *count_dev, int32_t *edge_i, int32_t *edge_j,
+                        float *edge_dsq, float *edge_filt, int32_t *adj_ptr, int32_t *adj_idx,
+                        int32_t *adj_rank, float *adj_dsq, uint8_t *mst_mask, float *h0,
+                        int32_t *converged, void *ws, size_t ws_bytes, void **graph_out);
+int fph_graph_launch(void *graph, void *stream);
+int fph_graph_destroy(void *graph);
+
 /* ---- autotune table ---------------------------------------------------- */
 
 /*

@@ -121,3 +121,17 @@ def test_thresholds_need_gpu_except_trivial():
             auto_threshold(pts)
         with pytest.raises((RuntimeError, ImportError)):
             enclosing_radius(pts)
+
+
+def test_graph_argument_errors_without_gpu():
+    from flash_ph.graph import RipsH0Graph
+    with pytest.raises(ValueError):
+        RipsH0Graph(100, 3, -1.0)
+    with pytest.raises(ValueError):
+        RipsH0Graph(1, 3, 1.0)
+    with pytest.raises(ValueError):
+        RipsH0Graph(92_683, 3, 1.0)
+    with pytest.raises(ValueError):
+        RipsH0Graph(100, 0, 1.0)
+    with pytest.raises(RuntimeError):
+        RipsH0Graph(100, 3, 1.0)
