@@ -25,9 +25,13 @@
 //            column; then it publishes the content and claims the pivot with
 //            one 64-bit atomicMax.  A lost race continues with the winner; a
 //            displaced lower-rank owner goes back into the queue.  Ends when
-//            no column is pending.  (Across workgroups the claims would need
-//            device-scope fences, which on a multi-XCD GPU write back and
-//            invalidate the L2s on every claim: measured 20x slower.)
+//            no column is pending.  With one segment the queue can instead
+//            span several workgroups (k_reduce_queue_dev, the default from
+//            8192 columns): ordering claims with device-scope fences would
+//            write back and invalidate the per-XCD L2s on every claim
+//            (measured 20x slower), so there published columns are
+//            write-once over a fill value and readers wait, word by word,
+//            with device-coherent loads until the column is complete.
 //
 // A column is a sorted list of row ranks stored in a bump-allocated pool as
 // [length, rows...].  XOR of two sorted lists is a merge with cancellation
@@ -232,16 +236,16 @@ struct Alloc {
     // Two off-chip working buffers per wavefront (pool offsets, capacities
     // in words), reused by every long XOR of the wavefront; grown by
     // doubling when a longer result comes along.
-    unsigned long long big_off[2];
-    int big_cap[2];
+    unsigned long long big_off[3];
+    int big_cap[3];  // [2]: staging of long owner columns (device-wide queue)
 };
 
 __device__ __forceinline__ Alloc make_alloc(unsigned long long chunk) {
     Alloc al;
     al.cur = al.end = 0;
     al.chunk = chunk;
-    al.big_off[0] = al.big_off[1] = 0;
-    al.big_cap[0] = al.big_cap[1] = 0;
+    al.big_off[0] = al.big_off[1] = al.big_off[2] = 0;
+    al.big_cap[0] = al.big_cap[1] = al.big_cap[2] = 0;
     return al;
 }
 
@@ -318,17 +322,147 @@ __device__ unsigned* big_buffer(const RedArgs& a, Alloc& al, int i, int n) {
     return a.pool + al.big_off[i];
 }
 
+// ---- device-coherent access for the device-wide queue --------------------
+// Columns published during that queue are written once into pool space
+// filled beforehand with kUnwritten (no row rank or length takes that
+// value).  Readers use device-coherent loads and wait until no word of
+// [len, rows...] reads kUnwritten, so publication needs no fence: each word
+// is coherent on its own, and a column is complete once every word is.
+constexpr unsigned kUnwritten = 0xFFFFFFFFu;
+
+__device__ __forceinline__ unsigned ld_coh(const unsigned* p) {
+    return __hip_atomic_load(const_cast<unsigned*>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_coh(unsigned* p, unsigned v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ bool fail_wait(const RedArgs& a) {
+    if (fph_lane() == 0) atomicExch(&a.status[0], 5);
+    return false;
+}
+
+// Copies n published words from src to dst (wavefront-private memory),
+// waiting for words not yet written; false after `wait` ticks.
+__device__ bool copy_rows_dev(const unsigned* src, int n, unsigned* dst, long long wait) {
+    const int lane = fph_lane();
+    const long long t0 = wall_clock64();
+    for (int base = 0; base < n; base += FPH_WAVE) {
+        const int i = base + lane;
+        bool miss = i < n;
+        while (true) {
+            if (miss) {
+                const unsigned v = ld_coh(src + i);
+                if (v != kUnwritten) {
+                    dst[i] = v;
+                    miss = false;
+                }
+            }
+            if (!__ballot(miss)) break;
+            if (wall_clock64() - t0 > wait) return false;
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    wave_sync();
+    return true;
+}
+
+// One round trip for columns of up to 63 rows: lane l loads word l of the
+// published [len, rows...] at src into v; returns len (waiting for it), or
+// -1 after `wait` ticks.  The read may pass the column's end (the pool has
+// 64 words of slack past its capacity); those words are ignored.
+__device__ int spec_read_dev(const unsigned* src, unsigned& v, long long wait) {
+    const long long t0 = wall_clock64();
+    while (true) {
+        v = ld_coh(src + fph_lane());
+        const unsigned len = __shfl(v, 0);
+        if (len != kUnwritten) return (int)len;
+        if (wall_clock64() - t0 > wait) return -1;
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Rows of the column whose first 64 words are in v (spec_read_dev) into
+// dst, re-reading words not yet written; false after `wait` ticks.
+__device__ bool spec_finish_dev(const unsigned* src, unsigned v, int len, unsigned* dst,
+                                long long wait) {
+    const int lane = fph_lane();
+    const long long t0 = wall_clock64();
+    bool miss = lane >= 1 && lane <= len;
+    while (true) {
+        if (miss) {
+            if (v != kUnwritten) {
+                dst[lane - 1] = v;
+                miss = false;
+            } else {
+                v = ld_coh(src + lane);
+            }
+        }
+        if (!__ballot(miss)) break;
+        if (wall_clock64() - t0 > wait) return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    if (len > FPH_WAVE - 1)
+        return copy_rows_dev(src + FPH_WAVE, len - (FPH_WAVE - 1), dst + FPH_WAVE - 1, wait);
+    wave_sync();
+    return true;
+}
+
 // Work::half: 0 / 1 = LDS arena halves, 2 / 3 = the wavefront's off-chip
 // buffers, -1 = read-only memory (pool or CSR).
 //
 // w <- w xor (pooled column at ob), whose pivot equals w's.  False on error.
-template <int H, int O>
+// kDev: the owner column is read with device-coherent loads, waiting for
+// words not yet written (device-wide queue), into the stage area or the
+// wavefront's third off-chip buffer.
+//
+// pre (kDev, may be null): lane 0 receives the owner entry of the result's
+// pivot, loaded before the merge so that the load overlaps it (the pivot is
+// the first element of the symmetric difference, found by a short walk),
+// and np its row (0xFFFFFFFF when the result is empty).
+template <int H, int O, bool kDev = false>
 __device__ bool xor_step(Work& w, const unsigned* ob, unsigned* arena, const RedArgs& a,
-                         Alloc& al, int limit = 0x7FFFFFFF) {
-    unsigned v;
-    const int nb = read_line(ob, v);
+                         Alloc& al, int limit = 0x7FFFFFFF, long long wait = 0,
+                         unsigned long long* pre = nullptr, unsigned* np = nullptr) {
+    unsigned v = 0;
+    int nb;
+    const unsigned* bs = ob + 1;
+    if (kDev) {
+        nb = spec_read_dev(ob, v, wait);
+        if (nb < 0) return fail_wait(a);
+        unsigned* dst = nb <= O ? arena + 2 * H : big_buffer(a, al, 2, nb);
+        if (!dst) return false;
+        if (!spec_finish_dev(ob, v, nb, dst, wait)) return fail_wait(a);
+        bs = dst;
+    } else {
+        nb = read_line(ob, v);
+        if (nb <= O) {
+            finish_copy(ob, v, nb, arena + 2 * H);
+            bs = arena + 2 * H;
+        }
+    }
     const unsigned p_old = w.ptr[0];
     const int na = w.len;
+    if (kDev && pre) {
+        unsigned long long pv = kNoOwner;
+        unsigned q = 0xFFFFFFFFu;
+        if (fph_lane() == 0) {
+            const unsigned* A = w.ptr;
+            int i = 1, j = 1;
+            while (i < na && j < nb && A[i] == bs[j]) {
+                i++;
+                j++;
+            }
+            if (i < na && j < nb) q = min(A[i], bs[j]);
+            else if (i < na) q = A[i];
+            else if (j < nb) q = bs[j];
+            if (q != 0xFFFFFFFFu) pv = load_own(a, q);
+        }
+        *pre = pv;
+        *np = q;
+    }
     unsigned* out;
     int out_half;
     const bool on_chip = (w.half == 0 || w.half == 1) && na + nb <= H;
@@ -341,11 +475,6 @@ __device__ bool xor_step(Work& w, const unsigned* ob, unsigned* arena, const Red
         out = big_buffer(a, al, bi, na + nb);
         if (!out) return false;
         out_half = 2 + bi;
-    }
-    const unsigned* bs = ob + 1;
-    if (nb <= O) {
-        finish_copy(ob, v, nb, arena + 2 * H);
-        bs = arena + 2 * H;
     }
     int len;
     if (on_chip && nb <= O) len = wave_xor(w.ptr, na, bs, nb, out);
@@ -631,6 +760,223 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_queue(
     }
     if (lane == 0 && nxor) atomicAdd(&a.status[2], nxor);
     if (tid == 0) atomicMax(&a.status[1], 1);
+}
+
+// ---- device-wide queue ----------------------------------------------------
+// The queue of k_reduce_queue spread over several workgroups (one segment
+// only).  Nothing is ordered by fences: pool words are write-once over
+// kUnwritten and read with device-coherent waits (above); own[] changes only
+// by atomicMax and is read coherently; desc[c] may be read stale, which only
+// restarts c from an older published state of c (every state of c is c's
+// column plus owners of higher rank, so any of them reduces to a valid
+// column).  pivot[] is not written here: k_final_* rebuild pivot[] and
+// desc[] from the final own[] afterwards.
+
+__device__ long long publish_dev(const Work& w, const RedArgs& a, Alloc& al) {
+    const long long at = pool_alloc(a, al, w.len + 1);
+    if (at < 0) return -1;
+    for (int i = fph_lane(); i < w.len; i += FPH_WAVE) st_coh(a.pool + at + 1 + i, w.ptr[i]);
+    if (fph_lane() == 0) st_coh(a.pool + at, (unsigned)w.len);
+    wave_sync();
+    return at;
+}
+
+// Reduce column c, whose current state is published at pool offset src,
+// and claim its pivot.  Returns the displaced column (>= 0), -1 if none, -2
+// on error.
+__device__ int reduce_claim_dev(const RedArgs& a, int c, unsigned src, unsigned* arena,
+                                int* nxor, Alloc& al, long long wait) {
+    Work w;
+    unsigned v0;
+    const int len0 = spec_read_dev(a.pool + src, v0, wait);
+    if (len0 < 0) {
+        fail_wait(a);
+        return -2;
+    }
+    if (len0 <= kHalfB) {
+        w.ptr = arena;
+        w.half = 0;
+    } else {
+        unsigned* b = big_buffer(a, al, 0, len0);
+        if (!b) return -2;
+        w.ptr = b;
+        w.half = 2;
+    }
+    if (!spec_finish_dev(a.pool + src, v0, len0, const_cast<unsigned*>(w.ptr), wait)) {
+        fail_wait(a);
+        return -2;
+    }
+    w.len = len0;
+    unsigned long long pre = kNoOwner;
+    unsigned np = 0xFFFFFFFFu;
+    bool have_pre = false;
+    while (true) {
+        if (w.len == 0) return -1;
+        const unsigned p = w.ptr[0];
+        unsigned long long v;
+        // the owner entry prefetched during the last merge, if it is p's
+        if (have_pre && __shfl(np, 0) == p) v = __shfl(pre, 0);
+        else v = load_own(a, p);
+        have_pre = false;
+        if ((int)(v >> 32) - 1 <= c) {
+            const long long at = publish_dev(w, a, al);
+            if (at < 0) return -2;
+            if (fph_lane() == 0) st_coh(&a.desc[c], (unsigned)at);
+            const unsigned long long mine =
+                ((unsigned long long)(c + 1) << 32) | (unsigned long long)at;
+            unsigned long long old = 0;
+            if (fph_lane() == 0) old = atomicMax(&a.own[(long long)p - a.row_base], mine);
+            old = __shfl(old, 0);
+            const int oc = (int)(old >> 32) - 1;
+            if (oc < c) {
+                note_len(a, w.len);
+                if (oc >= 0 && fph_lane() == 0)
+                    st_coh(&a.desc[oc], (unsigned)(old & 0xFFFFFFFFull));
+                return oc;
+            }
+            v = old;
+        }
+        if (!xor_step<kHalfB, kStageB, true>(w, a.pool + (v & 0xFFFFFFFFull), arena, a, al,
+                                             0x7FFFFFFF, wait, &pre, &np))
+            return -2;
+        have_pre = true;
+        (*nxor)++;
+    }
+}
+
+// qs: [0] next residual index, [1] next ring slot to read, [2] ring tail,
+// [3] pending columns, [4] abort.
+__global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_queue_dev(
+    RedArgs a, const unsigned* __restrict__ order, const int* __restrict__ res_count,
+    unsigned long long* __restrict__ ring, int* __restrict__ qs, long long spin_limit) {
+    extern __shared__ unsigned smem[];
+    const int wid = threadIdx.x / FPH_WAVE;
+    const int lane = fph_lane();
+    unsigned* arena = smem + wid * kArenaB;
+    const int R = *res_count;
+    const unsigned cap = (unsigned)a.K;  // >= pending columns
+    int nxor = 0;
+    Alloc al = make_alloc(512);  // many wavefronts: small private chunks
+    while (true) {
+        if (ld_coh(reinterpret_cast<const unsigned*>(qs + 4))) break;
+        unsigned k = 0;
+        if (lane == 0) k = atomicAdd(reinterpret_cast<unsigned*>(qs), 1u);
+        k = __shfl(k, 0);
+        int c = -1;
+        if (k < (unsigned)R) {
+            c = (int)order[k];
+        } else {
+            unsigned slot = 0;
+            if (lane == 0) slot = atomicAdd(reinterpret_cast<unsigned*>(qs + 1), 1u);
+            slot = __shfl(slot, 0);
+            // poll the own slot every round and the shared counters (one
+            // cache line that every waiting wavefront reads) every 8th, so
+            // that idle wavefronts leave the memory path to the working ones
+            long long t_last = wall_clock64();
+            int last_pend = -1;
+            unsigned last_tail = 0xFFFFFFFFu;
+            for (unsigned it = 0;; it++) {
+                unsigned long long v = 0;
+                if (lane == 0)
+                    v = __hip_atomic_load(&ring[slot % cap], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+                v = __shfl(v, 0);
+                if ((unsigned)(v >> 32) == slot) {
+                    c = (int)(v & 0xFFFFFFFFull);
+                    break;
+                }
+                if ((it & 7) == 7) {
+                    int pend = 0, ab = 0;
+                    unsigned tail = 0;
+                    if (lane == 0) {
+                        pend = (int)ld_coh(reinterpret_cast<const unsigned*>(qs + 3));
+                        ab = (int)ld_coh(reinterpret_cast<const unsigned*>(qs + 4));
+                        tail = ld_coh(reinterpret_cast<const unsigned*>(qs + 2));
+                    }
+                    pend = __shfl(pend, 0);
+                    ab = __shfl(ab, 0);
+                    tail = __shfl(tail, 0);
+                    if (pend == 0 || ab) break;
+                    const long long now = wall_clock64();
+                    if (pend != last_pend || tail != last_tail) {
+                        last_pend = pend;
+                        last_tail = tail;
+                        t_last = now;
+                    }
+                    if (now - t_last > spin_limit) {
+                        if (lane == 0) {
+                            atomicExch(&a.status[0], 5);
+                            atomicExch(qs + 4, 1);
+                        }
+                        break;
+                    }
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            if (c < 0) break;
+        }
+        int d = -1;
+        if (a.pivot[c] >= 0) {  // zero after phase A otherwise
+            d = reduce_claim_dev(a, c, ld_coh(&a.desc[c]), arena, &nxor, al, spin_limit);
+            if (d == -2) {
+                if (lane == 0) {
+                    atomicExch(qs + 4, 1);
+                    atomicSub(qs + 3, 1);
+                }
+                break;
+            }
+        }
+        if (lane == 0) {
+            if (d >= 0) {
+                atomicAdd(qs + 3, 1);
+                const unsigned t = atomicAdd(reinterpret_cast<unsigned*>(qs + 2), 1u);
+                __hip_atomic_store(&ring[t % cap], ((unsigned long long)t << 32) | (unsigned)d,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            atomicSub(qs + 3, 1);
+        }
+    }
+    if (lane == 0 && nxor) atomicAdd(&a.status[2], nxor);
+    if (threadIdx.x == 0) atomicMax(&a.status[1], 1);
+}
+
+// Pool space not handed out yet reads kUnwritten: [*pool_ptr, end).
+__global__ void k_fill_unwritten(unsigned* __restrict__ pool,
+                                 const unsigned long long* __restrict__ pool_ptr,
+                                 long long end) {
+    const long long start = (long long)*pool_ptr;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = start + (long long)blockIdx.x * blockDim.x + threadIdx.x; i < end;
+         i += stride)
+        pool[i] = kUnwritten;
+}
+
+__global__ void k_queue_state_init(int* __restrict__ qs, const int* __restrict__ res_count,
+                                   const int* __restrict__ status) {
+    if (threadIdx.x == 0) {
+        qs[0] = 0;
+        qs[1] = 0;
+        qs[2] = 0;
+        qs[3] = *res_count;
+        qs[4] = status[0] != 0;
+    }
+}
+
+// After the device-wide queue: every nonzero column owns exactly its pivot
+// row, so own[] alone gives pivot[] and desc[] (reset first, then set).
+__global__ void k_final_reset(RedArgs a) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < a.K && (a.kind[c] == 2 || a.kind[c] == 3)) a.pivot[c] = -1;
+}
+
+__global__ void k_final_own(RedArgs a) {
+    long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.nrows) return;
+    const unsigned long long v = a.own[r];
+    if (v == kNoOwner) return;
+    const long long c = (long long)(v >> 32) - 1;
+    a.pivot[c] = (int)(r + a.row_base);
+    a.desc[c] = (unsigned)(v & 0xFFFFFFFFull);
 }
 
 __global__ void k_fill_u64(unsigned long long* __restrict__ p, long long n, unsigned long long v) {
@@ -1371,9 +1717,27 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
     FPH_CHECK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
     const long long secs = wait_seconds > 0 ? (long long)wait_seconds : 120LL;
     const long long limit_ticks = secs * (long long)std::max(khz, 1) * 1000LL;
-    k_reduce_queue<<<(unsigned)nseg, kPhaseBWaves * FPH_WAVE, lds_b, s>>>(
-        a, order, res_count, seg_cols, ring, limit_ticks);
-    FPH_LAUNCH_CHECK();
+    // one segment: the queue may span several workgroups (autotuned)
+    const int qwgs = nseg == 1 ? fph_tune_value(FPH_TUNE_QUEUE_WGS, K, 0) : 1;
+    if (qwgs > 1) {
+        int* qs = (int*)(w + 5 * k4 + sort_sz + 64);  // in the 256-byte tail
+        k_fill_unwritten<<<1024, 256, 0, s>>>(pool, a.pool_ptr, pool_cap + 64);
+        k_queue_state_init<<<1, 64, 0, s>>>(qs, res_count, status);
+        FPH_LAUNCH_CHECK();
+        FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_reduce_queue_dev,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)lds_b));
+        k_reduce_queue_dev<<<(unsigned)qwgs, kPhaseBWaves * FPH_WAVE, lds_b, s>>>(
+            a, order, res_count, ring, qs, limit_ticks);
+        FPH_LAUNCH_CHECK();
+        k_final_reset<<<fph_blocks(K, 256), 256, 0, s>>>(a);
+        k_final_own<<<fph_blocks(std::max<int64_t>(nrows, 1), 256), 256, 0, s>>>(a);
+        FPH_LAUNCH_CHECK();
+    } else {
+        k_reduce_queue<<<(unsigned)nseg, kPhaseBWaves * FPH_WAVE, lds_b, s>>>(
+            a, order, res_count, seg_cols, ring, limit_ticks);
+        FPH_LAUNCH_CHECK();
+    }
     k_reduce_fini<<<1, 64, 0, s>>>(status, a.pool_ptr);
     FPH_LAUNCH_CHECK();
     return FPH_OK;

@@ -1,6 +1,6 @@
 // Autotune table: launch parameters chosen per problem-size bucket.
 //
-// Three knobs change how work is tiled, never what is computed (every value
+// Four knobs change how work is spread, never what is computed (every value
 // gives bit-identical outputs):
 //   FPH_TUNE_SCREEN_TILE     points per side of a distance tile of the direct
 //                            edge screen (32, 64 or 128), keyed by (n, d);
@@ -8,7 +8,10 @@
 //                            (triangle / tetrahedron count and emit, cofacet
 //                            columns), keyed by the number of items;
 //   FPH_TUNE_APPARENT_BLOCK  columns per workgroup of the apparent-pair and
-//                            clearing kernels, keyed by the number of columns.
+//                            clearing kernels, keyed by the number of columns;
+//   FPH_TUNE_QUEUE_WGS       workgroups of the residual reduction's queue for
+//                            one segment (1: one workgroup, ordered on chip;
+//                            more: the device-wide queue), keyed by columns.
 // A bucket is (floor(log2 size), floor(log2 d)); 0 in a slot means "default".
 // Slots are atomics, so lookups from concurrent host threads need no lock.
 // The reference tunes Triton block sizes by hand (edge_kernel.py:193-222,
@@ -25,6 +28,7 @@ std::atomic<int> g_table[FPH_TUNE_NKNOBS][kBuckets][kBuckets];
 
 const int kScreenTiles[] = {32, 64, 128};
 const int kBlocks[] = {64, 128, 256, 512, 1024};
+const int kQueueWgs[] = {1, 2, 4, 8, 16, 32, 64};
 
 int bucket_of(int64_t x) {
     if (x < 1) return 0;
@@ -36,11 +40,14 @@ int bucket_of(int64_t x) {
     return b;
 }
 
-int default_of(int knob) {
+int default_of(int knob, int64_t size) {
     switch (knob) {
         case FPH_TUNE_SCREEN_TILE: return 64;
         case FPH_TUNE_ENUM_BLOCK: return 256;
         case FPH_TUNE_APPARENT_BLOCK: return 256;
+        // the device-wide queue pays a few extra launches; from 8192 columns
+        // on its 16 workgroups win (n=10K, d=10: 2.4 -> 1.3 ms for H1)
+        case FPH_TUNE_QUEUE_WGS: return size >= 8192 ? 16 : 1;
         default: return 0;
     }
 }
@@ -54,6 +61,10 @@ int candidates_of(int knob, const int** list) {
         *list = kBlocks;
         return (int)(sizeof(kBlocks) / sizeof(int));
     }
+    if (knob == FPH_TUNE_QUEUE_WGS) {
+        *list = kQueueWgs;
+        return (int)(sizeof(kQueueWgs) / sizeof(int));
+    }
     return 0;
 }
 
@@ -62,7 +73,7 @@ int candidates_of(int knob, const int** list) {
 int fph_tune_value(int knob, int64_t size, int64_t d) {
     if (knob < 0 || knob >= FPH_TUNE_NKNOBS) return 0;
     const int v = g_table[knob][bucket_of(size)][bucket_of(d)].load(std::memory_order_relaxed);
-    return v ? v : default_of(knob);
+    return v ? v : default_of(knob, size);
 }
 
 extern "C" int fph_autotune_candidates(int knob, int32_t* out, int cap) {

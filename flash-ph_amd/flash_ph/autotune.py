@@ -1,13 +1,15 @@
 """Autotuner for libflashph's launch parameters.
 
-Three knobs set how work is tiled on the GPU and never what is computed:
+Four knobs set how work is spread over the GPU and never what is computed:
 
 * ``screen_tile``: points per side of a distance tile of the direct edge
   screen (32, 64, 128), per (n, d) bucket;
 * ``enum_block``: edges per workgroup of the triangle / tetrahedron /
   cofacet intersection kernels, per item-count bucket;
 * ``apparent_block``: columns per workgroup of the apparent-pair and
-  clearing kernels, per column-count bucket.
+  clearing kernels, per column-count bucket;
+* ``queue_wgs``: workgroups of the residual reduction's queue (1: one
+  workgroup; more: the device-wide queue), per column-count bucket.
 
 The library keeps one value per bucket (floor(log2 size), floor(log2 d)).
 ``tune`` times every candidate on the caller's own cloud with HIP events on
@@ -29,7 +31,7 @@ import torch
 
 from flash_ph import _lib
 
-KNOBS = {"screen_tile": 0, "enum_block": 1, "apparent_block": 2}
+KNOBS = {"screen_tile": 0, "enum_block": 1, "apparent_block": 2, "queue_wgs": 3}
 
 
 def _knob_id(knob) -> int:
@@ -155,9 +157,10 @@ def tune(points, max_edge_length: float, max_dim: int = 1, repeats: int = 5,
         "screen_tile": [(n, d)],
         "enum_block": [(E, 0)] + ([(T, 0)] if max_dim >= 2 else []),
         "apparent_block": [(E, 0)] + ([(K2, 0)] if max_dim >= 2 else []),
+        "queue_wgs": [(E, 0)] + ([(K2, 0)] if max_dim >= 2 else []),
     }
     stage = {"screen_tile": "edge_screen", "enum_block": "enumerate",
-             "apparent_block": "apparent"}
+             "apparent_block": "apparent", "queue_wgs": "reduce"}
     out, entries = {}, []
     for knob in knobs:
         _knob_id(knob)

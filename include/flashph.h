@@ -462,7 +462,8 @@ int fph_graph_destroy(void *graph);
  * the default.  Keys: SCREEN_TILE by (n, d) of fph_edge_screen;
  * ENUM_BLOCK by (E, 0) for the triangle / tetrahedron / cofacet kernels
  * (by (T, 0) for fph_cofacets_h2_count and fph_cofacets_h2); APPARENT_BLOCK by
- * (K, 0) for fph_apparent and fph_clear_pivots.  The reference picks its
+ * (K, 0) for fph_apparent and fph_clear_pivots; QUEUE_WGS by (K, 0) for
+ * fph_reduce with nseg == 1.  The reference picks its
  * Triton BLOCK sizes as constants (kernels/edge_kernel.py:193-222,
  * kernels/triangle_kernel.py:366-397, reduce_h1_gpu.py:159-223); this table
  * replaces them with values a tuner measures (flash_ph.autotune).
@@ -470,7 +471,8 @@ int fph_graph_destroy(void *graph);
 #define FPH_TUNE_SCREEN_TILE 0    /* points per tile side: 32, 64 (default), 128 */
 #define FPH_TUNE_ENUM_BLOCK 1     /* threads per workgroup: 64 .. 1024 (default 256) */
 #define FPH_TUNE_APPARENT_BLOCK 2 /* threads per workgroup: 64 .. 1024 (default 256) */
-#define FPH_TUNE_NKNOBS 3
+#define FPH_TUNE_QUEUE_WGS 3      /* fph_reduce queue workgroups, one segment: 1 .. 64 (default 16 from 8192 columns, else 1) */
+#define FPH_TUNE_NKNOBS 4
 
 /* Candidate values of `knob` into out[0..cap); returns their number. */
 int fph_autotune_candidates(int knob, int32_t *out, int cap);

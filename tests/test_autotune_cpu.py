@@ -21,6 +21,11 @@ def test_candidates_and_defaults():
     assert autotune.candidates("screen_tile") == [32, 64, 128]
     assert autotune.candidates("enum_block") == [64, 128, 256, 512, 1024]
     assert autotune.candidates("apparent_block") == [64, 128, 256, 512, 1024]
+    assert autotune.candidates("queue_wgs") == [1, 2, 4, 8, 16, 32, 64]
+    # the device-wide queue from 8192 columns on, one workgroup below
+    assert autotune.query("queue_wgs", 26_000) == 16
+    assert autotune.query("queue_wgs", 8192) == 16
+    assert autotune.query("queue_wgs", 8191) == 1
     assert autotune.query("screen_tile", 10_000, 10) == 64
     assert autotune.query("enum_block", 26_000) == 256
     assert autotune.query("apparent_block", 26_000) == 256
@@ -52,7 +57,8 @@ def test_invalid_values_rejected():
         autotune.query("tile", 10)
     lib = _lib.load()
     assert lib.fph_autotune_query(7, 10, 0) == _lib.FPH_ERR_ARG
-    assert lib.fph_autotune_set(3, 10, 0, 64) == _lib.FPH_ERR_ARG
+    assert lib.fph_autotune_set(4, 10, 0, 64) == _lib.FPH_ERR_ARG
+    assert lib.fph_autotune_set(3, 10, 0, 3) == _lib.FPH_ERR_ARG
     assert autotune.query("screen_tile", 1000, 10) == 64
 
 

@@ -76,12 +76,47 @@ def test_block_sizes_match_oracle(knob, value):
         _eq(name, dgms[k].cpu().numpy(), ref[name])
 
 
+QUEUE_CASES = [
+    ("randn_d10_n2000", lambda: cases.randn(2000, 10, 1), 2.2, 2, "reference"),
+    ("clifford_1000_noisy", lambda: cases.clifford(1000, 3, 0.02), 0.45, 2, "reference"),
+    ("o3_1024_exact", lambda: cases.o3(1024, 42), 1.8, 1, "exact"),
+    ("randn_d10_n10000_cfg2", lambda: cases.randn(10000, 10, 0), 1.6, 2, "reference"),
+]
+
+
+@pytest.mark.parametrize("wgs", [4, 16, 64])
+@pytest.mark.parametrize("name,make,t,max_dim,sem", QUEUE_CASES, ids=[c[0] for c in QUEUE_CASES])
+def test_queue_workgroups_match_oracle(wgs, name, make, t, max_dim, sem, monkeypatch):
+    """The residual reduction's queue spread over several workgroups (device-
+    coherent publication, pivots rebuilt from the owner table) gives the
+    oracle's diagrams."""
+    import oracle
+    from flash_ph import _pipeline, autotune
+    monkeypatch.setattr(_pipeline, "_WAIT_SECONDS", 10)
+    print(f"[gpu] queue_wgs={wgs} {name}", flush=True)
+    pts = make()
+    ref = oracle.run(pts, t, max_dim, exact=(sem == "exact"))
+    x = torch.from_numpy(pts).cuda()
+    _, cx0 = _pipeline.compute(x, t, max_dim, semantics=sem)
+    for size in (cx0.E, cx0.T):
+        autotune.set_value("queue_wgs", size, 0, wgs)
+    for rep in range(2):
+        dgms, cx = _pipeline.compute(x, t, max_dim, semantics=sem)
+        torch.cuda.synchronize()
+        for k in range(max_dim + 1):
+            name_k = ("h0", "h1", "h2")[k]
+            _eq(f"{name_k} rep {rep}", dgms[k].cpu().numpy(), ref[name_k])
+        if max_dim >= 2:
+            _eq("cleared", cx.cleared[:cx.T].cpu().numpy().astype(bool),
+                np.isin(np.arange(cx.T) + cx.E, ref["h1_allpiv"]))
+
+
 def test_tune_installs_and_caches(tmp_path):
     from flash_ph import autotune
     pts, t = cases.randn(3000, 10, 2), 1.9
     path = str(tmp_path / "tune.json")
     res = autotune.tune(pts, t, max_dim=2, repeats=2, persist=True, path=path)
-    assert set(res) == {"screen_tile", "enum_block", "apparent_block"}
+    assert set(res) == {"screen_tile", "enum_block", "apparent_block", "queue_wgs"}
     for knob, r in res.items():
         assert r["value"] in autotune.candidates(knob)
         assert set(r["ms"]) == set(autotune.candidates(knob))
