@@ -238,6 +238,11 @@ struct Alloc {
     // doubling when a longer result comes along.
     unsigned long long big_off[3];
     int big_cap[3];  // [2]: staging of long owner columns (device-wide queue)
+    // statistics kept per wavefront and added once at the end (device-wide
+    // queue: a shared counter updated per addition would serialise them)
+    unsigned long long merged;
+    int offchip;
+    int maxlen;
 };
 
 __device__ __forceinline__ Alloc make_alloc(unsigned long long chunk) {
@@ -246,6 +251,9 @@ __device__ __forceinline__ Alloc make_alloc(unsigned long long chunk) {
     al.chunk = chunk;
     al.big_off[0] = al.big_off[1] = al.big_off[2] = 0;
     al.big_cap[0] = al.big_cap[1] = al.big_cap[2] = 0;
+    al.merged = 0;
+    al.offchip = 0;
+    al.maxlen = 0;
     return al;
 }
 
@@ -335,6 +343,14 @@ __device__ __forceinline__ unsigned ld_coh(const unsigned* p) {
                              __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Workgroup-scope load (misses the CU's L1 but may hit the L2).  Under the
+// write-once rule any value other than kUnwritten it returns is final; a
+// kUnwritten one is re-read with ld_coh.
+__device__ __forceinline__ unsigned ld_fast(const unsigned* p) {
+    return __hip_atomic_load(const_cast<unsigned*>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 __device__ __forceinline__ void st_coh(unsigned* p, unsigned v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -352,14 +368,16 @@ __device__ bool copy_rows_dev(const unsigned* src, int n, unsigned* dst, long lo
     for (int base = 0; base < n; base += FPH_WAVE) {
         const int i = base + lane;
         bool miss = i < n;
+        bool first = true;
         while (true) {
             if (miss) {
-                const unsigned v = ld_coh(src + i);
+                const unsigned v = first ? ld_fast(src + i) : ld_coh(src + i);
                 if (v != kUnwritten) {
                     dst[i] = v;
                     miss = false;
                 }
             }
+            first = false;
             if (!__ballot(miss)) break;
             if (wall_clock64() - t0 > wait) return false;
             __builtin_amdgcn_s_sleep(1);
@@ -375,8 +393,10 @@ __device__ bool copy_rows_dev(const unsigned* src, int n, unsigned* dst, long lo
 // 64 words of slack past its capacity); those words are ignored.
 __device__ int spec_read_dev(const unsigned* src, unsigned& v, long long wait) {
     const long long t0 = wall_clock64();
+    bool first = true;
     while (true) {
-        v = ld_coh(src + fph_lane());
+        v = first ? ld_fast(src + fph_lane()) : ld_coh(src + fph_lane());
+        first = false;
         const unsigned len = __shfl(v, 0);
         if (len != kUnwritten) return (int)len;
         if (wall_clock64() - t0 > wait) return -1;
@@ -470,7 +490,8 @@ __device__ bool xor_step(Work& w, const unsigned* ob, unsigned* arena, const Red
         out_half = 1 - w.half;
         out = arena + out_half * H;
     } else {
-        if (fph_lane() == 0) atomicAdd(&a.status[4], 1);
+        if (kDev) al.offchip++;
+        else if (fph_lane() == 0) atomicAdd(&a.status[4], 1);
         const int bi = w.half == 2 ? 1 : 0;
         out = big_buffer(a, al, bi, na + nb);
         if (!out) return false;
@@ -486,7 +507,8 @@ __device__ bool xor_step(Work& w, const unsigned* ob, unsigned* arena, const Red
         out = arena + h * H;
         out_half = h;
     }
-    if (fph_lane() == 0) count_merged(a, (long long)na + nb + len);
+    if (kDev) al.merged += (unsigned long long)(na + nb + len);
+    else if (fph_lane() == 0) count_merged(a, (long long)na + nb + len);
     w.ptr = out;
     w.half = out_half;
     w.len = len;
@@ -829,7 +851,7 @@ __device__ int reduce_claim_dev(const RedArgs& a, int c, unsigned src, unsigned*
             old = __shfl(old, 0);
             const int oc = (int)(old >> 32) - 1;
             if (oc < c) {
-                note_len(a, w.len);
+                al.maxlen = max(al.maxlen, w.len);
                 if (oc >= 0 && fph_lane() == 0)
                     st_coh(&a.desc[oc], (unsigned)(old & 0xFFFFFFFFull));
                 return oc;
@@ -844,12 +866,223 @@ __device__ int reduce_claim_dev(const RedArgs& a, int c, unsigned src, unsigned*
     }
 }
 
+// ---- bitmap working column (device-wide queue, rows fit the arena) -------
+// When the rows [row_base, row_base + nrows) fit the wavefront's arena as a
+// bitmap, the working column lives there: an addition flips the owner's rows
+// (LDS atomic XOR, one row per lane) and the pivot moves to the next set bit
+// at or above the old one (pivots only increase).  The cost of an addition
+// is then the owner's length, not the working column's (long working
+// columns of a few hundred rows meet owners of a few rows).
+
+struct BmWork {
+    unsigned* bm;   // wavefront arena
+    int lo_w;       // first word that may hold a set bit
+    int hi_w;       // last word that may hold a set bit
+    int len;        // set bits
+    unsigned piv;   // lowest set row (global rank) when len > 0
+};
+
+// Owner entry read through the L2 (may be stale; see reduce_claim_dev_bm).
+__device__ __forceinline__ unsigned long long load_own_fast(const RedArgs& a, unsigned p) {
+    return __hip_atomic_load(&a.own[(long long)p - a.row_base], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Waits until every lane with `need` holds a written word in x (first read
+// already issued by the caller); false after `wait` ticks.
+__device__ __forceinline__ bool settle_word(const unsigned* p, bool need, unsigned& x,
+                                            long long t0, long long wait) {
+    bool miss = need && x == kUnwritten;
+    while (__ballot(miss)) {
+        if (wall_clock64() - t0 > wait) return false;
+        __builtin_amdgcn_s_sleep(1);
+        if (miss) {
+            x = ld_coh(p);
+            miss = x == kUnwritten;
+        }
+    }
+    return true;
+}
+
+// Flips the rows of the published column at src ([n, rows...]; its first 64
+// words already in v, lane l holding word l) into the bitmap.  Returns the
+// lowest row flipped (all lanes), or 0xFFFFFFFF after `wait` ticks.
+__device__ unsigned bm_flip_published(const RedArgs& a, BmWork& w, const unsigned* src,
+                                      unsigned v, int n, long long wait) {
+    const int lane = fph_lane();
+    const long long t0 = wall_clock64();
+    int delta = 0, hi = -1;
+    unsigned first = 0;
+    for (int base = 0; base <= n; base += FPH_WAVE) {
+        const int i = base + lane;            // word index in [len, rows...]
+        const bool need = i >= 1 && i <= n;
+        unsigned x = base == 0 ? v : (need ? ld_fast(src + i) : 0u);
+        if (!settle_word(src + i, need, x, t0, wait)) return 0xFFFFFFFFu;
+        if (base == 0) first = __shfl(x, 1);
+        if (need) {
+            const unsigned r = x - (unsigned)a.row_base;
+            const int wd = (int)(r >> 5);
+            const unsigned bit = 1u << (r & 31);
+            const unsigned old = atomicXor(&w.bm[wd], bit);
+            delta += (old & bit) ? -1 : 1;
+            hi = max(hi, wd);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        delta += __shfl_xor(delta, o);
+        hi = max(hi, __shfl_xor(hi, o));
+    }
+    w.len += delta;
+    w.hi_w = max(w.hi_w, hi);
+    wave_sync();
+    return first;
+}
+
+// Lowest set bit at or above the current pivot (len > 0).
+__device__ void bm_next_pivot(const RedArgs& a, BmWork& w) {
+    const int lane = fph_lane();
+    for (int wd = (int)((w.piv - (unsigned)a.row_base) >> 5); wd <= w.hi_w; wd += FPH_WAVE) {
+        const int i = wd + lane;
+        const unsigned x = i <= w.hi_w ? w.bm[i] : 0u;
+        const unsigned long long b = __ballot(x != 0u);
+        if (b) {
+            const int l = __ffsll((long long)b) - 1;
+            const unsigned xw = __shfl(x, l);
+            w.piv = (unsigned)a.row_base + 32u * (unsigned)(wd + l) + (unsigned)(__ffs(xw) - 1);
+            return;
+        }
+    }
+}
+
+// Publishes the set rows as [len, rows...] (write-once words); -1 on error.
+__device__ long long bm_publish(const RedArgs& a, const BmWork& w, Alloc& al) {
+    const int lane = fph_lane();
+    const long long at = pool_alloc(a, al, w.len + 1);
+    if (at < 0) return -1;
+    unsigned* dst = a.pool + at + 1;
+    int done = 0;
+    for (int wd = (int)((w.piv - (unsigned)a.row_base) >> 5); wd <= w.hi_w && done < w.len;
+         wd += FPH_WAVE) {
+        const int i = wd + lane;
+        unsigned x = i <= w.hi_w ? w.bm[i] : 0u;
+        const int cnt = __popc(x);
+        int incl = cnt;
+        for (int o = 1; o < FPH_WAVE; o <<= 1) {
+            const int y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        int pos = done + incl - cnt;
+        while (x && pos < w.len) {  // pos < len always; the bound keeps writes inside
+            const int b = __ffs(x) - 1;
+            st_coh(dst + pos, (unsigned)a.row_base + 32u * (unsigned)i + (unsigned)b);
+            pos++;
+            x &= x - 1u;
+        }
+        done += __shfl(incl, FPH_WAVE - 1);
+    }
+    if (lane == 0) st_coh(a.pool + at, (unsigned)w.len);
+    wave_sync();
+    return at;
+}
+
+__device__ void bm_clear(BmWork& w) {
+    for (int i = w.lo_w + fph_lane(); i <= w.hi_w; i += FPH_WAVE) w.bm[i] = 0u;
+    wave_sync();
+}
+
+// reduce_claim_dev with the bitmap working column (bm: the wavefront's
+// arena, all zero on entry and on return).  Owner entries are read through
+// the L2 and may be stale: a stale owner is still a column of higher rank
+// with the same pivot (a valid addition), a stale "no owner" leads to a
+// claim whose atomicMax returns the true owner, and every addition raises
+// the pivot, so the loop ends.
+__device__ int reduce_claim_dev_bm(const RedArgs& a, int c, unsigned src, unsigned* bm,
+                                   int* nxor, Alloc& al, long long wait) {
+    BmWork w;
+    w.bm = bm;
+    w.len = 0;
+    w.hi_w = -1;
+    unsigned v0;
+    const int n0 = spec_read_dev(a.pool + src, v0, wait);
+    if (n0 < 0) {
+        fail_wait(a);
+        return -2;
+    }
+    if (n0 == 0) return -1;
+    w.piv = bm_flip_published(a, w, a.pool + src, v0, n0, wait);
+    if (w.piv == 0xFFFFFFFFu) {
+        fail_wait(a);
+        return -2;
+    }
+    w.lo_w = (int)((w.piv - (unsigned)a.row_base) >> 5);
+    while (true) {
+        if (w.len == 0) {
+            bm_clear(w);
+            return -1;
+        }
+        const unsigned p = w.piv;
+        unsigned long long v = load_own_fast(a, p);
+        if ((int)(v >> 32) - 1 <= c) {
+            const long long at = bm_publish(a, w, al);
+            if (at < 0) return -2;
+            if (fph_lane() == 0) st_coh(&a.desc[c], (unsigned)at);
+            const unsigned long long mine =
+                ((unsigned long long)(c + 1) << 32) | (unsigned long long)at;
+            unsigned long long old = 0;
+            if (fph_lane() == 0) old = atomicMax(&a.own[(long long)p - a.row_base], mine);
+            old = __shfl(old, 0);
+            const int oc = (int)(old >> 32) - 1;
+            if (oc < c) {
+                al.maxlen = max(al.maxlen, w.len);
+                if (oc >= 0 && fph_lane() == 0)
+                    st_coh(&a.desc[oc], (unsigned)(old & 0xFFFFFFFFull));
+                bm_clear(w);
+                return oc;
+            }
+            v = old;
+        }
+        const unsigned* ob = a.pool + (v & 0xFFFFFFFFull);
+        unsigned vb;
+        const int nb = spec_read_dev(ob, vb, wait);
+        if (nb < 0) {
+            fail_wait(a);
+            return -2;
+        }
+        const int na = w.len;
+        if (bm_flip_published(a, w, ob, vb, nb, wait) == 0xFFFFFFFFu) {
+            fail_wait(a);
+            return -2;
+        }
+        if (w.len > 0) bm_next_pivot(a, w);
+        // the owner's pivot is p, so p is cleared and the pivot must rise
+        if (w.len > 0 && !(w.piv > p)) {
+            if (fph_lane() == 0) atomicExch(&a.status[0], 3);
+            return -2;
+        }
+        al.merged += (unsigned long long)(na + nb + w.len);
+        (*nxor)++;
+    }
+}
+
 // qs: [0] next residual index, [1] next ring slot to read, [2] ring tail,
-// [3] pending columns, [4] abort.
+// [3] pending columns, [4] abort, [5] XCD taking part (-1: none yet).
 __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_queue_dev(
     RedArgs a, const unsigned* __restrict__ order, const int* __restrict__ res_count,
-    unsigned long long* __restrict__ ring, int* __restrict__ qs, long long spin_limit) {
+    unsigned long long* __restrict__ ring, int* __restrict__ qs, long long spin_limit,
+    int bitmap) {
     extern __shared__ unsigned smem[];
+    __shared__ int s_part;
+    // Only the workgroups on one XCD take part (the XCD of the first one to
+    // start): they share that XCD's L2, where the columns they publish stay,
+    // so workgroup-scope reads (ld_fast) mostly hit.  Correctness does not
+    // depend on it (write-once words, coherent re-reads).
+    if (threadIdx.x == 0) {
+        const int xcc = (int)(__builtin_amdgcn_s_getreg((20) | (0 << 6) | (3 << 11)) & 0xF);
+        const int lead = atomicCAS(qs + 5, -1, xcc);
+        s_part = lead == -1 || lead == xcc;
+    }
+    __syncthreads();
+    if (!s_part) return;
     const int wid = threadIdx.x / FPH_WAVE;
     const int lane = fph_lane();
     unsigned* arena = smem + wid * kArenaB;
@@ -857,6 +1090,10 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_queue_dev(
     const unsigned cap = (unsigned)a.K;  // >= pending columns
     int nxor = 0;
     Alloc al = make_alloc(512);  // many wavefronts: small private chunks
+    if (bitmap) {  // the bitmap starts (and is left) all zero
+        for (int i = lane; i < kArenaB; i += FPH_WAVE) arena[i] = 0u;
+        wave_sync();
+    }
     while (true) {
         if (ld_coh(reinterpret_cast<const unsigned*>(qs + 4))) break;
         unsigned k = 0;
@@ -917,7 +1154,9 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_queue_dev(
         }
         int d = -1;
         if (a.pivot[c] >= 0) {  // zero after phase A otherwise
-            d = reduce_claim_dev(a, c, ld_coh(&a.desc[c]), arena, &nxor, al, spin_limit);
+            const unsigned src = ld_coh(&a.desc[c]);
+            d = bitmap ? reduce_claim_dev_bm(a, c, src, arena, &nxor, al, spin_limit)
+                       : reduce_claim_dev(a, c, src, arena, &nxor, al, spin_limit);
             if (d == -2) {
                 if (lane == 0) {
                     atomicExch(qs + 4, 1);
@@ -936,7 +1175,12 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_queue_dev(
             atomicSub(qs + 3, 1);
         }
     }
-    if (lane == 0 && nxor) atomicAdd(&a.status[2], nxor);
+    if (lane == 0) {
+        if (nxor) atomicAdd(&a.status[2], nxor);
+        if (al.merged) count_merged(a, (long long)al.merged);
+        if (al.offchip) atomicAdd(&a.status[4], al.offchip);
+        if (al.maxlen) atomicMax(&a.status[3], al.maxlen);
+    }
     if (threadIdx.x == 0) atomicMax(&a.status[1], 1);
 }
 
@@ -959,6 +1203,7 @@ __global__ void k_queue_state_init(int* __restrict__ qs, const int* __restrict__
         qs[2] = 0;
         qs[3] = *res_count;
         qs[4] = status[0] != 0;
+        qs[5] = -1;
     }
 }
 
@@ -1727,8 +1972,11 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
         FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_reduce_queue_dev,
                                           hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)lds_b));
-        k_reduce_queue_dev<<<(unsigned)qwgs, kPhaseBWaves * FPH_WAVE, lds_b, s>>>(
-            a, order, res_count, ring, qs, limit_ticks);
+        // 8 XCDs receive workgroups in turn: 8 x qwgs puts about qwgs on each
+        // bitmap working columns when the rows fit a wavefront's arena
+        const int bitmap = (nrows + 31) / 32 <= kArenaB ? 1 : 0;
+        k_reduce_queue_dev<<<(unsigned)(8 * qwgs), kPhaseBWaves * FPH_WAVE, lds_b, s>>>(
+            a, order, res_count, ring, qs, limit_ticks, bitmap);
         FPH_LAUNCH_CHECK();
         k_final_reset<<<fph_blocks(K, 256), 256, 0, s>>>(a);
         k_final_own<<<fph_blocks(std::max<int64_t>(nrows, 1), 256), 256, 0, s>>>(a);
