@@ -368,13 +368,13 @@ def main():
                    "rows": [int(d.shape[0]) for d in dg]},
         "roofline": {
             "bound": "hbm",
-            "kernel": "fph_reduce launch (k_reduce_queue dominant; latency-bound Z/2 "
+            "kernel": "fph_reduce launch (k_reduce_queue_dev dominant; latency-bound Z/2 "
                       "column additions)",
             "achieved": round(red_gbs, 3),
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": round(red_gbs / PEAK_HBM_GBS, 6),
-            "traffic": traffic_from_profile("k_reduce_queue"),
+            "traffic": traffic_from_profile("k_reduce_queue_dev"),
             "kernel_ms": round(reduce_ms, 4),
             "bytes_per_launch": int(4 * merged),
             "residual_columns_per_s": round(resid / (reduce_ms * 1e-3), 1),
