@@ -46,35 +46,48 @@ __global__ void k_diag_keys(long long K, const unsigned char* __restrict__ kind,
                             const unsigned char* __restrict__ seg_ess_asc, int nseg,
                             unsigned long long* __restrict__ keys, int* __restrict__ counts) {
     long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= K) return;
-    const unsigned char kd = kind[c];
-    const int be = col_edge ? col_edge[c] : (int)c;
-    int group = -1;
-    if (kd == 1) {
-        group = 2;
-    } else if (kd == 2 || kd == 3) {
-        const int p = pivot[c];
-        if (p < 0) {
+    int group = -1, s = 0;
+    if (c < K) {
+        const unsigned char kd = kind[c];
+        const int be = col_edge ? col_edge[c] : (int)c;
+        if (kd == 1) {
             group = 2;
+        } else if (kd == 2 || kd == 3) {
+            const int p = pivot[c];
+            if (p < 0) {
+                group = 2;
+            } else {
+                const float b = edge_filt[be];
+                const float dv = edge_filt[row_edge[(long long)p - row_base]];
+                if (dv > b) group = kd == 2 ? 0 : 1;
+            }
+        }
+        if (group < 0) {
+            keys[c] = ~0ull;
         } else {
-            const float b = edge_filt[be];
-            const float dv = edge_filt[row_edge[(long long)p - row_base]];
-            if (dv > b) group = kd == 2 ? 0 : 1;
+            s = segment_of(seg_cols, nseg, c);
+            const long long e0 = seg_edges ? seg_edges[s] : 0;
+            const long long e1 = seg_edges ? seg_edges[s + 1] : E;
+            const int asc = seg_ess_asc ? seg_ess_asc[s] : ess_asc_all;
+            const float bval = edge_filt[be];
+            const unsigned long long cls = (unsigned long long)(
+                e0 + fph_lower_bound(edge_filt + e0, (int)(e1 - e0), bval));
+            keys[c] = (cls << 33) | ((unsigned long long)group << 31) | order_of(group, c, asc);
         }
     }
-    if (group < 0) {
-        keys[c] = ~0ull;
-        return;
+    // rows per segment: one atomic per wavefront and segment (consecutive
+    // columns share their segment), not one per row on a shared counter
+    bool pend = group >= 0;
+    while (true) {
+        const unsigned long long m = __ballot(pend);
+        if (!m) break;
+        const int s0 = __shfl(s, __ffsll((long long)m) - 1);
+        const unsigned long long mine = __ballot(pend && s == s0);
+        if (pend && s == s0) {
+            if ((mine & fph_lanemask_lt()) == 0) atomicAdd(&counts[s0], __popcll(mine));
+            pend = false;
+        }
     }
-    const int s = segment_of(seg_cols, nseg, c);
-    const long long e0 = seg_edges ? seg_edges[s] : 0;
-    const long long e1 = seg_edges ? seg_edges[s + 1] : E;
-    const int asc = seg_ess_asc ? seg_ess_asc[s] : ess_asc_all;
-    const float bval = edge_filt[be];
-    const unsigned long long cls =
-        (unsigned long long)(e0 + fph_lower_bound(edge_filt + e0, (int)(e1 - e0), bval));
-    keys[c] = (cls << 33) | ((unsigned long long)group << 31) | order_of(group, c, asc);
-    atomicAdd(&counts[s], 1);
 }
 
 __global__ void k_diag_write(const unsigned long long* __restrict__ keys, long long K,

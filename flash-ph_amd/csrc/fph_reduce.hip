@@ -938,6 +938,28 @@ __device__ unsigned bm_flip_published(const RedArgs& a, BmWork& w, const unsigne
     return first;
 }
 
+// Flips n rows of an immutable list (the CSR columns) into the bitmap;
+// returns the lowest row.
+__device__ unsigned bm_flip_list(const RedArgs& a, BmWork& w, const unsigned* rows, int n) {
+    int delta = 0, hi = -1;
+    for (int i = fph_lane(); i < n; i += FPH_WAVE) {
+        const unsigned r = rows[i] - (unsigned)a.row_base;
+        const int wd = (int)(r >> 5);
+        const unsigned bit = 1u << (r & 31);
+        const unsigned old = atomicXor(&w.bm[wd], bit);
+        delta += (old & bit) ? -1 : 1;
+        hi = max(hi, wd);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        delta += __shfl_xor(delta, o);
+        hi = max(hi, __shfl_xor(hi, o));
+    }
+    w.len += delta;
+    w.hi_w = max(w.hi_w, hi);
+    wave_sync();
+    return rows[0];
+}
+
 // Lowest set bit at or above the current pivot (len > 0).
 __device__ void bm_next_pivot(const RedArgs& a, BmWork& w) {
     const int lane = fph_lane();
@@ -996,23 +1018,30 @@ __device__ void bm_clear(BmWork& w) {
 // with the same pivot (a valid addition), a stale "no owner" leads to a
 // claim whose atomicMax returns the true owner, and every addition raises
 // the pivot, so the loop ends.
-__device__ int reduce_claim_dev_bm(const RedArgs& a, int c, unsigned src, unsigned* bm,
+// csr: c starts from its cofacet column in the CSR (not yet published).
+__device__ int reduce_claim_dev_bm(const RedArgs& a, int c, unsigned src, bool csr, unsigned* bm,
                                    int* nxor, Alloc& al, long long wait) {
     BmWork w;
     w.bm = bm;
     w.len = 0;
     w.hi_w = -1;
-    unsigned v0;
-    const int n0 = spec_read_dev(a.pool + src, v0, wait);
-    if (n0 < 0) {
-        fail_wait(a);
-        return -2;
-    }
-    if (n0 == 0) return -1;
-    w.piv = bm_flip_published(a, w, a.pool + src, v0, n0, wait);
-    if (w.piv == 0xFFFFFFFFu) {
-        fail_wait(a);
-        return -2;
+    if (csr) {
+        const int lo = a.off[c], n0 = a.off[c + 1] - lo;
+        if (n0 == 0) return -1;
+        w.piv = bm_flip_list(a, w, a.data + lo, n0);
+    } else {
+        unsigned v0;
+        const int n0 = spec_read_dev(a.pool + src, v0, wait);
+        if (n0 < 0) {
+            fail_wait(a);
+            return -2;
+        }
+        if (n0 == 0) return -1;
+        w.piv = bm_flip_published(a, w, a.pool + src, v0, n0, wait);
+        if (w.piv == 0xFFFFFFFFu) {
+            fail_wait(a);
+            return -2;
+        }
     }
     w.lo_w = (int)((w.piv - (unsigned)a.row_base) >> 5);
     while (true) {
@@ -1069,7 +1098,7 @@ __device__ int reduce_claim_dev_bm(const RedArgs& a, int c, unsigned src, unsign
 __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_queue_dev(
     RedArgs a, const unsigned* __restrict__ order, const int* __restrict__ res_count,
     unsigned long long* __restrict__ ring, int* __restrict__ qs, long long spin_limit,
-    int bitmap) {
+    int bitmap, int from_csr) {
     extern __shared__ unsigned smem[];
     __shared__ int s_part;
     // Only the workgroups on one XCD take part (the XCD of the first one to
@@ -1153,9 +1182,13 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_queue_dev(
             if (c < 0) break;
         }
         int d = -1;
-        if (a.pivot[c] >= 0) {  // zero after phase A otherwise
+        // from_csr: no phase A ran; desc[c] reads kUnwritten until c first
+        // publishes, and then c starts from its CSR column (never zero).
+        // Otherwise pivot < 0 marks columns phase A reduced to zero.
+        if (from_csr || a.pivot[c] >= 0) {
             const unsigned src = ld_coh(&a.desc[c]);
-            d = bitmap ? reduce_claim_dev_bm(a, c, src, arena, &nxor, al, spin_limit)
+            const bool csr = from_csr && src == kUnwritten;
+            d = bitmap ? reduce_claim_dev_bm(a, c, src, csr, arena, &nxor, al, spin_limit)
                        : reduce_claim_dev(a, c, src, arena, &nxor, al, spin_limit);
             if (d == -2) {
                 if (lane == 0) {
@@ -1222,6 +1255,11 @@ __global__ void k_final_own(RedArgs a) {
     const long long c = (long long)(v >> 32) - 1;
     a.pivot[c] = (int)(r + a.row_base);
     a.desc[c] = (unsigned)(v & 0xFFFFFFFFull);
+}
+
+__global__ void k_fill_u32(unsigned* __restrict__ p, long long n, unsigned v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 __global__ void k_fill_u64(unsigned long long* __restrict__ p, long long n, unsigned long long v) {
@@ -1942,8 +1980,15 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
     const unsigned grid_w =
         (unsigned)std::min<int64_t>(1024, (K + kPhaseAWaves - 1) / kPhaseAWaves);
     k_publish_apparent<<<grid_w, kPhaseAWaves * FPH_WAVE, 0, s>>>(a, K, 1);
+    // one segment: the queue may span several workgroups (autotuned); with
+    // bitmap working columns it also takes phase A's additions (cheaper
+    // there than as list merges) and starts columns from the CSR
+    const int qwgs = nseg == 1 ? fph_tune_value(FPH_TUNE_QUEUE_WGS, K, 0) : 1;
+    const int bitmap = (nrows + 31) / 32 <= kArenaB ? 1 : 0;
+    const int from_csr = qwgs > 1 && bitmap;
     const size_t lds_a = kPhaseAWaves * kArenaA * 4;
-    k_reduce_chain<<<grid_w, kPhaseAWaves * FPH_WAVE, lds_a, s>>>(a, res_list, res_count);
+    if (!from_csr)
+        k_reduce_chain<<<grid_w, kPhaseAWaves * FPH_WAVE, lds_a, s>>>(a, res_list, res_count);
     k_order_keys<<<fph_blocks(K, 256), 256, 0, s>>>(res_list, res_count, K, keys);
     FPH_LAUNCH_CHECK();
     FPH_CHECK_HIP(rocprim::radix_sort_keys_desc(sort_tmp, sort_sz, keys, order, (unsigned)K, 0,
@@ -1962,11 +2007,13 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
     FPH_CHECK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
     const long long secs = wait_seconds > 0 ? (long long)wait_seconds : 120LL;
     const long long limit_ticks = secs * (long long)std::max(khz, 1) * 1000LL;
-    // one segment: the queue may span several workgroups (autotuned)
-    const int qwgs = nseg == 1 ? fph_tune_value(FPH_TUNE_QUEUE_WGS, K, 0) : 1;
     if (qwgs > 1) {
         int* qs = (int*)(w + 5 * k4 + sort_sz + 64);  // in the 256-byte tail
         k_fill_unwritten<<<1024, 256, 0, s>>>(pool, a.pool_ptr, pool_cap + 64);
+        if (from_csr) {  // no published state yet: desc[] reads kUnwritten
+            k_fill_u32<<<fph_blocks(K, 256), 256, 0, s>>>(a.desc, K, kUnwritten);
+            FPH_LAUNCH_CHECK();
+        }
         k_queue_state_init<<<1, 64, 0, s>>>(qs, res_count, status);
         FPH_LAUNCH_CHECK();
         FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_reduce_queue_dev,
@@ -1974,9 +2021,8 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
                                           (int)lds_b));
         // 8 XCDs receive workgroups in turn: 8 x qwgs puts about qwgs on each
         // bitmap working columns when the rows fit a wavefront's arena
-        const int bitmap = (nrows + 31) / 32 <= kArenaB ? 1 : 0;
         k_reduce_queue_dev<<<(unsigned)(8 * qwgs), kPhaseBWaves * FPH_WAVE, lds_b, s>>>(
-            a, order, res_count, ring, qs, limit_ticks, bitmap);
+            a, order, res_count, ring, qs, limit_ticks, bitmap, from_csr);
         FPH_LAUNCH_CHECK();
         k_final_reset<<<fph_blocks(K, 256), 256, 0, s>>>(a);
         k_final_own<<<fph_blocks(std::max<int64_t>(nrows, 1), 256), 256, 0, s>>>(a);

@@ -160,25 +160,34 @@ __global__ void k_apparent(long long K, long long col_base, const int* __restric
                            int* __restrict__ pivot, int* __restrict__ res_list,
                            int* __restrict__ res_count) {
     long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= K) return;
     int p = -1;
-    unsigned char kd;
-    if (skip && skip[c]) {
-        kd = 0;
-    } else if (off[c + 1] == off[c]) {
-        kd = 1;
-    } else {
-        const unsigned y = data[off[c]];
-        if ((long long)row_cmf[y - row_base] == col_base + c) {
-            kd = 2;
-            p = (int)y;
+    unsigned char kd = 0;
+    if (c < K) {
+        if (skip && skip[c]) {
+            kd = 0;
+        } else if (off[c + 1] == off[c]) {
+            kd = 1;
         } else {
-            kd = 3;
-            res_list[atomicAdd(res_count, 1)] = (int)c;
+            const unsigned y = data[off[c]];
+            if ((long long)row_cmf[y - row_base] == col_base + c) {
+                kd = 2;
+                p = (int)y;
+            } else {
+                kd = 3;
+            }
         }
+        kind[c] = kd;
+        pivot[c] = p;
     }
-    kind[c] = kd;
-    pivot[c] = p;
+    // residual list: one atomic per wavefront (the order is sorted later)
+    const bool res = c < K && kd == 3;
+    const unsigned long long m = __ballot(res);
+    if (!m) return;
+    const int first = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (fph_lane() == first) base = atomicAdd(res_count, __popcll(m));
+    base = __shfl(base, first);
+    if (res) res_list[base + __popcll(m & fph_lanemask_lt())] = (int)c;
 }
 
 // H1 -> H2 clearing: a triangle that is the pivot of an H1 column (apparent
@@ -198,21 +207,44 @@ __global__ void k_clear_pivots(long long K, const unsigned char* __restrict__ ki
 // colex4 ranking of apparent_pair_h2_kernel.py:52-58).
 // ---------------------------------------------------------------------------
 
-__global__ void k_tet_count(long long E, const int* __restrict__ aptr, const int* __restrict__ aidx,
-                            const int* __restrict__ arank, const int* __restrict__ tri_off,
-                            const int* __restrict__ tri_w, int* __restrict__ cnt) {
-    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// Tetrahedra per edge e: pairs a < b of e's own triangles (third vertices
+// x = tri_w[lo+a] < y = tri_w[lo+b]) whose edge (x, y) is older than e.  One
+// wavefront per edge, one pair per lane, pairs in colex order k = b(b-1)/2+a
+// (= the (y, x) order of the output): the pair count of an edge is quadratic
+// in its triangles, so a thread per edge left the few heavy edges as a long
+// serial tail.
+__device__ __forceinline__ void pair_of(long long k, int* a, int* b) {
+    long long bb = (long long)((1.0 + sqrt(8.0 * (double)k + 1.0)) * 0.5);
+    while (bb * (bb - 1) / 2 > k) bb--;
+    while ((bb + 1) * bb / 2 <= k) bb++;
+    *b = (int)bb;
+    *a = (int)(k - bb * (bb - 1) / 2);
+}
+
+constexpr int kTetWaves = 4;  // wavefronts (edges) per workgroup
+
+__global__ __launch_bounds__(kTetWaves * FPH_WAVE) void k_tet_count(
+    long long E, const int* __restrict__ aptr, const int* __restrict__ aidx,
+    const int* __restrict__ arank, const int* __restrict__ tri_off,
+    const int* __restrict__ tri_w, int* __restrict__ cnt) {
+    const long long e = (long long)blockIdx.x * kTetWaves + threadIdx.x / FPH_WAVE;
     if (e >= E) return;
+    const int lane = fph_lane();
     const int lo = tri_off[e], len = tri_off[e + 1] - lo;
+    const long long np = (long long)len * (len - 1) / 2;
     int c = 0;
-    for (int b = 1; b < len; b++) {
-        const int y = tri_w[lo + b];
-        for (int a = 0; a < b; a++) {
-            const int r = fph_edge_rank(aptr, aidx, arank, y, tri_w[lo + a]);
-            if (r >= 0 && r < e) c++;
+    for (long long base = 0; base < np; base += FPH_WAVE) {
+        const long long k = base + lane;
+        bool ok = false;
+        if (k < np) {
+            int a, b;
+            pair_of(k, &a, &b);
+            const int r = fph_edge_rank(aptr, aidx, arank, tri_w[lo + b], tri_w[lo + a]);
+            ok = r >= 0 && r < e;
         }
+        c += __popcll(__ballot(ok));
     }
-    cnt[e] = c;
+    if (lane == 0) cnt[e] = c;
 }
 
 __device__ __forceinline__ int tri_rank_local(const int* tri_off, const int* tri_w, int m,
@@ -230,24 +262,35 @@ __device__ __forceinline__ int tri_face_local(const int* tri_off, const int* tri
     return tri_rank_local(tri_off, tri_w, rxy, s);
 }
 
-__global__ void k_tet_emit(long long E, const int* __restrict__ ei, const int* __restrict__ ej,
-                           const int* __restrict__ aptr, const int* __restrict__ aidx,
-                           const int* __restrict__ arank, const int* __restrict__ tri_off,
-                           const int* __restrict__ tri_w, const int* __restrict__ tri_ru,
-                           const int* __restrict__ tri_rv, const unsigned char* __restrict__ cleared,
-                           const int* __restrict__ tet_off, unsigned* __restrict__ tet_key,
-                           int* __restrict__ tet_e, int* __restrict__ tet_cmf) {
-    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kTetWaves * FPH_WAVE) void k_tet_emit(
+    long long E, const int* __restrict__ ei, const int* __restrict__ ej,
+    const int* __restrict__ aptr, const int* __restrict__ aidx, const int* __restrict__ arank,
+    const int* __restrict__ tri_off, const int* __restrict__ tri_w,
+    const int* __restrict__ tri_ru, const int* __restrict__ tri_rv,
+    const unsigned char* __restrict__ cleared, const int* __restrict__ tet_off,
+    unsigned* __restrict__ tet_key, int* __restrict__ tet_e, int* __restrict__ tet_cmf) {
+    const long long e = (long long)blockIdx.x * kTetWaves + threadIdx.x / FPH_WAVE;
     if (e >= E) return;
-    const int u = ei[e], v = ej[e];
+    const int lane = fph_lane();
     const int lo = tri_off[e], len = tri_off[e + 1] - lo;
+    const long long np = (long long)len * (len - 1) / 2;
+    if (np == 0) return;
+    const int u = ei[e], v = ej[e];
     int q = tet_off[e];
-    for (int b = 1; b < len; b++) {
-        const int y = tri_w[lo + b];
-        for (int a = 0; a < b; a++) {
-            const int x = tri_w[lo + a];
-            const int rxy = fph_edge_rank(aptr, aidx, arank, y, x);
-            if (rxy < 0 || rxy >= e) continue;
+    const unsigned long long lt = fph_lanemask_lt();
+    for (long long base = 0; base < np; base += FPH_WAVE) {
+        const long long k = base + lane;
+        bool ok = false;
+        int a = 0, b = 0, x = 0, y = 0, rxy = -1;
+        if (k < np) {
+            pair_of(k, &a, &b);
+            x = tri_w[lo + a];
+            y = tri_w[lo + b];
+            rxy = fph_edge_rank(aptr, aidx, arank, y, x);
+            ok = rxy >= 0 && rxy < e;
+        }
+        const unsigned long long m = __ballot(ok);
+        if (ok) {
             // faces as local triangle ranks, largest first
             const int f_uvy = lo + b, f_uvx = lo + a;
             const int f_uxy = tri_face_local(tri_off, tri_w, u, x, y, tri_ru[lo + a],
@@ -261,11 +304,12 @@ __global__ void k_tet_emit(long long E, const int* __restrict__ ei, const int* _
                 if (!cleared[f_uxy]) cmf = f_uxy;
                 if (!cleared[f_vxy] && f_vxy > cmf) cmf = f_vxy;
             }
-            tet_key[q] = ((unsigned)y << 16) | (unsigned)x;
-            tet_e[q] = (int)e;
-            tet_cmf[q] = cmf < 0 ? -1 : (int)(E + cmf);
-            q++;
+            const int at = q + __popcll(m & lt);
+            tet_key[at] = ((unsigned)y << 16) | (unsigned)x;
+            tet_e[at] = (int)e;
+            tet_cmf[at] = cmf < 0 ? -1 : (int)(E + cmf);
         }
+        q += __popcll(m);
     }
 }
 
@@ -515,9 +559,8 @@ extern "C" int fph_tet_count(int64_t E, const int32_t* adj_ptr, const int32_t* a
     if (*ws_bytes < need) return FPH_ERR_ARG;
     int* cnt = (int*)((char*)ws + tmp);
     if (E > 0) {
-        const int tb = fph_tune_value(FPH_TUNE_ENUM_BLOCK, E, 0);
-        k_tet_count<<<fph_blocks(E, tb), tb, 0, s>>>(E, adj_ptr, adj_idx, adj_rank, tri_off,
-                                                      tri_w, cnt);
+        k_tet_count<<<fph_blocks(E, kTetWaves), kTetWaves * FPH_WAVE, 0, s>>>(
+            E, adj_ptr, adj_idx, adj_rank, tri_off, tri_w, cnt);
         FPH_LAUNCH_CHECK();
     }
     return scan_counts(cnt, tet_off, E, (long long*)total, ws, tmp, s);
@@ -532,10 +575,9 @@ extern "C" int fph_tet_emit(int64_t E, const int32_t* edge_i, const int32_t* edg
     hipStream_t s = (hipStream_t)stream;
     if (E < 0) return FPH_ERR_ARG;
     if (E == 0) return FPH_OK;
-    const int tb = fph_tune_value(FPH_TUNE_ENUM_BLOCK, E, 0);
-    k_tet_emit<<<fph_blocks(E, tb), tb, 0, s>>>(E, edge_i, edge_j, adj_ptr, adj_idx, adj_rank,
-                                                 tri_off, tri_w, tri_ru, tri_rv, cleared, tet_off,
-                                                 tet_key, tet_e, tet_cmf);
+    k_tet_emit<<<fph_blocks(E, kTetWaves), kTetWaves * FPH_WAVE, 0, s>>>(
+        E, edge_i, edge_j, adj_ptr, adj_idx, adj_rank, tri_off, tri_w, tri_ru, tri_rv, cleared,
+        tet_off, tet_key, tet_e, tet_cmf);
     FPH_LAUNCH_CHECK();
     return FPH_OK;
 }
