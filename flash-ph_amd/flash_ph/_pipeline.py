@@ -241,12 +241,15 @@ def build_mst(cx: Complex, want_labels: bool = False) -> None:
     cx.info["n_mst"] = nm
 
 
-def build_h1(cx: Complex, max_triangles: int = MAX_TRIANGLES, semantics: str = "exact") -> None:
+def build_h1(cx: Complex, max_triangles: int = MAX_TRIANGLES, semantics: str = "exact",
+             defer: bool = False) -> None:
     """Stages (d), (c), (e) for H1 and the H1 diagram.
 
     semantics: "exact" (parallel reduction; the pairs of any complete
     reduction) or "reference" (the reference's capacity-limited serial
-    reduction, identical to its output even where its store overflows)."""
+    reduction, identical to its output even where its store overflows).
+    defer (one cloud): leave the reduction's status unread; build_h2 reads it
+    with its own sizes (one host synchronisation instead of two)."""
     lib = load()
     dev, n, E = cx.device, cx.n, cx.E
     s = stream_of(dev)
@@ -260,23 +263,25 @@ def build_h1(cx: Complex, max_triangles: int = MAX_TRIANGLES, semantics: str = "
             (s,), dev)
     TIMER.stop(tok)
     segs = None
+    # one read for the sizes and the forest's convergence flag
+    conv = cx.info["converged"].long()
     if cx.edge_off is None:
-        T, C = (int(x) for x in totals.tolist())
+        T, C, converged = (int(x) for x in torch.cat([totals, conv]).tolist())
         tmax = T
     else:
         # batch: triangles of cloud c = those whose largest edge lies in its
         # edge range; the limit applies per cloud, as for one cloud.
         eo = cx.edge_off.long()
         tc = cx.tri_off[eo[1:]] - cx.tri_off[eo[:-1]]
-        host = torch.cat([totals, tc.long(), eo]).tolist()
-        T, C = host[0], host[1]
+        host = torch.cat([totals, tc.long(), eo, conv]).tolist()
+        T, C, converged = host[0], host[1], host[-1]
         nc = len(cx.vbase_host) - 1
         tcs = host[2:2 + nc]
         tmax = max(tcs) if tcs else 0
         segs = {"cols": cx.edge_off, "edges": cx.edge_off,
-                "ess_asc": (tc == 0).to(torch.uint8), "cols_host": host[2 + nc:]}
+                "ess_asc": (tc == 0).to(torch.uint8), "cols_host": host[2 + nc:-1]}
         ecs = [b - a for a, b in zip(segs["cols_host"][:-1], segs["cols_host"][1:])]
-    if not int(cx.info["converged"].item()):
+    if not converged:
         raise _lib.FlashPHError("Boruvka did not converge")
     if tmax > max_triangles:
         raise ValueError(
@@ -314,15 +319,24 @@ def build_h1(cx: Complex, max_triangles: int = MAX_TRIANGLES, semantics: str = "
         ref_sizes = shapes if serial_exact else None
     else:
         raise ValueError(f"unknown reduction semantics {semantics!r}")
-    cx.kind1, cx.pivot1, h1, stats = _reduce_and_diagram(
+    res = _reduce_and_diagram(
         K=E, col_base=0, off=cx.cof_off, data=cx.cof_data, skip=cx.mst_mask, row_cmf=cx.tri_cmf,
         row_base=E, nrows=T, C=C, col_edge=None, row_edge=cx.tri_e, edge_filt=cx.edge_filt,
         E=E, ess_ascending=(T == 0), device=dev, segs=segs, ref_sizes=ref_sizes,
-        serial_exact=serial_exact, want_edges=cx.want_edges)
-    cx.h1 = h1 if segs is not None else h1[0]
+        serial_exact=serial_exact, want_edges=cx.want_edges, defer=defer and segs is None)
+    if defer and segs is None:
+        cx.kind1, cx.pivot1, info32, fin = res
+        cx.info["h1_pending"] = (info32, fin)
+        return
+    _set_h1(cx, res, segs is not None)
+
+
+def _set_h1(cx: Complex, res, batch: bool) -> None:
+    cx.kind1, cx.pivot1, h1, stats = res
+    cx.h1 = h1 if batch else h1[0]
     if cx.want_edges:
         re = stats.pop("row_edges")
-        cx.h1_edges = re if segs is not None else re[0]
+        cx.h1_edges = re if batch else re[0]
     cx.info["h1_stats"] = stats
 
 
@@ -338,7 +352,7 @@ def _round32(x: int) -> int:
 
 def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, C, col_edge,
                         row_edge, edge_filt, E, ess_ascending, device, segs=None,
-                        ref_sizes=None, serial_exact=False, want_edges=False):
+                        ref_sizes=None, serial_exact=False, want_edges=False, defer=False):
     """Apparent pairs, residual reduction and the diagram of one dimension.
 
     ``segs`` (batches) = dict(cols=, edges=, ess_asc= device tensors,
@@ -398,7 +412,10 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
         variant = SERIAL_VARIANT | (2 if serial_exact else 0)
     out = _empty(torch.float32, device, max(K, 1), 2)
     out_edges = _empty(torch.int32, device, max(K, 1), 2) if want_edges else None
-    while True:
+    state = {"pool_words": pool_words, "variant": variant if ref_sizes is not None else 0}
+
+    def launch():
+        pool_words = state["pool_words"]
         tok = TIMER.start("apparent", device)
         check(lib.fph_apparent(K, col_base, ptr(off), ptr(data), ptr(skip), ptr(row_cmf),
                                row_base, nrows, ptr(kind), ptr(own), ptr(pivot), ptr(res_list),
@@ -421,35 +438,46 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
                                         nseg, ptr(seg_pool_t), pool_start, ptr(kind), ptr(own),
                                         ptr(pivot), ptr(res_list), rescnt_p, ptr(pool),
                                         pool_words, ptr(scratch), scratch_bytes, status_p,
-                                        variant, s), "fph_reduce_serial")
+                                        state["variant"], s), "fph_reduce_serial")
         TIMER.stop(tok)
         call_ws("fph_diagram",
                 (K, ptr(kind), ptr(off), ptr(data), ptr(pivot), ptr(col_edge), row_base,
                  ptr(row_edge), ptr(edge_filt), E, int(bool(ess_ascending)), ptr(seg_cols),
                  ptr(seg_edges), ptr(seg_ess_asc), nseg, ptr(out), ptr(out_edges), counts_p),
                 (s,), device)
-        st = info32.tolist()
-        if REDUCE_LOG is not None:
-            REDUCE_LOG.append((pool_words, st[:9]))
-        if st[0] == 0:
-            break
-        if st[0] == 1:
-            pool_words = max(2 * pool_words, 32 * st[5] + (32 * st[5]) // 4)
-            continue
-        if st[0] == 4 and ref_sizes is not None and (variant & 1) == 0:
-            variant |= 1  # a column would pass the reference's MAX_COL: list form
-            continue
-        raise _lib.FlashPHError(f"fph_reduce failed with status {st[0]}")
-    merged = (st[6] & 0xFFFFFFFF) | ((st[7] & 0xFFFFFFFF) << 32)
-    stats = {"xors": st[2], "longest": st[3], "offchip_xors": st[4],
-             "pool_words": 32 * st[5], "pool_cap": pool_words, "residual": st[8],
-             "merged": merged}
-    counts = st[12:12 + nseg]
-    starts = [0] if segs is None else segs["cols_host"][:-1]
-    dgms = [out[a:a + c] for a, c in zip(starts, counts)]
-    if want_edges:
-        stats["row_edges"] = [out_edges[a:a + c] for a, c in zip(starts, counts)]
-    return kind, pivot, dgms, stats
+
+    def finish(st):
+        """Retries on a recoverable status, then the diagrams and stats."""
+        while True:
+            if REDUCE_LOG is not None:
+                REDUCE_LOG.append((state["pool_words"], st[:9]))
+            if st[0] == 0:
+                break
+            if st[0] == 1:
+                state["pool_words"] = max(2 * state["pool_words"], 32 * st[5] + (32 * st[5]) // 4)
+            elif st[0] == 4 and ref_sizes is not None and (state["variant"] & 1) == 0:
+                state["variant"] |= 1  # a column would pass the reference's MAX_COL: list form
+            else:
+                raise _lib.FlashPHError(f"fph_reduce failed with status {st[0]}")
+            launch()
+            st = info32.tolist()
+        merged = (st[6] & 0xFFFFFFFF) | ((st[7] & 0xFFFFFFFF) << 32)
+        stats = {"xors": st[2], "longest": st[3], "offchip_xors": st[4],
+                 "pool_words": 32 * st[5], "pool_cap": state["pool_words"], "residual": st[8],
+                 "merged": merged}
+        counts = st[12:12 + nseg]
+        starts = [0] if segs is None else segs["cols_host"][:-1]
+        dgms = [out[a:a + c] for a, c in zip(starts, counts)]
+        if want_edges:
+            stats["row_edges"] = [out_edges[a:a + c] for a, c in zip(starts, counts)]
+        return kind, pivot, dgms, stats
+
+    launch()
+    if defer:
+        # the caller reads info32 together with its next sizes and calls
+        # finish(that list); kind/pivot are final unless finish retries
+        return kind, pivot, info32, finish
+    return finish(info32.tolist())
 
 
 def build_h2(cx: Complex) -> None:
@@ -458,22 +486,39 @@ def build_h2(cx: Complex) -> None:
     dev, E, T = cx.device, cx.E, cx.T
     s = stream_of(dev)
     cx.cleared = _empty(torch.uint8, dev, max(T, 1))
-    tok = TIMER.start("apparent", dev)
-    check(lib.fph_clear_pivots(E, ptr(cx.kind1), ptr(cx.pivot1), E, T, ptr(cx.cleared), s),
-          "fph_clear_pivots")
-    TIMER.stop(tok)
     cx.tet_off = _empty(torch.int32, dev, E + 1)
     cx.cof2_off = _empty(torch.int32, dev, T + 1)
     totals = _empty(torch.int64, dev, 2)
-    tok = TIMER.start("enumerate", dev)
-    call_ws("fph_tet_count",
-            (E, ptr(cx.adj_ptr), ptr(cx.adj_idx), ptr(cx.adj_rank), ptr(cx.tri_off),
-             ptr(cx.tri_w), ptr(cx.tet_off), totals.data_ptr()), (s,), dev)
-    call_ws("fph_cofacets_h2_count",
-            (T, ptr(cx.edge_i), ptr(cx.edge_j), ptr(cx.adj_ptr), ptr(cx.adj_idx), ptr(cx.tri_w),
-             ptr(cx.tri_e), ptr(cx.cleared), ptr(cx.cof2_off), totals.data_ptr() + 8), (s,), dev)
-    TIMER.stop(tok)
-    Q, C2 = (int(x) for x in totals.tolist())
+
+    def clear_and_count():
+        tok = TIMER.start("apparent", dev)
+        check(lib.fph_clear_pivots(E, ptr(cx.kind1), ptr(cx.pivot1), E, T, ptr(cx.cleared), s),
+              "fph_clear_pivots")
+        TIMER.stop(tok)
+        tok = TIMER.start("enumerate", dev)
+        call_ws("fph_tet_count",
+                (E, ptr(cx.adj_ptr), ptr(cx.adj_idx), ptr(cx.adj_rank), ptr(cx.tri_off),
+                 ptr(cx.tri_w), ptr(cx.tet_off), totals.data_ptr()), (s,), dev)
+        call_ws("fph_cofacets_h2_count",
+                (T, ptr(cx.edge_i), ptr(cx.edge_j), ptr(cx.adj_ptr), ptr(cx.adj_idx),
+                 ptr(cx.tri_w), ptr(cx.tri_e), ptr(cx.cleared), ptr(cx.cof2_off),
+                 totals.data_ptr() + 8), (s,), dev)
+        TIMER.stop(tok)
+
+    clear_and_count()
+    pending = cx.info.pop("h1_pending", None)
+    if pending is None:
+        Q, C2 = (int(x) for x in totals.tolist())
+    else:
+        # the H1 status read together with these sizes; a retried H1 reduction
+        # (rare: its store was too small) changes the clearing, so recount
+        info32, fin = pending
+        host = torch.cat([totals, info32.long()]).tolist()
+        Q, C2, st1 = host[0], host[1], host[2:]
+        _set_h1(cx, fin(st1), False)
+        if st1[0] != 0:
+            clear_and_count()
+            Q, C2 = (int(x) for x in totals.tolist())
     if E + T + Q >= 2**31 - 1:
         raise ValueError("simplex count exceeds the int32 rank range")
     cx.Q, cx.C2 = Q, C2
@@ -510,12 +555,13 @@ def compute(pts: torch.Tensor, max_edge_length: float, max_dim: int,
     cx.want_edges = want_edges
     build_mst(cx)
     out = [cx.h0]
-    if max_dim >= 1:
-        build_h1(cx, semantics=semantics if max_dim == 1 else "exact")
+    if max_dim == 1:
+        build_h1(cx, semantics=semantics)
         out.append(cx.h1)
     if max_dim >= 2:
+        build_h1(cx, semantics="exact", defer=True)
         build_h2(cx)
-        out.append(cx.h2)
+        out += [cx.h1, cx.h2]
     return out, cx
 
 

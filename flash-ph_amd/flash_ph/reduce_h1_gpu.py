@@ -44,7 +44,7 @@ def rips_h1_gpu_native(filt, n, mst_idx, max_edge_length, device, backend='auto'
     if mst_idx.numel() > 0:
         mask[mst_idx.long()] = 1
     cx.mst_mask = mask
-    cx.info["converged"] = torch.ones(1, dtype=torch.int32)
+    cx.info["converged"] = torch.ones(1, dtype=torch.int32, device=cx.device)
     # 'auto' and 'gpu': the reference's store semantics (see rips_persistence)
     _pipeline.build_h1(cx, semantics="reference")
     return cx.h1.to(device)
