@@ -1543,19 +1543,25 @@ __device__ __forceinline__ void serial_bitmap(RedArgs& a, SerShared& sh, const u
             if (len == 0) break;
             // lowest set bit at or above the old pivot's word: one strided
             // sweep of the summary, each thread stopping at its first hit
+            // and offering that word's lowest set row.  The row, not the
+            // word, goes through LDS: after the barrier no wave reads the
+            // bitmap, which the next addition's flips may already change.
+            // (The summary mirrors the nonzero words exactly: each zero <->
+            // nonzero transition of a word is seen by exactly one flip.)
             const int ws = p >> 5;
             for (int i = (ws >> 5) + t; i < S; i += kSerThreads) {
                 unsigned val = summ[i];
                 if (i == (ws >> 5)) val &= ~0u << (ws & 31);
                 if (val) {
-                    atomicMin(&sh.first[par], i * 32 + __builtin_ctz(val));
+                    const int w = i * 32 + __builtin_ctz(val);
+                    const unsigned bw = bm_load<G>(&bits[w]);
+                    if (bw) atomicMin(&sh.first[par], w * 32 + __builtin_ctz(bw));
                     break;
                 }
             }
             __syncthreads();
             const int found = sh.first[par];
-            const unsigned bw = found != 0x7FFFFFFF ? bm_load<G>(&bits[found]) : 0u;
-            const int np = found != 0x7FFFFFFF && bw ? found * 32 + __builtin_ctz(bw) : -1;
+            const int np = found != 0x7FFFFFFF ? found : -1;
             if (!(np > p)) {  // pivot must increase
                 if (t == 0) atomicExch(&a.status[0], 3);
                 return;
