@@ -26,115 +26,132 @@ inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 // 509-521).
 // ---------------------------------------------------------------------------
 
-// own[e]: triangles whose largest edge is e (both other edges older);
-// cof[e]: all triangles containing e (0 for forest edges, whose columns are
-// cleared by H0).
-__global__ void k_tri_count(const int* __restrict__ ei, const int* __restrict__ ej, long long E,
-                            const int* __restrict__ aptr, const int* __restrict__ aidx,
-                            const int* __restrict__ arank, const unsigned char* __restrict__ mst,
-                            int* __restrict__ own_cnt, int* __restrict__ cof_cnt) {
-    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= E) return;
-    const int u = ei[e], v = ej[e];
-    int iu = aptr[u], eu = aptr[u + 1], iv = aptr[v], ev = aptr[v + 1];
-    int own = 0, cof = 0;
-    while (iu < eu && iv < ev) {
-        const int a = aidx[iu], b = aidx[iv];
-        if (a == b) {
-            cof++;
-            if (arank[iu] < e && arank[iv] < e) own++;
-            iu++;
-            iv++;
-        } else if (a < b) {
-            iu++;
-        } else {
-            iv++;
-        }
-    }
-    own_cnt[e] = own;
-    cof_cnt[e] = mst[e] ? 0 : cof;
-}
+// Links of the edges, one wavefront per edge e = (u, v): the common
+// neighbours w are found by probing the shorter neighbour list against the
+// longer one (staged in LDS when it fits), one probe per lane, and come out
+// in ascending w (the probe list's order) through a wave ballot prefix.  A
+// thread per edge walked both lists serially, which left long tails on the
+// high-degree edges of dense clouds.  Modes:
+//   kLinkCount  own[e] = triangles whose largest edge is e (both other edges
+//               older), cof[e] = all triangles containing e (0 for forest
+//               edges, whose columns H0 clears);
+//   kLinkEmit   e's own triangles in ascending w: w, the owning edge, the
+//               ranks of (u, w) and (v, w), and cmf = the largest rank among
+//               the triangle's non-forest faces (-1 if none) -- what the
+//               reference computes by scatter-max (reduce_h1_gpu.py:186-197);
+//   kLinkCof    the cofacet column of a non-forest edge: the global rank of
+//               every triangle containing it, unsorted (sorted per column
+//               afterwards).
+constexpr int kLinkWaves = 4;     // wavefronts (edges) per workgroup
+constexpr int kLinkStage = 1024;  // staged neighbour-list words per wavefront
+enum { kLinkCount = 0, kLinkEmit = 1, kLinkCof = 2 };
 
-// Emit e's own triangles in ascending third vertex w: w, the owning edge, the
-// ranks of (u, w) and (v, w), and cmf = the largest rank among the
-// triangle's non-forest faces (-1 if none) -- the quantity the reference
-// computes by scatter-max (reduce_h1_gpu.py:186-197).
-__global__ void k_tri_emit(const int* __restrict__ ei, const int* __restrict__ ej, long long E,
-                           const int* __restrict__ aptr, const int* __restrict__ aidx,
-                           const int* __restrict__ arank, const unsigned char* __restrict__ mst,
-                           const int* __restrict__ tri_off, int* __restrict__ tri_w,
-                           int* __restrict__ tri_e, int* __restrict__ tri_ru,
-                           int* __restrict__ tri_rv, int* __restrict__ tri_cmf) {
-    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= E) return;
-    const int u = ei[e], v = ej[e];
-    int iu = aptr[u], eu = aptr[u + 1], iv = aptr[v], ev = aptr[v + 1];
-    int t = tri_off[e];
+struct LinkOut {
+    int* own_cnt;
+    int* cof_cnt;
+    const int* tri_off;
+    int* tri_w;
+    int* tri_e;
+    int* tri_ru;
+    int* tri_rv;
+    int* tri_cmf;
+    const int* cof_off;
+    unsigned* cof_raw;
+};
+
+template <int MODE>
+__global__ __launch_bounds__(kLinkWaves * FPH_WAVE) void k_edge_link(
+    const int* __restrict__ ei, const int* __restrict__ ej, long long E,
+    const int* __restrict__ aptr, const int* __restrict__ aidx, const int* __restrict__ arank,
+    const unsigned char* __restrict__ mst, LinkOut o) {
+    __shared__ int stage[kLinkWaves][kLinkStage];
+    const int wid = threadIdx.x / FPH_WAVE, lane = fph_lane();
+    const long long e = (long long)blockIdx.x * kLinkWaves + wid;
+    if (e >= E) return;  // whole wavefront
     const bool me = mst[e] != 0;
-    while (iu < eu && iv < ev) {
-        const int a = aidx[iu], b = aidx[iv];
-        if (a == b) {
-            const int r1 = arank[iu], r2 = arank[iv];
-            if (r1 < e && r2 < e) {
-                tri_w[t] = a;
-                tri_e[t] = (int)e;
-                tri_ru[t] = r1;
-                tri_rv[t] = r2;
+    if (MODE == kLinkCof && me) return;
+    const int u = ei[e], v = ej[e];
+    const int pu = aptr[u], du = aptr[u + 1] - pu, pv = aptr[v], dv = aptr[v + 1] - pv;
+    const bool a_is_u = du <= dv;  // probe the shorter list
+    const int pa = a_is_u ? pu : pv, da = a_is_u ? du : dv;
+    const int pb = a_is_u ? pv : pu, db = a_is_u ? dv : du;
+    const bool staged = db <= kLinkStage;
+    int* sb = stage[wid];
+    if (staged) {
+        for (int i = lane; i < db; i += FPH_WAVE) sb[i] = aidx[pb + i];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+    }
+    const unsigned long long lt = fph_lanemask_lt();
+    int own = 0, cof = 0;
+    int pos = 0;
+    if (MODE == kLinkEmit) pos = o.tri_off[e];
+    if (MODE == kLinkCof) pos = o.cof_off[e];
+    for (int base = 0; base < da; base += FPH_WAVE) {
+        const int i = base + lane;
+        bool hit = false;
+        int w = 0, ra = 0, rb = 0;
+        if (i < da) {
+            w = aidx[pa + i];
+            int lo = 0, hi = db;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                const int x = staged ? sb[mid] : aidx[pb + mid];
+                if (x < w) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < db && (staged ? sb[lo] : aidx[pb + lo]) == w) {
+                hit = true;
+                ra = arank[pa + i];
+                rb = arank[pb + lo];
+            }
+        }
+        const int r1 = a_is_u ? ra : rb;  // rank of (u, w)
+        const int r2 = a_is_u ? rb : ra;  // rank of (v, w)
+        const bool is_own = hit && r1 < e && r2 < e;
+        const unsigned long long m_own = __ballot(is_own);
+        if (MODE == kLinkCount) {
+            own += __popcll(m_own);
+            cof += __popcll(__ballot(hit));
+        }
+        if (MODE == kLinkEmit) {
+            if (is_own) {
+                const int at = pos + __popcll(m_own & lt);
+                o.tri_w[at] = w;
+                o.tri_e[at] = (int)e;
+                o.tri_ru[at] = r1;
+                o.tri_rv[at] = r2;
                 int cmf = -1;
                 if (!me) cmf = (int)e;
                 else {
                     if (!mst[r1]) cmf = r1;
                     if (!mst[r2] && r2 > cmf) cmf = r2;
                 }
-                tri_cmf[t] = cmf;
-                t++;
+                o.tri_cmf[at] = cmf;
             }
-            iu++;
-            iv++;
-        } else if (a < b) {
-            iu++;
-        } else {
-            iv++;
+            pos += __popcll(m_own);
+        }
+        if (MODE == kLinkCof) {
+            const unsigned long long m_hit = __ballot(hit);
+            if (hit) {
+                long long rank;
+                if (is_own) {
+                    rank = E + o.tri_off[e] + own + __popcll(m_own & lt);
+                } else {
+                    const int mx = r1 > r2 ? r1 : r2;
+                    const int third = r1 > r2 ? v : u;
+                    const int lo = o.tri_off[mx], len = o.tri_off[mx + 1] - lo;
+                    rank = E + lo + fph_lower_bound(o.tri_w + lo, len, third);
+                }
+                o.cof_raw[pos + __popcll(m_hit & lt)] = (unsigned)rank;
+            }
+            own += __popcll(m_own);
+            pos += __popcll(m_hit);
         }
     }
-}
-
-// Cofacet column of every non-forest edge: the global ranks of all triangles
-// containing it (unsorted here; sorted per row afterwards).  Replaces the
-// face lookup and two-level argsort of reduce_h1_gpu.py:65-152.
-__global__ void k_cof1_emit(const int* __restrict__ ei, const int* __restrict__ ej, long long E,
-                            const int* __restrict__ aptr, const int* __restrict__ aidx,
-                            const int* __restrict__ arank, const unsigned char* __restrict__ mst,
-                            const int* __restrict__ tri_off, const int* __restrict__ tri_w,
-                            const int* __restrict__ cof_off, unsigned* __restrict__ cof_raw) {
-    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= E || mst[e]) return;
-    const int u = ei[e], v = ej[e];
-    int iu = aptr[u], eu = aptr[u + 1], iv = aptr[v], ev = aptr[v + 1];
-    int k = cof_off[e], kown = 0;
-    const long long base = E;
-    while (iu < eu && iv < ev) {
-        const int a = aidx[iu], b = aidx[iv];
-        if (a == b) {
-            const int r1 = arank[iu], r2 = arank[iv];
-            long long rank;
-            if (r1 < e && r2 < e) {
-                rank = base + tri_off[e] + kown;
-                kown++;
-            } else {
-                const int m = r1 > r2 ? r1 : r2;
-                const int third = r1 > r2 ? v : u;
-                const int lo = tri_off[m], len = tri_off[m + 1] - lo;
-                rank = base + lo + fph_lower_bound(tri_w + lo, len, third);
-            }
-            cof_raw[k++] = (unsigned)rank;
-            iu++;
-            iv++;
-        } else if (a < b) {
-            iu++;
-        } else {
-            iv++;
-        }
+    if (MODE == kLinkCount && lane == 0) {
+        o.own_cnt[e] = own;
+        o.cof_cnt[e] = me ? 0 : cof;
     }
 }
 
@@ -455,9 +472,11 @@ extern "C" int fph_triangle_count(const int32_t* edge_i, const int32_t* edge_j, 
     int* own = (int*)(w + tmp);
     int* cof = (int*)(w + tmp + align_up(4 * (size_t)std::max<int64_t>(E, 1)));
     if (E > 0) {
-        const int tb = fph_tune_value(FPH_TUNE_ENUM_BLOCK, E, 0);
-        k_tri_count<<<fph_blocks(E, tb), tb, 0, s>>>(edge_i, edge_j, E, adj_ptr, adj_idx,
-                                                      adj_rank, mst_mask, own, cof);
+        LinkOut o{};
+        o.own_cnt = own;
+        o.cof_cnt = cof;
+        k_edge_link<kLinkCount><<<fph_blocks(E, kLinkWaves), kLinkWaves * FPH_WAVE, 0, s>>>(
+            edge_i, edge_j, E, adj_ptr, adj_idx, adj_rank, mst_mask, o);
         FPH_LAUNCH_CHECK();
     }
     int rc = scan_counts(own, tri_off, E, (long long*)totals, ws, tmp, s);
@@ -474,10 +493,15 @@ extern "C" int fph_triangle_emit(const int32_t* edge_i, const int32_t* edge_j, i
     hipStream_t s = (hipStream_t)stream;
     if (E < 0) return FPH_ERR_ARG;
     if (E == 0) return FPH_OK;
-    const int tb = fph_tune_value(FPH_TUNE_ENUM_BLOCK, E, 0);
-    k_tri_emit<<<fph_blocks(E, tb), tb, 0, s>>>(edge_i, edge_j, E, adj_ptr, adj_idx, adj_rank,
-                                                 mst_mask, tri_off, tri_w, tri_e, tri_ru, tri_rv,
-                                                 tri_cmf);
+    LinkOut o{};
+    o.tri_off = tri_off;
+    o.tri_w = tri_w;
+    o.tri_e = tri_e;
+    o.tri_ru = tri_ru;
+    o.tri_rv = tri_rv;
+    o.tri_cmf = tri_cmf;
+    k_edge_link<kLinkEmit><<<fph_blocks(E, kLinkWaves), kLinkWaves * FPH_WAVE, 0, s>>>(
+        edge_i, edge_j, E, adj_ptr, adj_idx, adj_rank, mst_mask, o);
     FPH_LAUNCH_CHECK();
     return FPH_OK;
 }
@@ -497,9 +521,13 @@ extern "C" int fph_cofacets_h1(const int32_t* edge_i, const int32_t* edge_j, int
     if (*ws_bytes < need) return FPH_ERR_ARG;
     if (C == 0) return FPH_OK;
     unsigned* raw = (unsigned*)((char*)ws + tmp);
-    const int tb = fph_tune_value(FPH_TUNE_ENUM_BLOCK, E, 0);
-    k_cof1_emit<<<fph_blocks(E, tb), tb, 0, s>>>(edge_i, edge_j, E, adj_ptr, adj_idx, adj_rank,
-                                                  mst_mask, tri_off, tri_w, cof_off, raw);
+    LinkOut o{};
+    o.tri_off = tri_off;
+    o.tri_w = const_cast<int*>(tri_w);  // read only in this mode
+    o.cof_off = cof_off;
+    o.cof_raw = raw;
+    k_edge_link<kLinkCof><<<fph_blocks(E, kLinkWaves), kLinkWaves * FPH_WAVE, 0, s>>>(
+        edge_i, edge_j, E, adj_ptr, adj_idx, adj_rank, mst_mask, o);
     FPH_LAUNCH_CHECK();
     size_t t = tmp;
     const int end_bit = std::max(1, bit_width((unsigned long long)(E + T)));
