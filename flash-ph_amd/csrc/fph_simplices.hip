@@ -42,9 +42,15 @@ inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 //   kLinkCof    the cofacet column of a non-forest edge: the global rank of
 //               every triangle containing it, unsorted (sorted per column
 //               afterwards).
-constexpr int kLinkWaves = 4;     // wavefronts (edges) per workgroup
+// Wavefronts per workgroup: the autotuned ENUM_BLOCK / 64 (dynamic LDS of
+// kLinkStage words per wavefront).
 constexpr int kLinkStage = 1024;  // staged neighbour-list words per wavefront
 enum { kLinkCount = 0, kLinkEmit = 1, kLinkCof = 2 };
+
+// Wavefronts per workgroup of the link kernels for n items.
+inline int link_waves(long long n) {
+    return std::max(1, fph_tune_value(FPH_TUNE_ENUM_BLOCK, n, 0) / FPH_WAVE);
+}
 
 struct LinkOut {
     int* own_cnt;
@@ -60,13 +66,13 @@ struct LinkOut {
 };
 
 template <int MODE>
-__global__ __launch_bounds__(kLinkWaves * FPH_WAVE) void k_edge_link(
+__global__ __launch_bounds__(1024) void k_edge_link(
     const int* __restrict__ ei, const int* __restrict__ ej, long long E,
     const int* __restrict__ aptr, const int* __restrict__ aidx, const int* __restrict__ arank,
     const unsigned char* __restrict__ mst, LinkOut o) {
-    __shared__ int stage[kLinkWaves][kLinkStage];
+    extern __shared__ int stage[];
     const int wid = threadIdx.x / FPH_WAVE, lane = fph_lane();
-    const long long e = (long long)blockIdx.x * kLinkWaves + wid;
+    const long long e = (long long)blockIdx.x * (blockDim.x / FPH_WAVE) + wid;
     if (e >= E) return;  // whole wavefront
     const bool me = mst[e] != 0;
     if (MODE == kLinkCof && me) return;
@@ -76,7 +82,7 @@ __global__ __launch_bounds__(kLinkWaves * FPH_WAVE) void k_edge_link(
     const int pa = a_is_u ? pu : pv, da = a_is_u ? du : dv;
     const int pb = a_is_u ? pv : pu, db = a_is_u ? dv : du;
     const bool staged = db <= kLinkStage;
-    int* sb = stage[wid];
+    int* sb = stage + wid * kLinkStage;
     if (staged) {
         for (int i = lane; i < db; i += FPH_WAVE) sb[i] = aidx[pb + i];
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -238,13 +244,11 @@ __device__ __forceinline__ void pair_of(long long k, int* a, int* b) {
     *a = (int)(k - bb * (bb - 1) / 2);
 }
 
-constexpr int kTetWaves = 4;  // wavefronts (edges) per workgroup
-
-__global__ __launch_bounds__(kTetWaves * FPH_WAVE) void k_tet_count(
+__global__ __launch_bounds__(1024) void k_tet_count(
     long long E, const int* __restrict__ aptr, const int* __restrict__ aidx,
     const int* __restrict__ arank, const int* __restrict__ tri_off,
     const int* __restrict__ tri_w, int* __restrict__ cnt) {
-    const long long e = (long long)blockIdx.x * kTetWaves + threadIdx.x / FPH_WAVE;
+    const long long e = (long long)blockIdx.x * (blockDim.x / FPH_WAVE) + threadIdx.x / FPH_WAVE;
     if (e >= E) return;
     const int lane = fph_lane();
     const int lo = tri_off[e], len = tri_off[e + 1] - lo;
@@ -279,14 +283,14 @@ __device__ __forceinline__ int tri_face_local(const int* tri_off, const int* tri
     return tri_rank_local(tri_off, tri_w, rxy, s);
 }
 
-__global__ __launch_bounds__(kTetWaves * FPH_WAVE) void k_tet_emit(
+__global__ __launch_bounds__(1024) void k_tet_emit(
     long long E, const int* __restrict__ ei, const int* __restrict__ ej,
     const int* __restrict__ aptr, const int* __restrict__ aidx, const int* __restrict__ arank,
     const int* __restrict__ tri_off, const int* __restrict__ tri_w,
     const int* __restrict__ tri_ru, const int* __restrict__ tri_rv,
     const unsigned char* __restrict__ cleared, const int* __restrict__ tet_off,
     unsigned* __restrict__ tet_key, int* __restrict__ tet_e, int* __restrict__ tet_cmf) {
-    const long long e = (long long)blockIdx.x * kTetWaves + threadIdx.x / FPH_WAVE;
+    const long long e = (long long)blockIdx.x * (blockDim.x / FPH_WAVE) + threadIdx.x / FPH_WAVE;
     if (e >= E) return;
     const int lane = fph_lane();
     const int lo = tri_off[e], len = tri_off[e + 1] - lo;
@@ -332,33 +336,10 @@ __global__ __launch_bounds__(kTetWaves * FPH_WAVE) void k_tet_emit(
 
 // Cofacet column of every uncleared triangle {u, v, w} (u, v the ends of its
 // largest edge): tetrahedra {u, v, w, d}, d a common neighbour of all three.
-__global__ void k_cof2_count(long long T, const int* __restrict__ ei, const int* __restrict__ ej,
-                             const int* __restrict__ aptr, const int* __restrict__ aidx,
-                             const int* __restrict__ tri_w, const int* __restrict__ tri_e,
-                             const unsigned char* __restrict__ cleared, int* __restrict__ cnt) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= T) return;
-    if (cleared[t]) { cnt[t] = 0; return; }
-    const int e = tri_e[t];
-    const int u = ei[e], v = ej[e], w = tri_w[t];
-    int ia = aptr[u], ea = aptr[u + 1], ib = aptr[v], eb = aptr[v + 1], ic = aptr[w],
-        ec = aptr[w + 1];
-    int c = 0;
-    while (ia < ea && ib < eb && ic < ec) {
-        const int na = aidx[ia], nb = aidx[ib], nc = aidx[ic];
-        const int mx = max(na, max(nb, nc));
-        if (na == mx && nb == mx && nc == mx) {
-            c++;
-            ia++; ib++; ic++;
-        } else {
-            if (na < mx) ia++;
-            if (nb < mx) ib++;
-            if (nc < mx) ic++;
-        }
-    }
-    cnt[t] = c;
-}
-
+// One wavefront per triangle: the shortest of the three neighbour lists is
+// probed against the other two (staged in LDS when they fit), one probe per
+// lane.  kCount writes the column lengths, kEmit the tetrahedra's global
+// ranks (unsorted; sorted per column afterwards).
 __device__ __forceinline__ long long tet_rank_in(const int* tet_off, const unsigned* tet_key,
                                                  int m, int p, int q) {
     const unsigned key = p > q ? (((unsigned)p << 16) | (unsigned)q)
@@ -367,40 +348,90 @@ __device__ __forceinline__ long long tet_rank_in(const int* tet_off, const unsig
     return lo + fph_lower_bound(tet_key + lo, tet_off[m + 1] - lo, key);
 }
 
-__global__ void k_cof2_emit(long long T, long long E, const int* __restrict__ ei,
-                            const int* __restrict__ ej, const int* __restrict__ aptr,
-                            const int* __restrict__ aidx, const int* __restrict__ arank,
-                            const int* __restrict__ tri_w, const int* __restrict__ tri_e,
-                            const unsigned char* __restrict__ cleared,
-                            const int* __restrict__ tet_off, const unsigned* __restrict__ tet_key,
-                            const int* __restrict__ cof_off, unsigned* __restrict__ cof_raw) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= T || cleared[t]) return;
+template <int EMIT>
+__global__ __launch_bounds__(1024) void k_tri_link(
+    long long T, long long E, const int* __restrict__ ei, const int* __restrict__ ej,
+    const int* __restrict__ aptr, const int* __restrict__ aidx, const int* __restrict__ arank,
+    const int* __restrict__ tri_w, const int* __restrict__ tri_e,
+    const unsigned char* __restrict__ cleared, const int* __restrict__ tet_off,
+    const unsigned* __restrict__ tet_key, int* __restrict__ cnt, const int* __restrict__ cof_off,
+    unsigned* __restrict__ cof_raw) {
+    extern __shared__ int stage[];
+    const int wid = threadIdx.x / FPH_WAVE, lane = fph_lane();
+    const long long t = (long long)blockIdx.x * (blockDim.x / FPH_WAVE) + wid;
+    if (t >= T) return;  // whole wavefront
+    if (cleared[t]) {
+        if (!EMIT && lane == 0) cnt[t] = 0;
+        return;
+    }
     const int e = tri_e[t];
     const int u = ei[e], v = ej[e], w = tri_w[t];
-    int ia = aptr[u], ea = aptr[u + 1], ib = aptr[v], eb = aptr[v + 1], ic = aptr[w],
-        ec = aptr[w + 1];
-    int k = cof_off[t];
+    const int pu = aptr[u], du = aptr[u + 1] - pu;
+    const int pv = aptr[v], dv = aptr[v + 1] - pv;
+    const int pw = aptr[w], dw = aptr[w + 1] - pw;
+    // probe list a = the shortest (0 = u, 1 = v, 2 = w); b, c the others in
+    // vertex order (staged halves)
+    const int a = (du <= dv && du <= dw) ? 0 : (dv <= dw ? 1 : 2);
+    const int pa = a == 0 ? pu : (a == 1 ? pv : pw), da = a == 0 ? du : (a == 1 ? dv : dw);
+    const int pb = a == 0 ? pv : pu, db = a == 0 ? dv : du;
+    const int pc = a == 2 ? pv : pw, dc = a == 2 ? dv : dw;
+    const int half = kLinkStage / 2;
+    int* sb = stage + wid * kLinkStage;
+    int* sc = sb + half;
+    const bool stb = db <= half, stc = dc <= half;
+    if (stb) for (int i = lane; i < db; i += FPH_WAVE) sb[i] = aidx[pb + i];
+    if (stc) for (int i = lane; i < dc; i += FPH_WAVE) sc[i] = aidx[pc + i];
+    if (stb || stc) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+    }
+    const unsigned long long lt = fph_lanemask_lt();
+    int pos = EMIT ? cof_off[t] : 0;
+    int count = 0;
     const long long base = E + T;
-    while (ia < ea && ib < eb && ic < ec) {
-        const int na = aidx[ia], nb = aidx[ib], nc = aidx[ic];
-        const int mx = max(na, max(nb, nc));
-        if (na == mx && nb == mx && nc == mx) {
-            const int d = na;
-            const int rud = arank[ia], rvd = arank[ib], rwd = arank[ic];
+    for (int b0 = 0; b0 < da; b0 += FPH_WAVE) {
+        const int i = b0 + lane;
+        bool hit = false;
+        int d = 0, jb = 0, jc = 0;
+        if (i < da) {
+            d = aidx[pa + i];
+            int lo = 0, hi = db;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if ((stb ? sb[mid] : aidx[pb + mid]) < d) lo = mid + 1;
+                else hi = mid;
+            }
+            jb = lo;
+            if (jb < db && (stb ? sb[jb] : aidx[pb + jb]) == d) {
+                lo = 0;
+                hi = dc;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if ((stc ? sc[mid] : aidx[pc + mid]) < d) lo = mid + 1;
+                    else hi = mid;
+                }
+                jc = lo;
+                hit = jc < dc && (stc ? sc[jc] : aidx[pc + jc]) == d;
+            }
+        }
+        const unsigned long long m = __ballot(hit);
+        if (EMIT && hit) {
+            const int ra = arank[pa + i], rb = arank[pb + jb], rc = arank[pc + jc];
+            // back to vertex order: b is the first of the other two, c the second
+            const int rud = a == 0 ? ra : rb;
+            const int rvd = a == 1 ? ra : (a == 0 ? rb : rc);
+            const int rwd = a == 2 ? ra : rc;
             long long local;
             if (rud < e && rvd < e && rwd < e) local = tet_rank_in(tet_off, tet_key, e, w, d);
             else if (rud > rvd && rud > rwd) local = tet_rank_in(tet_off, tet_key, rud, v, w);
             else if (rvd > rwd) local = tet_rank_in(tet_off, tet_key, rvd, u, w);
             else local = tet_rank_in(tet_off, tet_key, rwd, u, v);
-            cof_raw[k++] = (unsigned)(base + local);
-            ia++; ib++; ic++;
-        } else {
-            if (na < mx) ia++;
-            if (nb < mx) ib++;
-            if (nc < mx) ic++;
+            cof_raw[pos + __popcll(m & lt)] = (unsigned)(base + local);
         }
+        pos += __popcll(m);
+        count += __popcll(m);
     }
+    if (!EMIT && lane == 0) cnt[t] = count;
 }
 
 // counts[0..K) -> off[0..K], total into *total_dev (int64).
@@ -475,7 +506,8 @@ extern "C" int fph_triangle_count(const int32_t* edge_i, const int32_t* edge_j, 
         LinkOut o{};
         o.own_cnt = own;
         o.cof_cnt = cof;
-        k_edge_link<kLinkCount><<<fph_blocks(E, kLinkWaves), kLinkWaves * FPH_WAVE, 0, s>>>(
+        const int nw = link_waves(E);
+        k_edge_link<kLinkCount><<<fph_blocks(E, nw), nw * FPH_WAVE, nw * kLinkStage * 4, s>>>(
             edge_i, edge_j, E, adj_ptr, adj_idx, adj_rank, mst_mask, o);
         FPH_LAUNCH_CHECK();
     }
@@ -500,7 +532,8 @@ extern "C" int fph_triangle_emit(const int32_t* edge_i, const int32_t* edge_j, i
     o.tri_ru = tri_ru;
     o.tri_rv = tri_rv;
     o.tri_cmf = tri_cmf;
-    k_edge_link<kLinkEmit><<<fph_blocks(E, kLinkWaves), kLinkWaves * FPH_WAVE, 0, s>>>(
+    const int nw = link_waves(E);
+    k_edge_link<kLinkEmit><<<fph_blocks(E, nw), nw * FPH_WAVE, nw * kLinkStage * 4, s>>>(
         edge_i, edge_j, E, adj_ptr, adj_idx, adj_rank, mst_mask, o);
     FPH_LAUNCH_CHECK();
     return FPH_OK;
@@ -526,7 +559,8 @@ extern "C" int fph_cofacets_h1(const int32_t* edge_i, const int32_t* edge_j, int
     o.tri_w = const_cast<int*>(tri_w);  // read only in this mode
     o.cof_off = cof_off;
     o.cof_raw = raw;
-    k_edge_link<kLinkCof><<<fph_blocks(E, kLinkWaves), kLinkWaves * FPH_WAVE, 0, s>>>(
+    const int nw = link_waves(E);
+    k_edge_link<kLinkCof><<<fph_blocks(E, nw), nw * FPH_WAVE, nw * kLinkStage * 4, s>>>(
         edge_i, edge_j, E, adj_ptr, adj_idx, adj_rank, mst_mask, o);
     FPH_LAUNCH_CHECK();
     size_t t = tmp;
@@ -587,7 +621,8 @@ extern "C" int fph_tet_count(int64_t E, const int32_t* adj_ptr, const int32_t* a
     if (*ws_bytes < need) return FPH_ERR_ARG;
     int* cnt = (int*)((char*)ws + tmp);
     if (E > 0) {
-        k_tet_count<<<fph_blocks(E, kTetWaves), kTetWaves * FPH_WAVE, 0, s>>>(
+        const int nw = link_waves(E);
+        k_tet_count<<<fph_blocks(E, nw), nw * FPH_WAVE, 0, s>>>(
             E, adj_ptr, adj_idx, adj_rank, tri_off, tri_w, cnt);
         FPH_LAUNCH_CHECK();
     }
@@ -603,7 +638,8 @@ extern "C" int fph_tet_emit(int64_t E, const int32_t* edge_i, const int32_t* edg
     hipStream_t s = (hipStream_t)stream;
     if (E < 0) return FPH_ERR_ARG;
     if (E == 0) return FPH_OK;
-    k_tet_emit<<<fph_blocks(E, kTetWaves), kTetWaves * FPH_WAVE, 0, s>>>(
+    const int nw = link_waves(E);
+    k_tet_emit<<<fph_blocks(E, nw), nw * FPH_WAVE, 0, s>>>(
         E, edge_i, edge_j, adj_ptr, adj_idx, adj_rank, tri_off, tri_w, tri_ru, tri_rv, cleared,
         tet_off, tet_key, tet_e, tet_cmf);
     FPH_LAUNCH_CHECK();
@@ -623,9 +659,10 @@ extern "C" int fph_cofacets_h2_count(int64_t T, const int32_t* edge_i, const int
     if (*ws_bytes < need) return FPH_ERR_ARG;
     int* cnt = (int*)((char*)ws + tmp);
     if (T > 0) {
-        const int tb = fph_tune_value(FPH_TUNE_ENUM_BLOCK, T, 0);
-        k_cof2_count<<<fph_blocks(T, tb), tb, 0, s>>>(T, edge_i, edge_j, adj_ptr, adj_idx,
-                                                       tri_w, tri_e, cleared, cnt);
+        const int nw = link_waves(T);
+        k_tri_link<0><<<fph_blocks(T, nw), nw * FPH_WAVE, nw * kLinkStage * 4, s>>>(
+            T, 0, edge_i, edge_j, adj_ptr, adj_idx, nullptr, tri_w, tri_e, cleared, nullptr,
+            nullptr, cnt, nullptr, nullptr);
         FPH_LAUNCH_CHECK();
     }
     return scan_counts(cnt, cof_off, T, (long long*)total, ws, tmp, s);
@@ -647,10 +684,10 @@ extern "C" int fph_cofacets_h2(int64_t T, int64_t E, int64_t Q, const int32_t* e
     if (*ws_bytes < need) return FPH_ERR_ARG;
     if (C == 0) return FPH_OK;
     unsigned* raw = (unsigned*)((char*)ws + tmp);
-    const int tb = fph_tune_value(FPH_TUNE_ENUM_BLOCK, T, 0);
-    k_cof2_emit<<<fph_blocks(T, tb), tb, 0, s>>>(T, E, edge_i, edge_j, adj_ptr, adj_idx,
-                                                  adj_rank, tri_w, tri_e, cleared, tet_off,
-                                                  tet_key, cof_off, raw);
+    const int nw = link_waves(T);
+    k_tri_link<1><<<fph_blocks(T, nw), nw * FPH_WAVE, nw * kLinkStage * 4, s>>>(
+        T, E, edge_i, edge_j, adj_ptr, adj_idx, adj_rank, tri_w, tri_e, cleared, tet_off,
+        tet_key, nullptr, cof_off, raw);
     FPH_LAUNCH_CHECK();
     size_t t = tmp;
     const int end_bit = std::max(1, bit_width((unsigned long long)(E + T + Q)));

@@ -4,9 +4,10 @@
 // gives bit-identical outputs):
 //   FPH_TUNE_SCREEN_TILE     points per side of a distance tile of the direct
 //                            edge screen (32, 64 or 128), keyed by (n, d);
-//   FPH_TUNE_ENUM_BLOCK      edges per workgroup of the intersection kernels
-//                            (triangle count and emit, cofacet columns),
-//                            keyed by the number of items;
+//   FPH_TUNE_ENUM_BLOCK      threads per workgroup of the link kernels
+//                            (triangles, tetrahedra, cofacet columns; one
+//                            wavefront per edge or triangle), keyed by the
+//                            number of items;
 //   FPH_TUNE_APPARENT_BLOCK  columns per workgroup of the apparent-pair and
 //                            clearing kernels, keyed by the number of columns;
 //   FPH_TUNE_QUEUE_WGS       workgroups of the residual reduction's queue for

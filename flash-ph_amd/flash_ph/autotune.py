@@ -4,8 +4,9 @@ Four knobs set how work is spread over the GPU and never what is computed:
 
 * ``screen_tile``: points per side of a distance tile of the direct edge
   screen (32, 64, 128), per (n, d) bucket;
-* ``enum_block``: edges per workgroup of the triangle / tetrahedron /
-  cofacet intersection kernels, per item-count bucket;
+* ``enum_block``: threads per workgroup of the link kernels (triangles,
+  tetrahedra, cofacet columns; one wavefront per edge or triangle), per
+  item-count bucket;
 * ``apparent_block``: columns per workgroup of the apparent-pair and
   clearing kernels, per column-count bucket;
 * ``queue_wgs``: workgroups of the residual reduction's queue (1: one

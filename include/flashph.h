@@ -460,8 +460,9 @@ int fph_graph_destroy(void *graph);
  * Launch parameters that change the tiling, never the results.  Values are
  * kept per bucket (floor(log2 size), floor(log2 d)), host memory only; 0 =
  * the default.  Keys: SCREEN_TILE by (n, d) of fph_edge_screen;
- * ENUM_BLOCK by (E, 0) for the triangle / cofacet kernels (tetrahedra run
- * one wavefront per edge)
+ * ENUM_BLOCK by (E, 0) for the link kernels of triangles, tetrahedra and H1
+ * cofacets (one wavefront per edge; threads per workgroup = 64 x edges per
+ * workgroup)
  * (by (T, 0) for fph_cofacets_h2_count and fph_cofacets_h2); APPARENT_BLOCK by
  * (K, 0) for fph_apparent and fph_clear_pivots; QUEUE_WGS by (K, 0) for
  * fph_reduce with nseg == 1.  The reference picks its
