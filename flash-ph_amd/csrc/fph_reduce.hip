@@ -69,7 +69,38 @@ struct RedArgs {
     unsigned long long pool_cap;
     unsigned long long* pool_ptr;
     int* status;  // [0] error, [1] ran, [2] XORs, [3] longest column, [4] off-chip XORs, ...
+    // owner lines (device-wide bitmap queue): per row kLineWords 64-bit
+    // words ((owner + 1) << 32 | value), values = the owner column's length
+    // and first rows; see reduce_claim_dev_bm.  use_lines asks for them;
+    // `lines` is set in the kernels (lines_of), null when not used.
+    int use_lines;
+    unsigned long long* lines;
 };
+
+constexpr int kLineWords = 16;  // words per owner line: length + 15 rows
+
+// Word offset of the owner lines in the pool: right after the apparent
+// region (off[K] + K words, line-rounded) and one line of padding.
+__device__ __forceinline__ unsigned long long lines_base(const int* off, long long K) {
+    return ((unsigned long long)off[K] + (unsigned long long)K + 31ull) / 32ull * 32ull + 32ull;
+}
+
+// The owner lines of this reduction, or null: not asked for, or no room for
+// them below the pool capacity (the same test everywhere, so every kernel
+// agrees).
+__device__ __forceinline__ unsigned long long* lines_of(const RedArgs& a) {
+    if (!a.use_lines) return nullptr;
+    const unsigned long long base = lines_base(a.off, a.K);
+    if (base + 2ull * kLineWords * (unsigned long long)a.nrows > a.pool_cap) return nullptr;
+    return (unsigned long long*)(a.pool + base);
+}
+
+__device__ __forceinline__ void write_line_word(const RedArgs& a, long long row, int k, int c,
+                                                unsigned value) {
+    __hip_atomic_store(&a.lines[row * kLineWords + k],
+                       ((unsigned long long)(unsigned)(c + 1) << 32) | value, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // status[6..7]: 64-bit count of entries merged (na + nb + result length
 // per column addition), for the algorithmic traffic of the reduction.
@@ -539,6 +570,7 @@ __device__ __forceinline__ void note_len(const RedArgs& a, int len) {
 // (no allocation), and own[] records them.  Dynamic allocations start after
 // this region.
 __global__ void k_publish_apparent(RedArgs a, long long K, int set_own) {
+    a.lines = lines_of(a);
     const int wpb = blockDim.x / FPH_WAVE;
     const long long nw = (long long)gridDim.x * wpb;
     for (long long c = (long long)blockIdx.x * wpb + threadIdx.x / FPH_WAVE; c < K; c += nw) {
@@ -552,7 +584,23 @@ __global__ void k_publish_apparent(RedArgs a, long long K, int set_own) {
                     ((unsigned long long)(c + 1) << 32) | (unsigned long long)at;
         }
         for (int i = fph_lane(); i < len; i += FPH_WAVE) a.pool[at + 1 + i] = a.data[lo + i];
+        if (a.lines && set_own) {
+            const int k = fph_lane();
+            const long long row = (long long)a.data[lo] - a.row_base;
+            if (k == 0) write_line_word(a, row, 0, (int)c, (unsigned)len);
+            else if (k < kLineWords && k <= len) write_line_word(a, row, k, (int)c, a.data[lo + k - 1]);
+        }
     }
+}
+
+// Owner lines start all zero (tag 0 matches no owner).
+__global__ void k_zero_lines(RedArgs a) {
+    unsigned long long* lines = lines_of(a);
+    if (!lines) return;
+    const long long n = a.nrows * kLineWords;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        lines[i] = 0ull;
 }
 
 // Phase A: one wavefront per residual column, chaining apparent owners.
@@ -976,8 +1024,9 @@ __device__ void bm_next_pivot(const RedArgs& a, BmWork& w) {
     }
 }
 
-// Publishes the set rows as [len, rows...] (write-once words); -1 on error.
-__device__ long long bm_publish(const RedArgs& a, const BmWork& w, Alloc& al) {
+// Publishes the set rows as [len, rows...] (write-once words) and, with
+// owner lines, the line of the pivot row tagged with column c; -1 on error.
+__device__ long long bm_publish(const RedArgs& a, const BmWork& w, Alloc& al, int c) {
     const int lane = fph_lane();
     const long long at = pool_alloc(a, al, w.len + 1);
     if (at < 0) return -1;
@@ -996,13 +1045,19 @@ __device__ long long bm_publish(const RedArgs& a, const BmWork& w, Alloc& al) {
         int pos = done + incl - cnt;
         while (x && pos < w.len) {  // pos < len always; the bound keeps writes inside
             const int b = __ffs(x) - 1;
-            st_coh(dst + pos, (unsigned)a.row_base + 32u * (unsigned)i + (unsigned)b);
+            const unsigned row = (unsigned)a.row_base + 32u * (unsigned)i + (unsigned)b;
+            st_coh(dst + pos, row);
+            if (a.lines && pos < kLineWords - 1)
+                write_line_word(a, (long long)w.piv - a.row_base, pos + 1, c, row);
             pos++;
             x &= x - 1u;
         }
         done += __shfl(incl, FPH_WAVE - 1);
     }
-    if (lane == 0) st_coh(a.pool + at, (unsigned)w.len);
+    if (lane == 0) {
+        st_coh(a.pool + at, (unsigned)w.len);
+        if (a.lines) write_line_word(a, (long long)w.piv - a.row_base, 0, c, (unsigned)w.len);
+    }
     wave_sync();
     return at;
 }
@@ -1050,9 +1105,14 @@ __device__ int reduce_claim_dev_bm(const RedArgs& a, int c, unsigned src, bool c
             return -1;
         }
         const unsigned p = w.piv;
+        // the owner entry and, in the same round trip, the owner line
+        unsigned long long lw = 0;
+        if (a.lines && fph_lane() < kLineWords)
+            lw = __hip_atomic_load(&a.lines[((long long)p - a.row_base) * kLineWords + fph_lane()],
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         unsigned long long v = load_own_fast(a, p);
         if ((int)(v >> 32) - 1 <= c) {
-            const long long at = bm_publish(a, w, al);
+            const long long at = bm_publish(a, w, al, c);
             if (at < 0) return -2;
             if (fph_lane() == 0) st_coh(&a.desc[c], (unsigned)at);
             const unsigned long long mine =
@@ -1071,11 +1131,31 @@ __device__ int reduce_claim_dev_bm(const RedArgs& a, int c, unsigned src, bool c
             v = old;
         }
         const unsigned* ob = a.pool + (v & 0xFFFFFFFFull);
-        unsigned vb;
-        const int nb = spec_read_dev(ob, vb, wait);
+        // The line serves when every word up to the length carries the tag
+        // of the owner just read: (owner, pivot) names one published state
+        // of the owner (its pivot only rises), so the line holds that
+        // state's first rows.  Otherwise (stale, torn, longer than the line)
+        // the column itself is read.
+        unsigned vb = 0;
+        int nb = -1;
+        if (a.lines && (unsigned long long)v != kNoOwner) {
+            const unsigned tag = (unsigned)(v >> 32);
+            const unsigned lo32 = (unsigned)lw, hi32 = (unsigned)(lw >> 32);
+            const unsigned t0 = __shfl(hi32, 0), l0 = __shfl(lo32, 0);
+            if (t0 == tag && l0 <= (unsigned)(kLineWords - 1)) {
+                const bool need = fph_lane() >= 1 && fph_lane() <= (int)l0;
+                if (!__ballot(need && hi32 != tag)) {
+                    nb = (int)l0;
+                    vb = lo32;
+                }
+            }
+        }
         if (nb < 0) {
-            fail_wait(a);
-            return -2;
+            nb = spec_read_dev(ob, vb, wait);
+            if (nb < 0) {
+                fail_wait(a);
+                return -2;
+            }
         }
         const int na = w.len;
         if (bm_flip_published(a, w, ob, vb, nb, wait) == 0xFFFFFFFFu) {
@@ -1101,6 +1181,7 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_queue_dev(
     int bitmap, int from_csr) {
     extern __shared__ unsigned smem[];
     __shared__ int s_part;
+    a.lines = lines_of(a);
     // Only the workgroups on one XCD take part (the XCD of the first one to
     // start): they share that XCD's L2, where the columns they publish stay,
     // so workgroup-scope reads (ld_fast) mostly hit.  Correctness does not
@@ -1843,7 +1924,7 @@ __global__ void k_set_u64(unsigned long long* p, unsigned long long v) {
 }
 
 __global__ void k_reduce_init(int* __restrict__ status, unsigned long long* __restrict__ pool_ptr,
-                              const int* __restrict__ off, long long K) {
+                              const int* __restrict__ off, long long K, RedArgs a) {
     if (threadIdx.x == 0) {
         status[0] = 0;
         status[1] = 0;
@@ -1856,6 +1937,8 @@ __global__ void k_reduce_init(int* __restrict__ status, unsigned long long* __re
         // apparent region: off[K] + K words, one line of padding (speculative
         // reads of the last apparent column), then line-aligned dynamic space
         pool_ptr[0] = ((unsigned long long)off[K] + K + kLine - 1) / kLine * kLine + kLine;
+        // owner lines, when used, come next
+        if (lines_of(a)) pool_ptr[0] += 2ull * kLineWords * (unsigned long long)a.nrows;
     }
 }
 
@@ -1921,7 +2004,9 @@ extern "C" int fph_reduce_serial(int64_t K, const int32_t* col_off, const uint32
     a.pool = pool;
     a.pool_cap = (unsigned long long)pool_cap;
     a.status = status;
-    k_reduce_init<<<1, 64, 0, s>>>(status, a.pool_ptr, col_off, K);
+    a.use_lines = 0;
+    a.lines = nullptr;
+    k_reduce_init<<<1, 64, 0, s>>>(status, a.pool_ptr, col_off, K, a);
     FPH_LAUNCH_CHECK();
     k_set_u64<<<1, 64, 0, s>>>(a.pool_ptr, (unsigned long long)pool_start);
     FPH_LAUNCH_CHECK();
@@ -1976,22 +2061,29 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
     a.pool = pool;
     a.pool_cap = (unsigned long long)pool_cap;
     a.status = status;
-    k_reduce_init<<<1, 64, 0, s>>>(status, a.pool_ptr, col_off, K);
+    // one segment: the queue may span several workgroups (autotuned); with
+    // bitmap working columns it also takes phase A's additions (cheaper
+    // there than as list merges), starts columns from the CSR and keeps
+    // owner lines
+    const int qwgs = nseg == 1 ? fph_tune_value(FPH_TUNE_QUEUE_WGS, K, 0) : 1;
+    const int bitmap = (nrows + 31) / 32 <= kArenaB ? 1 : 0;
+    const int from_csr = qwgs > 1 && bitmap;
+    a.use_lines = from_csr;
+    a.lines = nullptr;
+    k_reduce_init<<<1, 64, 0, s>>>(status, a.pool_ptr, col_off, K, a);
     FPH_LAUNCH_CHECK();
     if (K == 0) {
         k_reduce_fini<<<1, 64, 0, s>>>(status, a.pool_ptr);
         FPH_LAUNCH_CHECK();
         return FPH_OK;
     }
+    if (from_csr) {
+        k_zero_lines<<<256, 256, 0, s>>>(a);
+        FPH_LAUNCH_CHECK();
+    }
     const unsigned grid_w =
         (unsigned)std::min<int64_t>(1024, (K + kPhaseAWaves - 1) / kPhaseAWaves);
     k_publish_apparent<<<grid_w, kPhaseAWaves * FPH_WAVE, 0, s>>>(a, K, 1);
-    // one segment: the queue may span several workgroups (autotuned); with
-    // bitmap working columns it also takes phase A's additions (cheaper
-    // there than as list merges) and starts columns from the CSR
-    const int qwgs = nseg == 1 ? fph_tune_value(FPH_TUNE_QUEUE_WGS, K, 0) : 1;
-    const int bitmap = (nrows + 31) / 32 <= kArenaB ? 1 : 0;
-    const int from_csr = qwgs > 1 && bitmap;
     const size_t lds_a = kPhaseAWaves * kArenaA * 4;
     if (!from_csr)
         k_reduce_chain<<<grid_w, kPhaseAWaves * FPH_WAVE, lds_a, s>>>(a, res_list, res_count);

@@ -379,8 +379,9 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
     counts_p = info32.data_ptr() + 48
     scratch_bytes = lib.fph_reduce_scratch_bytes(K)
     if ref_sizes is None:
-        # apparent columns in place (C + K words) + line-rounded reduced columns
-        pool_words = C + K + 64 + max(_POOL_MIN_WORDS, 64 * K + 4 * C)
+        # apparent columns in place (C + K words), owner lines (32 words per
+        # row, used by the device-wide queue) + line-rounded reduced columns
+        pool_words = C + K + 64 + 32 * nrows + max(_POOL_MIN_WORDS, 64 * K + 4 * C)
     else:
         caps = ([1 << 62] * len(ref_sizes) if serial_exact else
                 reference_pool_caps([(cof, kc) for cof, kc, _ in ref_sizes]))
