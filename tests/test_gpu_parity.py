@@ -186,3 +186,19 @@ def test_exact_serial_and_parallel_agree():
                 _check_case(make(), tt, 2)
     finally:
         _pipeline.EXACT_SERIAL_MIN_ROWS = old
+
+
+def test_device_queue_list_form():
+    """n=10K, d=10 at t=1.72: 94,249 triangle rows do not fit a wavefront's
+    bitmap arena (2496 words), so the device-wide queue runs phase A and
+    sorted-list working columns; the H1 diagram is the oracle's complete
+    reduction, bit for bit."""
+    import oracle
+    from flash_ph import _pipeline
+    print("[gpu] device queue, list working columns", flush=True)
+    x = cases.randn(10000, 10, 0)
+    t = 1.72
+    ex = oracle.run(x, t, 1, exact=True)
+    dg, cx = _pipeline.compute(torch.from_numpy(x).cuda(), t, 1, semantics="exact")
+    assert cx.T == ex["stats"]["T"] and (cx.T + 31) // 32 > 2496 and cx.T <= 100_000
+    _eq("h1 (list-form device queue)", _cpu(dg[1]), ex["h1"])
