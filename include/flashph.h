@@ -470,7 +470,7 @@ int fph_graph_destroy(void *graph);
  * kernels/triangle_kernel.py:366-397, reduce_h1_gpu.py:159-223); this table
  * replaces them with values a tuner measures (flash_ph.autotune).
  */
-#define FPH_TUNE_SCREEN_TILE 0    /* points per tile side: 32, 64 (default), 128 */
+#define FPH_TUNE_SCREEN_TILE 0    /* points per tile side: 32, 64, 128 (default) */
 #define FPH_TUNE_ENUM_BLOCK 1     /* threads per workgroup: 64 .. 1024 (default 256) */
 #define FPH_TUNE_APPARENT_BLOCK 2 /* threads per workgroup: 64 .. 1024 (default 256) */
 #define FPH_TUNE_QUEUE_WGS 3      /* fph_reduce queue workgroups, one segment: 1 .. 64 (default 16 from 8192 columns, else 1) */

@@ -26,24 +26,24 @@ def test_candidates_and_defaults():
     assert autotune.query("queue_wgs", 26_000) == 16
     assert autotune.query("queue_wgs", 8192) == 16
     assert autotune.query("queue_wgs", 8191) == 1
-    assert autotune.query("screen_tile", 10_000, 10) == 64
+    assert autotune.query("screen_tile", 10_000, 10) == 128
     assert autotune.query("enum_block", 26_000) == 256
     assert autotune.query("apparent_block", 26_000) == 256
 
 
 def test_buckets_are_log2_of_size_and_d():
-    autotune.set_value("screen_tile", 10_000, 10, 128)
+    autotune.set_value("screen_tile", 10_000, 10, 32)
     # same bucket: floor(log2) of 8192..16383 and of 8..15
-    assert autotune.query("screen_tile", 8192, 8) == 128
-    assert autotune.query("screen_tile", 16383, 15) == 128
+    assert autotune.query("screen_tile", 8192, 8) == 32
+    assert autotune.query("screen_tile", 16383, 15) == 32
     # neighbouring buckets keep the default
-    assert autotune.query("screen_tile", 16384, 10) == 64
-    assert autotune.query("screen_tile", 10_000, 16) == 64
-    assert autotune.query("screen_tile", 8191, 10) == 64
+    assert autotune.query("screen_tile", 16384, 10) == 128
+    assert autotune.query("screen_tile", 10_000, 16) == 128
+    assert autotune.query("screen_tile", 8191, 10) == 128
     # knobs are independent
     assert autotune.query("enum_block", 10_000, 10) == 256
     autotune.set_value("screen_tile", 10_000, 10, 0)
-    assert autotune.query("screen_tile", 10_000, 10) == 64
+    assert autotune.query("screen_tile", 10_000, 10) == 128
 
 
 def test_invalid_values_rejected():
@@ -59,7 +59,7 @@ def test_invalid_values_rejected():
     assert lib.fph_autotune_query(7, 10, 0) == _lib.FPH_ERR_ARG
     assert lib.fph_autotune_set(4, 10, 0, 64) == _lib.FPH_ERR_ARG
     assert lib.fph_autotune_set(3, 10, 0, 3) == _lib.FPH_ERR_ARG
-    assert autotune.query("screen_tile", 1000, 10) == 64
+    assert autotune.query("screen_tile", 1000, 10) == 128
 
 
 def test_reset_restores_defaults():
