@@ -593,16 +593,6 @@ __global__ void k_publish_apparent(RedArgs a, long long K, int set_own) {
     }
 }
 
-// Owner lines start all zero (tag 0 matches no owner).
-__global__ void k_zero_lines(RedArgs a) {
-    unsigned long long* lines = lines_of(a);
-    if (!lines) return;
-    const long long n = a.nrows * kLineWords;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x)
-        lines[i] = 0ull;
-}
-
 // Phase A: one wavefront per residual column, chaining apparent owners.
 __global__ __launch_bounds__(kPhaseAWaves * FPH_WAVE) void k_reduce_chain(
     RedArgs a, const int* __restrict__ res_list, const int* __restrict__ res_count) {
@@ -1321,6 +1311,38 @@ __global__ void k_queue_state_init(int* __restrict__ qs, const int* __restrict__
     }
 }
 
+// Set-up of the device-wide queue with bitmap working columns, one launch:
+// owner lines zero, the ring empty, desc[] unpublished, pivot[] reset (no
+// phase A ran and the queue never reads pivot[]; k_final_own rebuilds it),
+// the unallocated pool reading kUnwritten, and the queue state.
+__global__ void k_queue_prep(RedArgs a, unsigned long long* __restrict__ ring,
+                             int* __restrict__ qs, const int* __restrict__ res_count,
+                             long long fill_end) {
+    unsigned long long* lines = lines_of(a);
+    const long long nl = lines ? a.nrows * kLineWords : 0;
+    const long long start = (long long)*a.pool_ptr;
+    const long long nf = fill_end > start ? fill_end - start : 0;
+    const long long n = max(max(nl, a.K), nf);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (i < nl) lines[i] = 0ull;
+        if (i < a.K) {
+            ring[i] = ~0ull;
+            a.desc[i] = kUnwritten;
+            if (a.kind[i] == 2 || a.kind[i] == 3) a.pivot[i] = -1;
+        }
+        if (i < nf) a.pool[start + i] = kUnwritten;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        qs[0] = 0;
+        qs[1] = 0;
+        qs[2] = 0;
+        qs[3] = *res_count;
+        qs[4] = a.status[0] != 0;
+        qs[5] = -1;
+    }
+}
+
 // After the device-wide queue: every nonzero column owns exactly its pivot
 // row, so own[] alone gives pivot[] and desc[] (reset first, then set).
 __global__ void k_final_reset(RedArgs a) {
@@ -1336,11 +1358,6 @@ __global__ void k_final_own(RedArgs a) {
     const long long c = (long long)(v >> 32) - 1;
     a.pivot[c] = (int)(r + a.row_base);
     a.desc[c] = (unsigned)(v & 0xFFFFFFFFull);
-}
-
-__global__ void k_fill_u32(unsigned* __restrict__ p, long long n, unsigned v) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
 }
 
 __global__ void k_fill_u64(unsigned long long* __restrict__ p, long long n, unsigned long long v) {
@@ -2077,8 +2094,11 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
         FPH_LAUNCH_CHECK();
         return FPH_OK;
     }
-    if (from_csr) {
-        k_zero_lines<<<256, 256, 0, s>>>(a);
+    // ring of displaced owners: K 64-bit words over the two worklist areas
+    unsigned long long* ring = (unsigned long long*)wl0;
+    int* qs = (int*)(w + 5 * k4 + sort_sz + 64);  // queue state, in the 256-byte tail
+    if (from_csr) {  // before k_publish_apparent, which writes owner lines
+        k_queue_prep<<<1024, 256, 0, s>>>(a, ring, qs, res_count, pool_cap + 64);
         FPH_LAUNCH_CHECK();
     }
     const unsigned grid_w =
@@ -2091,10 +2111,10 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
     FPH_LAUNCH_CHECK();
     FPH_CHECK_HIP(rocprim::radix_sort_keys_desc(sort_tmp, sort_sz, keys, order, (unsigned)K, 0,
                                                 32, s));
-    // ring of displaced owners: K 64-bit words over the two worklist areas
-    unsigned long long* ring = (unsigned long long*)wl0;
-    k_fill_u64<<<fph_blocks(K, 256), 256, 0, s>>>(ring, K, ~0ull);
-    FPH_LAUNCH_CHECK();
+    if (!from_csr) {
+        k_fill_u64<<<fph_blocks(K, 256), 256, 0, s>>>(ring, K, ~0ull);
+        FPH_LAUNCH_CHECK();
+    }
     const size_t lds_b = kPhaseBWaves * kArenaB * 4;
     FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_reduce_queue,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
@@ -2106,14 +2126,11 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
     const long long secs = wait_seconds > 0 ? (long long)wait_seconds : 120LL;
     const long long limit_ticks = secs * (long long)std::max(khz, 1) * 1000LL;
     if (qwgs > 1) {
-        int* qs = (int*)(w + 5 * k4 + sort_sz + 64);  // in the 256-byte tail
-        k_fill_unwritten<<<1024, 256, 0, s>>>(pool, a.pool_ptr, pool_cap + 64);
-        if (from_csr) {  // no published state yet: desc[] reads kUnwritten
-            k_fill_u32<<<fph_blocks(K, 256), 256, 0, s>>>(a.desc, K, kUnwritten);
+        if (!from_csr) {
+            k_fill_unwritten<<<1024, 256, 0, s>>>(pool, a.pool_ptr, pool_cap + 64);
+            k_queue_state_init<<<1, 64, 0, s>>>(qs, res_count, status);
             FPH_LAUNCH_CHECK();
         }
-        k_queue_state_init<<<1, 64, 0, s>>>(qs, res_count, status);
-        FPH_LAUNCH_CHECK();
         FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_reduce_queue_dev,
                                           hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)lds_b));
@@ -2122,7 +2139,7 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
         k_reduce_queue_dev<<<(unsigned)(8 * qwgs), kPhaseBWaves * FPH_WAVE, lds_b, s>>>(
             a, order, res_count, ring, qs, limit_ticks, bitmap, from_csr);
         FPH_LAUNCH_CHECK();
-        k_final_reset<<<fph_blocks(K, 256), 256, 0, s>>>(a);
+        if (!from_csr) k_final_reset<<<fph_blocks(K, 256), 256, 0, s>>>(a);
         k_final_own<<<fph_blocks(std::max<int64_t>(nrows, 1), 256), 256, 0, s>>>(a);
         FPH_LAUNCH_CHECK();
     } else {
