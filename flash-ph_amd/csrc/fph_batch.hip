@@ -107,12 +107,9 @@ __global__ __launch_bounds__(kThreads) void k_edge_screen_batch(
             bal[r * 4 + q] = __ballot(ok);
             total += __popcll(bal[r * 4 + q]);
         }
+    unsigned long long pos = fph_wg_reserve<kThreads>(total, count);
     if (total == 0) return;
-    unsigned long long base = 0;
-    if (fph_lane() == 0) base = atomicAdd(count, (unsigned long long)total);
-    base = __shfl(base, 0);
     const unsigned long long lt = fph_lanemask_lt();
-    unsigned long long pos = base;
     const unsigned long long vb = (unsigned long long)vbase[c];
 #pragma unroll
     for (int r = 0; r < 4; r++)

@@ -74,6 +74,31 @@ __device__ __forceinline__ unsigned long long fph_lanemask_lt() {
     return l == 0 ? 0ull : ((~0ull) >> (64 - l));
 }
 
+// Reserves a workgroup's output range with one atomic on a shared counter
+// and returns the calling wave's start: the waves' totals (`total`, uniform
+// per wave) go through LDS, thread 0 adds their sum to *count, and wave w
+// starts after the totals of waves 0..w-1.  One append per wave serialised
+// on the counter at about 12 ns each on MI355X; per workgroup it is a
+// quarter of that.  Contains barriers: every thread of the workgroup calls it.
+template <int THREADS>
+__device__ __forceinline__ unsigned long long fph_wg_reserve(int total,
+                                                             unsigned long long* count) {
+    __shared__ int s_wt[THREADS / FPH_WAVE];
+    __shared__ unsigned long long s_base;
+    const int wid = (int)threadIdx.x / FPH_WAVE;
+    if (fph_lane() == 0) s_wt[wid] = total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int all = 0;
+        for (int w = 0; w < THREADS / FPH_WAVE; w++) all += s_wt[w];
+        s_base = all ? atomicAdd(count, (unsigned long long)all) : 0ull;
+    }
+    __syncthreads();
+    unsigned long long pos = s_base;
+    for (int w = 0; w < wid; w++) pos += (unsigned long long)s_wt[w];
+    return pos;
+}
+
 // Correctly rounded FP32 square root: sqrt in FP64 then round to FP32.  FP64
 // carries 53 >= 2*24 + 2 bits, so the double rounding is exact for sqrt.
 __device__ __forceinline__ float fph_sqrt_rn(float x) {

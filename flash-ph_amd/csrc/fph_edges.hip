@@ -26,6 +26,69 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
+// Selection of a thread's R x R block (rows i0 + ty*R + r, columns
+// j0 + tx*R + c): slot (r, c) is one compare whose ballot gives the wave's
+// kept lanes.  CHECKED adds the index tests of tiles that touch the
+// diagonal or the end of the cloud.
+template <int R, bool CHECKED>
+struct ScreenSel {
+    const float (&acc)[R][R];
+    int i0, j0, n, I, J;
+    float thr_sq;
+
+    __device__ __forceinline__ bool kept(int r, int c) const {
+        const bool in = acc[r][c] < thr_sq;
+        if constexpr (!CHECKED) {
+            return in;
+        } else {
+            const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+            const int i = i0 + ty * R + r, j = j0 + tx * R + c;
+            return in && i < n && j < n && (I < J || i < j);
+        }
+    }
+
+    // The wave's kept pairs: a sum of ballot counts (scalar).
+    __device__ __forceinline__ int count() const {
+        int total = 0;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int c = 0; c < R; c++) total += __popcll(__ballot(kept(r, c)));
+        return total;
+    }
+
+    // Writes the kept pairs from slot `pos` on, in (r, c, lane) order;
+    // slots whose ballot is zero cost one compare.
+    __device__ __forceinline__ void write(unsigned long long pos,
+                                          unsigned long long* __restrict__ keys,
+                                          unsigned long long* __restrict__ vals,
+                                          long long cap) const {
+        const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+        const unsigned long long lt = fph_lanemask_lt();
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int c = 0; c < R; c++) {
+                const bool ok = kept(r, c);
+                const unsigned long long b = __ballot(ok);
+                if (b == 0ull) continue;
+                if (ok) {
+                    const unsigned long long at = pos + __popcll(b & lt);
+                    if (at < (unsigned long long)cap) {
+                        const unsigned long long i = (unsigned long long)(i0 + ty * R + r);
+                        const unsigned long long j = (unsigned long long)(j0 + tx * R + c);
+                        unsigned bits;
+                        memcpy(&bits, &acc[r][c], 4);
+                        const unsigned long long colex = j * (j - 1) / 2 + i;
+                        keys[at] = ((unsigned long long)bits << 32) | (colex & 0xFFFFFFFFull);
+                        vals[at] = (j << 32) | i;
+                    }
+                }
+                pos += __popcll(b);
+            }
+    }
+};
+
 // One workgroup per upper-triangular TILE x TILE tile (TILE = 32, 64 or 128,
 // an autotuned launch parameter).  Each thread owns an R x R block of pairs
 // (rows ty*R+r, columns tx*R+c, R = TILE/16) and accumulates
@@ -35,7 +98,7 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 // FP32) are appended, unordered, as
 //     key = float_bits(acc) << 32 | (j(j-1)/2 + i) mod 2^32     (rips.py:41-49)
 //     val = j << 32 | i
-// through one atomic per wavefront.  The count keeps growing past `cap` so the
+// through one atomic per workgroup.  The count keeps growing past `cap` so the
 // caller can size a retry.
 template <int TILE>
 __global__ __launch_bounds__(kThreads) void k_edge_screen(
@@ -59,11 +122,14 @@ __global__ __launch_bounds__(kThreads) void k_edge_screen(
     for (int k0 = 0; k0 < d; k0 += kChunk) {
         const int kc = min(kChunk, d - k0);
         __syncthreads();
-        for (int idx = threadIdx.x; idx < TILE * kc; idx += kThreads) {
-            const int p = idx / kc, k = idx - p * kc;
-            const int gi = i0 + p, gj = j0 + p;
-            fa[k * TILE + p] = gi < n ? pts[(long long)gi * d + k0 + k] : 0.0f;
-            fb[k * TILE + p] = gj < n ? pts[(long long)gj * d + k0 + k] : 0.0f;
+        // one thread per (point, side): the point's kc coordinates go to
+        // column p of fa or fb (consecutive threads, consecutive columns)
+        for (int idx = threadIdx.x; idx < 2 * TILE; idx += kThreads) {
+            const int side = idx / TILE, p = idx - side * TILE;
+            const int g = (side ? j0 : i0) + p;
+            float* dst = (side ? fb : fa) + p;
+            const float* src = pts + (long long)g * d + k0;
+            for (int k = 0; k < kc; k++) dst[k * TILE] = g < n ? src[k] : 0.0f;
         }
         __syncthreads();
         for (int k = 0; k < kc; k++) {
@@ -82,48 +148,19 @@ __global__ __launch_bounds__(kThreads) void k_edge_screen(
         }
     }
 
-    // Selection and wave-level compaction.  Bit r*R+c of `keep` marks a kept
-    // pair (R*R <= 64); the wave's total is a lane sum, and the ballots are
-    // taken one at a time while writing, so none is kept live.
-    unsigned long long keep = 0;
-#pragma unroll
-    for (int r = 0; r < R; r++)
-#pragma unroll
-        for (int c = 0; c < R; c++) {
-            const int i = i0 + ty * R + r, j = j0 + tx * R + c;
-            const bool ok = (acc[r][c] < thr_sq) && i < n && j < n && (I < J || i < j);
-            keep |= (unsigned long long)ok << (r * R + c);
-        }
-    int total = __popcll(keep);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o);
-    if (total == 0) return;
-    unsigned long long base = 0;
-    if (fph_lane() == 0) base = atomicAdd(count, (unsigned long long)total);
-    base = __shfl(base, 0);
-    const unsigned long long lt = fph_lanemask_lt();
-    unsigned long long pos = base;
-#pragma unroll
-    for (int r = 0; r < R; r++)
-#pragma unroll
-        for (int c = 0; c < R; c++) {
-            const int ii = i0 + ty * R + r, jj = j0 + tx * R + c;
-            const bool ok = (keep >> (r * R + c)) & 1ull;
-            const unsigned long long b = __ballot(ok);
-            if (ok) {
-                const unsigned long long at = pos + __popcll(b & lt);
-                if (at < (unsigned long long)cap) {
-                    const unsigned long long i = (unsigned long long)ii;
-                    const unsigned long long j = (unsigned long long)jj;
-                    unsigned bits;
-                    memcpy(&bits, &acc[r][c], 4);
-                    const unsigned long long colex = j * (j - 1) / 2 + i;
-                    keys[at] = ((unsigned long long)bits << 32) | (colex & 0xFFFFFFFFull);
-                    vals[at] = (j << 32) | i;
-                }
-            }
-            pos += __popcll(b);
-        }
+    // Tiles strictly above the diagonal and inside the cloud need no index
+    // tests (I < J and j0 + TILE <= n give i < i0 + TILE <= j0 <= j < n).
+    if (I < J && j0 + TILE <= n) {
+        const ScreenSel<R, false> sel{acc, i0, j0, n, I, J, thr_sq};
+        const int total = sel.count();
+        const unsigned long long pos = fph_wg_reserve<kThreads>(total, count);
+        if (total) sel.write(pos, keys, vals, cap);
+    } else {
+        const ScreenSel<R, true> sel{acc, i0, j0, n, I, J, thr_sq};
+        const int total = sel.count();
+        const unsigned long long pos = fph_wg_reserve<kThreads>(total, count);
+        if (total) sel.write(pos, keys, vals, cap);
+    }
 }
 
 // Reference emission order of edge (i, j): tile (i/32, j/32) row-major over
