@@ -24,6 +24,10 @@
 // Value of autotune knob `knob` in effect for (size, d) (defined in fph_tune.hip).
 int fph_tune_value(int knob, int64_t size, int64_t d);
 
+// Sets, once per process, the LDS limit of the one-workgroup Boruvka form
+// (fph_mst.hip); call outside stream capture.
+hipError_t fph_mst_prepare();
+
 // Capacity forms of the edge sort + vertex CSR (fph_edges.hip) and of the H0
 // diagram (fph_mst.hip) for graph capture: sizes are the host capacity, the
 // live edge count is read on the device, and no memset node is recorded.

@@ -101,6 +101,7 @@ extern "C" int fph_h0_graph_create(const float* pts, int64_t n, int d, float thr
                                    void** graph_out) {
     if (!ws || !graph_out) return FPH_ERR_ARG;
     *graph_out = nullptr;
+    FPH_CHECK_HIP(fph_mst_prepare());  // not a stream call: before the capture
     // capture on a private stream (the legacy default stream cannot be
     // captured); the graph is launched later on the caller's stream
     hipStream_t cs;

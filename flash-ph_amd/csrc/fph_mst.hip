@@ -7,6 +7,8 @@
 // convergence.
 #include "fph_common.h"
 
+#include <mutex>
+
 #include <rocprim/rocprim.hpp>
 
 namespace {
@@ -109,6 +111,96 @@ __global__ void k_bvk_label(const int* __restrict__ comp, const unsigned* __rest
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v < n && out) out[v] = (long long)lab[comp[v]];
     if (v == 0 && converged) converged[0] = flags[0] ? 1 : (flags[rounds] ? 0 : 1);
+}
+
+// The whole round loop in one workgroup, its state in LDS (comp, nxt,
+// min_e: 12 bytes per vertex; gfx950 gives a workgroup up to 160 KB), for
+// clouds of up to kBvkLdsMaxN vertices.  Same steps as the kernels above --
+// find, hook, pointer jumping with path halving -- separated by barriers
+// instead of launches, and it stops at the first round without a crossing
+// edge instead of running every round: the forest (unique: distinct ranks),
+// the labels and the convergence flag are those of the multi-launch form.
+constexpr int kBvkLdsThreads = 1024;
+constexpr int kBvkLdsMaxN = 12288;          // 12 * n bytes <= 144 KB
+constexpr long long kBvkLdsMaxE = 1 << 19;  // beyond, the edge sweeps favour the grid
+
+__global__ __launch_bounds__(kBvkLdsThreads) void k_bvk_lds(
+    const int* __restrict__ ei, const int* __restrict__ ej, long long E, int n, int rounds,
+    unsigned char* __restrict__ mst, long long* __restrict__ comp_label,
+    int* __restrict__ converged) {
+    extern __shared__ int lds[];
+    int* comp = lds;
+    int* nxt = lds + n;
+    unsigned* min_e = reinterpret_cast<unsigned*>(lds + 2 * n);
+    __shared__ int s_cross;
+    const int tid = threadIdx.x;
+    for (int v = tid; v < n; v += kBvkLdsThreads) {
+        comp[v] = v;
+        nxt[v] = v;
+        min_e[v] = kNone;
+    }
+    for (long long e = tid; e < E; e += kBvkLdsThreads) mst[e] = 0;
+    int conv = 0;
+    for (int r = 0; r < rounds; r++) {
+        if (tid == 0) s_cross = 0;
+        __syncthreads();
+        // find: every crossing edge offers itself to both components
+        for (long long e = tid; e < E; e += kBvkLdsThreads) {
+            const int a = ei[e];
+            if (a < 0) continue;  // unused slot of a capacity-sized edge list
+            const int cu = comp[a], cv = comp[ej[e]];
+            if (cu == cv) continue;
+            atomicMin(&min_e[cu], (unsigned)e);
+            atomicMin(&min_e[cv], (unsigned)e);
+            s_cross = 1;
+        }
+        __syncthreads();
+        if (!s_cross) {
+            conv = 1;
+            break;
+        }
+        // hook: each root takes its lightest crossing edge; a mutual pair
+        // keeps the smaller root
+        for (int c = tid; c < n; c += kBvkLdsThreads) {
+            if (comp[c] != c) continue;
+            const unsigned e = min_e[c];
+            if (e == kNone) continue;
+            mst[e] = 1;
+            const int cu = comp[ei[e]], cv = comp[ej[e]];
+            const int other = (cu == c) ? cv : cu;
+            if (min_e[other] == e && other > c) continue;
+            nxt[c] = other;
+        }
+        __syncthreads();
+        // jump: every vertex walks its root's hook chain to the new root
+        for (int v = tid; v < n; v += kBvkLdsThreads) {
+            int x = comp[v];
+            for (int it = 0; it < n; it++) {
+                const int q = nxt[x];
+                if (q == x) break;
+                const int gq = nxt[q];
+                if (gq != q) nxt[x] = gq;
+                x = q;
+            }
+            comp[v] = x;
+        }
+        __syncthreads();
+        for (int v = tid; v < n; v += kBvkLdsThreads) {
+            nxt[v] = v;
+            min_e[v] = kNone;
+        }
+    }
+    if (comp_label) {
+        // label = smallest vertex of the component (nxt reused as labels)
+        unsigned* lab = reinterpret_cast<unsigned*>(nxt);
+        __syncthreads();
+        for (int v = tid; v < n; v += kBvkLdsThreads) lab[v] = kNone;
+        __syncthreads();
+        for (int v = tid; v < n; v += kBvkLdsThreads) atomicMin(&lab[comp[v]], (unsigned)v);
+        __syncthreads();
+        for (int v = tid; v < n; v += kBvkLdsThreads) comp_label[v] = (long long)lab[comp[v]];
+    }
+    if (tid == 0 && converged) converged[0] = conv;
 }
 
 __global__ void k_u8_to_i32(const unsigned char* __restrict__ m, long long E, int* __restrict__ o) {
@@ -229,6 +321,18 @@ int fph_h0_diagram_capacity(const uint8_t* mst_mask, const float* edge_dsq, int6
     return FPH_OK;
 }
 
+// Allows the one-workgroup form its LDS (once per process; called before
+// any stream capture by the graph builder, since it is not a stream call).
+hipError_t fph_mst_prepare() {
+    static std::once_flag once;
+    static hipError_t rc = hipSuccess;
+    std::call_once(once, [] {
+        rc = hipFuncSetAttribute((const void*)k_bvk_lds,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 12 * kBvkLdsMaxN);
+    });
+    return rc;
+}
+
 extern "C" int fph_boruvka_rounds(int64_t n) {
     int64_t m = n < 2 ? 2 : n;
     int r = 0;
@@ -250,6 +354,14 @@ extern "C" int fph_boruvka_mst(const int32_t* edge_i, const int32_t* edge_j, int
     if (n > 0x7FFFFFFFLL || E >= 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
     if (scratch_bytes < fph_boruvka_scratch_bytes(n)) return FPH_ERR_ARG;
     const int R = fph_boruvka_rounds(n);
+    if (n <= kBvkLdsMaxN && E <= kBvkLdsMaxE) {
+        const int lds_b = 12 * (int)n;
+        FPH_CHECK_HIP(fph_mst_prepare());
+        k_bvk_lds<<<1, kBvkLdsThreads, lds_b, s>>>(edge_i, edge_j, E, (int)n, R, mst_mask,
+                                                  (long long*)comp_label, converged);
+        FPH_LAUNCH_CHECK();
+        return FPH_OK;
+    }
     char* w = (char*)scratch;
     const size_t vb = align_up(4 * (size_t)n);
     BvkState st;

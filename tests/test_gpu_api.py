@@ -58,6 +58,26 @@ def test_rips_filtration_and_boruvka_api():
     assert np.array_equal(comp.cpu().numpy(), ref["comp"])
 
 
+@pytest.mark.parametrize("n", [12_288, 12_289])
+def test_boruvka_forms(n):
+    """Forests of up to 12,288 vertices come from the one-workgroup LDS form,
+    larger ones from the multi-launch form: both equal the oracle's (Kruskal
+    by rank) and its component labels."""
+    from flash_ph.rips import rips_filtration
+    from flash_ph.boruvka import gpu_boruvka_mst
+    import oracle
+    print(f"[gpu] boruvka form n={n}", flush=True)
+    pts = cases.randn(n, 3, 11)
+    f = rips_filtration(torch.from_numpy(pts).cuda(), 0.12)
+    ref = oracle.run(pts, 0.12, 0)
+    E = f.edge_i.shape[0]
+    mst, comp = gpu_boruvka_mst(f.edge_i, f.edge_j, torch.arange(E, device="cuda",
+                                                                 dtype=torch.int32), n)
+    assert 0 < len(ref["mst_idx"]) < n - 1  # several components
+    assert np.array_equal(mst.cpu().numpy(), ref["mst_idx"])
+    assert np.array_equal(comp.cpu().numpy(), ref["comp"])
+
+
 def test_complex_api():
     from flash_ph import RipsComplex
     print("[gpu] api RipsComplex", flush=True)
