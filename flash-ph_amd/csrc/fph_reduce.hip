@@ -1366,6 +1366,210 @@ __global__ void k_fill_u64(unsigned long long* __restrict__ p, long long n, unsi
 }
 
 // ---------------------------------------------------------------------
+// Replay of the reference's reduction from the final owners.
+//
+// _general_cohomology_reduce_v2 (cohomology_general.py:255-366) takes the
+// residual columns in descending order and adds to each the stored column
+// of its current pivot's owner until the pivot has none; a column is stored
+// (and its pivot gets an owner) only while the store has room, and working
+// columns stop at MAX_COL rows (_numba_utils.py:44-63).  While every column
+// fits, that is the standard reduction, whose pairs are those of any
+// complete reduction: the owner a column meets at pivot p is the column
+// whose final pivot is p, as fph_reduce leaves it in own[].  So each of the
+// reference's columns can be rebuilt on its own wavefront -- from its CSR
+// column, adding the final owner's column (the CSR column of an apparent
+// owner, else the column this replay published for it: owners come earlier
+// in the order, so their wavefronts are already running) until the pivot
+// is the column's own final pivot.  The replay reports what the reference's
+// store would hold and the longest working column: when both stay within
+// its limits, fph_reduce's pairs are the reference's.
+
+// Waits until desc[k] is published; kUnwritten on abort (qs[4]) or timeout
+// (status 5).
+__device__ unsigned replay_wait_desc(const RedArgs& a, int k, const int* qs, long long wait) {
+    unsigned src = __shfl(ld_fast(&a.desc[k]), 0);
+    const long long t0 = wall_clock64();
+    while (src == kUnwritten) {
+        if (__shfl(ld_coh(reinterpret_cast<const unsigned*>(qs + 4)), 0)) return kUnwritten;
+        if (wall_clock64() - t0 > wait) {
+            fail_wait(a);
+            return kUnwritten;
+        }
+        __builtin_amdgcn_s_sleep(2);
+        src = __shfl(ld_coh(&a.desc[k]), 0);
+    }
+    return src;
+}
+
+// Rebuilds the reference's reduced column c in the bitmap bm (zero on entry
+// and on return) and publishes it when non-zero.  Returns its length, or -1
+// (status set, or an abort seen).
+__device__ int replay_column(const RedArgs& a, int c, unsigned* bm, Alloc& al, const int* qs,
+                             int& adds, int& longest, long long wait) {
+    const int lo = a.off[c], n0 = a.off[c + 1] - lo;
+    if (n0 <= 0) return 0;  // no cofacets: essential, not stored
+    BmWork w;
+    w.bm = bm;
+    w.len = 0;
+    w.hi_w = -1;
+    w.piv = bm_flip_list(a, w, a.data + lo, n0);
+    w.lo_w = (int)((w.piv - (unsigned)a.row_base) >> 5);
+    longest = max(longest, n0);
+    const int target = a.pivot[c];
+    while (w.len > 0 && (int)w.piv != target) {
+        const unsigned p = w.piv;
+        const unsigned long long v = a.own[(long long)p - a.row_base];
+        const int k = (int)(v >> 32) - 1;
+        const int kk = (v == kNoOwner || k <= c || k >= a.K) ? 0 : (int)a.kind[k];
+        if (kk == 2) {  // apparent owner: its CSR column
+            const int lk = a.off[k];
+            bm_flip_list(a, w, a.data + lk, a.off[k + 1] - lk);
+        } else if (kk == 3) {
+            const unsigned src = replay_wait_desc(a, k, qs, wait);
+            if (src == kUnwritten) return -1;
+            unsigned vb;
+            const int nb = spec_read_dev(a.pool + src, vb, wait);
+            if (nb < 0) {
+                fail_wait(a);
+                return -1;
+            }
+            if (bm_flip_published(a, w, a.pool + src, vb, nb, wait) == 0xFFFFFFFFu) {
+                fail_wait(a);
+                return -1;
+            }
+        } else {  // no earlier owner of a pivot the column passes: inconsistent
+            if (fph_lane() == 0) atomicExch(&a.status[0], 3);
+            return -1;
+        }
+        adds++;
+        longest = max(longest, w.len);
+        if (w.len > 0) {
+            bm_next_pivot(a, w);
+            if (!(w.piv > p)) {
+                if (fph_lane() == 0) atomicExch(&a.status[0], 3);
+                return -1;
+            }
+        }
+    }
+    if (w.len == 0) {
+        if (target >= 0) {  // zero here but paired in the complete reduction
+            if (fph_lane() == 0) atomicExch(&a.status[0], 3);
+            return -1;
+        }
+        return 0;
+    }
+    const long long at = bm_publish(a, w, al, c);
+    if (at < 0) return -1;
+    if (fph_lane() == 0) st_coh(&a.desc[c], (unsigned)at);
+    const int len = w.len;
+    bm_clear(w);
+    return len;
+}
+
+// qs: [0] next column, [4] abort, [5] XCD taking part (-1: none yet).
+// status: [0] error, [1] ran, [2] additions, [3] longest working column,
+// [4..5] uint64 entries stored, [6] columns stored.
+__global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_replay(
+    RedArgs a, const unsigned* __restrict__ order, const int* __restrict__ res_count,
+    int* __restrict__ qs, long long spin_limit) {
+    extern __shared__ unsigned smem[];
+    __shared__ int s_part;
+    // the workgroups on one XCD take part (the columns they publish stay in
+    // its L2), as in the device-wide queue
+    if (threadIdx.x == 0) {
+        const int xcc = (int)(__builtin_amdgcn_s_getreg((20) | (0 << 6) | (3 << 11)) & 0xF);
+        const int lead = atomicCAS(qs + 5, -1, xcc);
+        s_part = lead == -1 || lead == xcc;
+    }
+    __syncthreads();
+    if (!s_part) return;
+    const int wid = threadIdx.x / FPH_WAVE;
+    const int lane = fph_lane();
+    unsigned* arena = smem + wid * kArenaB;
+    for (int i = lane; i < kArenaB; i += FPH_WAVE) arena[i] = 0u;
+    wave_sync();
+    const int R = *res_count;
+    Alloc al = make_alloc(512);
+    int adds = 0, longest = 0, nstored = 0;
+    unsigned long long stored = 0;
+    while (true) {
+        if (__shfl(ld_coh(reinterpret_cast<const unsigned*>(qs + 4)), 0)) break;
+        unsigned k = 0;
+        if (lane == 0) k = atomicAdd(reinterpret_cast<unsigned*>(qs), 1u);
+        k = __shfl(k, 0);
+        if (k >= (unsigned)R) break;
+        const int len = replay_column(a, (int)order[k], arena, al, qs, adds, longest, spin_limit);
+        if (len < 0) {
+            if (lane == 0) atomicExch(qs + 4, 1);
+            break;
+        }
+        if (len > 0) {
+            stored += (unsigned long long)len;
+            nstored++;
+        }
+    }
+    if (lane == 0) {
+        if (adds) atomicAdd(&a.status[2], adds);
+        if (longest) atomicMax(&a.status[3], longest);
+        if (stored) atomicAdd(reinterpret_cast<unsigned long long*>(a.status + 4), stored);
+        if (nstored) atomicAdd(&a.status[6], nstored);
+    }
+    if (threadIdx.x == 0) atomicMax(&a.status[1], 1);
+}
+
+// The apparent columns' share of the reference's store (it holds them from
+// the start, _numba_utils.py:68-104): their entries and count, and their
+// longest column.
+__global__ void k_replay_apparent(const unsigned char* __restrict__ kind,
+                                  const int* __restrict__ off, long long K,
+                                  int* __restrict__ status) {
+    __shared__ unsigned long long s_len;
+    __shared__ int s_cnt, s_max;
+    if (threadIdx.x == 0) {
+        s_len = 0;
+        s_cnt = 0;
+        s_max = 0;
+    }
+    __syncthreads();
+    unsigned long long len = 0;
+    int cnt = 0, mx = 0;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < K;
+         c += (long long)gridDim.x * blockDim.x) {
+        if (kind[c] == 2) {
+            const int n = off[c + 1] - off[c];
+            len += (unsigned long long)n;
+            cnt++;
+            mx = max(mx, n);
+        }
+    }
+    if (len) atomicAdd(&s_len, len);
+    if (cnt) atomicAdd(&s_cnt, cnt);
+    if (mx) atomicMax(&s_max, mx);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_len) atomicAdd(reinterpret_cast<unsigned long long*>(status + 4), s_len);
+        if (s_cnt) atomicAdd(&status[6], s_cnt);
+        if (s_max) atomicMax(&status[3], s_max);
+    }
+}
+
+__global__ void k_replay_init(int* __restrict__ status, unsigned long long* __restrict__ pool_ptr,
+                              int* __restrict__ qs) {
+    if (threadIdx.x < 8) status[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        *pool_ptr = 0;
+        qs[0] = 0;
+        qs[4] = 0;
+        qs[5] = -1;
+    }
+}
+
+__global__ void k_fill_u32(unsigned* __restrict__ p, long long n, unsigned v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------
 // The reference's own semantics, one wavefront per segment, in its order.
 //
 // _general_cohomology_reduce_v2 (cohomology_general.py:202-371) stores
@@ -2148,6 +2352,68 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
         FPH_LAUNCH_CHECK();
     }
     k_reduce_fini<<<1, 64, 0, s>>>(status, a.pool_ptr);
+    FPH_LAUNCH_CHECK();
+    return FPH_OK;
+}
+
+extern "C" int fph_reduce_replay(int64_t K, const int32_t* col_off, const uint32_t* col_data,
+                                 int64_t row_base, int64_t nrows, const uint8_t* kind,
+                                 const uint64_t* own, const int32_t* pivot,
+                                 const int32_t* res_list, const int32_t* res_count,
+                                 uint32_t* pool, int64_t pool_cap, void* scratch,
+                                 size_t scratch_bytes, int32_t* status, int wait_seconds,
+                                 void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (K < 0 || nrows < 0 || pool_cap < 0 || !scratch || !status || !pool || !res_count)
+        return FPH_ERR_ARG;
+    if (scratch_bytes < fph_reduce_scratch_bytes(K)) return FPH_ERR_ARG;
+    // working columns as bitmaps in the wavefront's arena only
+    if ((nrows + 31) / 32 > kArenaB) return FPH_ERR_LIMIT;
+    if (pool_cap > 0xFFFFFFFFLL - 64) return FPH_ERR_LIMIT;
+    const size_t k4 = align_up(4 * (size_t)std::max<int64_t>(K, 1));
+    char* w = (char*)scratch;
+    RedArgs a;
+    a.off = col_off;
+    a.data = col_data;
+    a.row_base = row_base;
+    a.nrows = nrows;
+    a.K = K;
+    a.kind = kind;
+    a.own = const_cast<unsigned long long*>(reinterpret_cast<const unsigned long long*>(own));
+    a.pivot = const_cast<int*>(pivot);
+    a.desc = (unsigned*)w;
+    unsigned* keys = (unsigned*)(w + 3 * k4);
+    unsigned* order = (unsigned*)(w + 4 * k4);
+    void* sort_tmp = w + 5 * k4;
+    size_t sort_sz = sort_bytes(K);
+    a.pool_ptr = (unsigned long long*)(w + 5 * k4 + sort_sz);
+    int* qs = (int*)(w + 5 * k4 + sort_sz + 64);
+    a.pool = pool;
+    a.pool_cap = (unsigned long long)pool_cap;
+    a.status = status;
+    a.use_lines = 0;
+    a.lines = nullptr;
+    k_replay_init<<<1, 64, 0, s>>>(status, a.pool_ptr, qs);
+    FPH_LAUNCH_CHECK();
+    if (K == 0) return FPH_OK;
+    k_fill_u32<<<fph_blocks(K, 256), 256, 0, s>>>(a.desc, K, kUnwritten);
+    k_fill_unwritten<<<1024, 256, 0, s>>>(pool, a.pool_ptr, pool_cap + 64);
+    k_replay_apparent<<<(unsigned)std::min<int64_t>(1024, fph_blocks(K, 256)), 256, 0, s>>>(
+        kind, col_off, K, status);
+    k_order_keys<<<fph_blocks(K, 256), 256, 0, s>>>(res_list, res_count, K, keys);
+    FPH_LAUNCH_CHECK();
+    FPH_CHECK_HIP(rocprim::radix_sort_keys_desc(sort_tmp, sort_sz, keys, order, (unsigned)K, 0,
+                                                32, s));
+    const size_t lds_b = kPhaseBWaves * kArenaB * 4;
+    FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_replay,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
+    int dev = 0, khz = 0;
+    FPH_CHECK_HIP(hipGetDevice(&dev));
+    FPH_CHECK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    const long long secs = wait_seconds > 0 ? (long long)wait_seconds : 120LL;
+    const long long limit_ticks = secs * (long long)std::max(khz, 1) * 1000LL;
+    // 8 XCDs receive workgroups in turn: 8 x 16 puts 16 on the XCD taking part
+    k_replay<<<8 * 16, kPhaseBWaves * FPH_WAVE, lds_b, s>>>(a, order, res_count, qs, limit_ticks);
     FPH_LAUNCH_CHECK();
     return FPH_OK;
 }

@@ -48,6 +48,8 @@ SIGNATURES = {
     "fph_reduce_scratch_bytes": (_SZ, [_I64]),
     "fph_reduce": (_I32, [_I64, _P, _P, _I64, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _P,
                           _SZ, _P, _I32, _P]),
+    "fph_reduce_replay": (_I32, [_I64, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _I64, _P,
+                                 _SZ, _P, _I32, _P]),
     "fph_reduce_serial": (_I32, [_I64, _P, _P, _I64, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P,
                                  _P, _P, _I64, _P, _SZ, _P, _I32, _P]),
     "fph_diagram": (_I32, [_I64, _P, _P, _P, _P, _P, _I64, _P, _P, _I64, _I32, _P, _P, _P, _I64,

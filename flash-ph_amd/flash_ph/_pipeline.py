@@ -29,6 +29,18 @@ _POOL_MIN_WORDS = 1 << 20
 _WAIT_SECONDS = 120
 # Entry cap of the reference's reduction store (_numba_utils.py:11).
 _REF_MAX_POOL = 100_000_000
+# Rows of a working column beyond which the reference truncates it
+# (_numba_utils.py:10).
+_REF_MAX_COL = 1_048_576
+# Reference semantics, one cloud: run the parallel reduction and replay the
+# reference's additions from its final owners (fph_reduce_replay); its pairs
+# are used when the reference's store would hold every column, else the
+# serial reducer runs.  Needs the rows to fit a wavefront's bitmap.
+REPLAY = True
+_REPLAY_MAX_ROWS = 2496 * 32
+# A replayed column waits only for earlier columns being rebuilt (at most
+# milliseconds); past this bound the replay gives up and the serial reducer runs.
+_REPLAY_WAIT_SECONDS = 10
 # Dynamic LDS words of the serial reducer (working-column bitmap + summary).
 _BITMAP_LDS_WORDS = 32768
 # The exact reduction of a segment with more rows than this runs in serial
@@ -311,19 +323,32 @@ def build_h1(cx: Complex, max_triangles: int = MAX_TRIANGLES, semantics: str = "
     shapes = ([(3 * T, E, T)] if segs is None else
               [(3 * t, e, t) for t, e in zip(tcs, ecs)])
     serial_exact = False
+    replay_cap = None
     if semantics == "reference":
         ref_sizes = shapes
+        if REPLAY and segs is None and T <= _REPLAY_MAX_ROWS:
+            # the parallel reduction, kept when the replay shows the
+            # reference's store holds every column (then its pairs are these)
+            replay_cap = reference_pool_caps([(3 * T, E)])[0]
+            ref_sizes = None
     elif semantics == "exact":
         # dense complexes (long columns): the serial complete reduction
         serial_exact = max(r for _, _, r in shapes) > EXACT_SERIAL_MIN_ROWS
         ref_sizes = shapes if serial_exact else None
     else:
         raise ValueError(f"unknown reduction semantics {semantics!r}")
-    res = _reduce_and_diagram(
+    args = dict(
         K=E, col_base=0, off=cx.cof_off, data=cx.cof_data, skip=cx.mst_mask, row_cmf=cx.tri_cmf,
         row_base=E, nrows=T, C=C, col_edge=None, row_edge=cx.tri_e, edge_filt=cx.edge_filt,
-        E=E, ess_ascending=(T == 0), device=dev, segs=segs, ref_sizes=ref_sizes,
-        serial_exact=serial_exact, want_edges=cx.want_edges, defer=defer and segs is None)
+        E=E, ess_ascending=(T == 0), device=dev, segs=segs, want_edges=cx.want_edges)
+    if replay_cap is not None:
+        res = _reduce_and_diagram(**args, replay_cap=replay_cap)
+        cx.info["h1_replay"] = res[3].get("replay")
+        if not res[3]["replay_ok"]:   # the reference's store overflows: its own reducer
+            res = _reduce_and_diagram(**args, ref_sizes=shapes)
+    else:
+        res = _reduce_and_diagram(**args, ref_sizes=ref_sizes, serial_exact=serial_exact,
+                                  defer=defer and segs is None)
     if defer and segs is None:
         cx.kind1, cx.pivot1, info32, fin = res
         cx.info["h1_pending"] = (info32, fin)
@@ -352,7 +377,8 @@ def _round32(x: int) -> int:
 
 def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, C, col_edge,
                         row_edge, edge_filt, E, ess_ascending, device, segs=None,
-                        ref_sizes=None, serial_exact=False, want_edges=False, defer=False):
+                        ref_sizes=None, serial_exact=False, want_edges=False, defer=False,
+                        replay_cap=None):
     """Apparent pairs, residual reduction and the diagram of one dimension.
 
     ``segs`` (batches) = dict(cols=, edges=, ess_asc= device tensors,
@@ -361,6 +387,9 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
     (reference cofacet entries, reference columns, rows) to run the serial
     reduction (fph_reduce_serial): the reference's capacity-limited one, or
     with ``serial_exact`` the complete one in serial order.
+    ``replay_cap`` (one segment, parallel reduction): the reference's store
+    capacity; the reduction is then followed by fph_reduce_replay, and
+    stats["replay_ok"] says whether its pairs are the reference's.
     Returns (kind, pivot, [diagram per segment], stats)."""
     lib = load()
     s = stream_of(device)
@@ -372,8 +401,10 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
     own = _empty(torch.int64, device, max(nrows, 1))
     pivot = _empty(torch.int32, device, max(K, 1))
     res_list = _empty(torch.int32, device, max(K, 1))
-    # info32: [0..8) reduction status, [8] residual count, [12..12+nseg) rows
-    info32 = torch.zeros(12 + nseg, dtype=torch.int32, device=device)
+    # info32: [0..8) reduction status, [8] residual count, [12..12+nseg) rows,
+    # [16..24) replay status
+    info32 = torch.zeros(max(12 + nseg, 24 if replay_cap is not None else 0),
+                         dtype=torch.int32, device=device)
     status_p = info32.data_ptr()
     rescnt_p = info32.data_ptr() + 32
     counts_p = info32.data_ptr() + 48
@@ -430,6 +461,17 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
                                  ptr(kind), ptr(own), ptr(pivot), ptr(res_list), rescnt_p,
                                  ptr(pool), pool_words, ptr(scratch), scratch_bytes, status_p,
                                  _WAIT_SECONDS, s), "fph_reduce")
+            if replay_cap is not None:
+                # rebuilt columns: at most the store capacity (else it
+                # overflows anyway), each rounded to 32-word lines, plus the
+                # wavefronts' partly used allocation chunks
+                words = replay_cap + 32 * K + (1 << 18)
+                pool2 = _empty(torch.int32, device, words + 64)
+                check(lib.fph_reduce_replay(K, ptr(off), ptr(data), row_base, nrows, ptr(kind),
+                                            ptr(own), ptr(pivot), ptr(res_list), rescnt_p,
+                                            ptr(pool2), words, ptr(scratch), scratch_bytes,
+                                            info32.data_ptr() + 64, _REPLAY_WAIT_SECONDS, s),
+                      "fph_reduce_replay")
         else:
             if pool_words > 0xFFFFFFFF:
                 raise _lib.FlashPHError("reduction store exceeds 2^32 words")
@@ -466,6 +508,13 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
         stats = {"xors": st[2], "longest": st[3], "offchip_xors": st[4],
                  "pool_words": 32 * st[5], "pool_cap": state["pool_words"], "residual": st[8],
                  "merged": merged}
+        if replay_cap is not None:
+            r = st[16:24]
+            held = (r[4] & 0xFFFFFFFF) | ((r[5] & 0xFFFFFFFF) << 32)
+            stats["replay"] = {"status": r[0], "xors": r[2], "longest": r[3], "held": held,
+                               "held_columns": r[6], "cap": replay_cap}
+            stats["replay_ok"] = (r[0] == 0 and (r[1] == 1 or K == 0) and held <= replay_cap
+                                  and r[3] <= _REF_MAX_COL and r[6] <= K)
         counts = st[12:12 + nseg]
         starts = [0] if segs is None else segs["cols_host"][:-1]
         dgms = [out[a:a + c] for a, c in zip(starts, counts)]

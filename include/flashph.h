@@ -207,6 +207,33 @@ int fph_reduce(int64_t K, const int32_t *col_off, const uint32_t *col_data, int6
                size_t scratch_bytes, int32_t *status, int wait_seconds, void *stream);
 
 /*
+ * Replay of the reference's own reduction after fph_reduce (one segment):
+ * every residual column (res_list, descending index) is rebuilt by adding,
+ * at each pivot, the column of its final owner from fph_reduce (own[],
+ * pivot[]) -- the CSR column of an apparent owner, else the owner's rebuilt
+ * column -- until it reaches its final pivot.  While every reduced column
+ * fits the reference's store and no working column passes MAX_COL, these
+ * are exactly the reference's additions (standard reduction, descending
+ * order), so the reported totals decide whether fph_reduce's pairs are the
+ * reference's.  Needs ceil(nrows / 32) <= 2496 (FPH_ERR_LIMIT otherwise).
+ * pool: pool_cap words plus 64 of slack, scratch: fph_reduce_scratch_bytes.
+ * status (device int32[8], 8-byte aligned): [0] = 0 ok, 1 pool too small
+ * (the rebuilt columns need more than pool_cap words), 3 inconsistent
+ * owners, 5 no progress for wait_seconds (<= 0: 120 s); [1] = 1 once run;
+ * [2] = column additions; [3] = longest working column (initial columns
+ * included); [4..5] = uint64 entries the reference's store holds (apparent
+ * columns plus non-zero reduced columns); [6] = columns it holds.
+ * Follows _general_cohomology_reduce_v2 (cohomology_general.py:255-366),
+ * the store test (reduce_h1_gpu.py:283-289, cohomology_general.py:337-345)
+ * and the MAX_COL cut (_numba_utils.py:10, 44-63).
+ */
+int fph_reduce_replay(int64_t K, const int32_t *col_off, const uint32_t *col_data,
+                      int64_t row_base, int64_t nrows, const uint8_t *kind, const uint64_t *own,
+                      const int32_t *pivot, const int32_t *res_list, const int32_t *res_count,
+                      uint32_t *pool, int64_t pool_cap, void *scratch, size_t scratch_bytes,
+                      int32_t *status, int wait_seconds, void *stream);
+
+/*
  * The reference's own reduction semantics, for results identical to it in
  * every case: per segment, apparent columns enter a store of seg_pool[2s+1]
  * entries in ascending column order, then residual columns are reduced one

@@ -70,7 +70,7 @@ enum {
 
 /* stats slots */
 enum { S_E, S_T, S_Q, S_BORUVKA_ROUNDS, S_RES1, S_RES2, S_POOL_DROPS, S_TRUNC,
-       S_XOR1, S_XOR2, S_NCOMP, S_NSTATS };
+       S_XOR1, S_XOR2, S_NCOMP, S_STORE1, S_STORED1, S_NSTATS };
 
 typedef struct {
     Arr a[A_COUNT];
@@ -324,6 +324,7 @@ typedef struct {
     int32_t *pb, *pd, *eb, *allpiv;
     int64_t npairs, ness, nallpiv;
     int64_t residual, pool_drops, truncations, xors;
+    int64_t store_entries, store_columns;  /* the store's final pool_ptr, num_piv */
 } ReduceOut;
 
 static int64_t xor_merge(const int32_t *a, int64_t na, const int32_t *b, int64_t nb,
@@ -484,6 +485,8 @@ static void reduce_columns(int64_t K, const int32_t *col_rank, const int32_t *of
         }
     }
     (void)maxrow;
+    ro->store_entries = pool_ptr;
+    ro->store_columns = num_piv;
     free(cur); free(tmp); free(carry); free(order); free(ext_skip);
     free(cmf); free(piv_of); free(ps); free(pl); free(pool);
 #undef ORC_STORE_FITS
@@ -730,6 +733,8 @@ OrcResult *orc_run(const float *pts, int64_t n, int d, float thr_sq, int max_dim
     arr_set(r, A_APPARENT1, app1, E, 1);
     r->stats[S_RES1] = ro1.residual;
     r->stats[S_XOR1] = ro1.xors;
+    r->stats[S_STORE1] = ro1.store_entries;
+    r->stats[S_STORED1] = ro1.store_columns;
     r->stats[S_POOL_DROPS] += ro1.pool_drops;
     r->stats[S_TRUNC] += ro1.truncations;
     {

@@ -33,7 +33,7 @@ NAMES = [
     "stats",
 ]
 STATS = ["E", "T", "Q", "boruvka_rounds", "residual1", "residual2",
-         "pool_drops", "truncations", "xor1", "xor2", "ncomp"]
+         "pool_drops", "truncations", "xor1", "xor2", "ncomp", "store1", "stored1"]
 
 _DTYPES = {
     "edge_dsq": np.float32, "adj_dsq": np.float32,

@@ -144,6 +144,58 @@ def test_store_overflow_reference_and_exact(idx):
     _eq("h1 exact", _cpu(got_ex[1]), ex["h1"])
 
 
+_REPLAY_CASES = {
+    "randn_n300_d4": lambda: (cases.randn(300, 4, 8), 1.3),
+    "clifford_n600": lambda: (cases.clifford(600, 3, 0.02), 0.6),
+    "randn_n10000_d10": lambda: (cases.randn(10_000, 10, 0), 1.6),
+}
+
+
+@pytest.mark.parametrize("name", list(_REPLAY_CASES))
+def test_replay_counts_the_reference_store(name):
+    """Reference semantics, one cloud whose reference store drops nothing:
+    the parallel reduction is kept, and its replay of the reference's
+    additions counts exactly the reference's additions, stored entries and
+    stored columns (oracle xor1, store1, stored1)."""
+    import oracle
+    from flash_ph import _pipeline
+    pts, t = _REPLAY_CASES[name]()
+    print(f"[gpu] replay {name}", flush=True)
+    ref = oracle.run(pts, t, 1)
+    assert ref["stats"]["pool_drops"] == 0 and ref["stats"]["truncations"] == 0
+    dg, cx = _pipeline.compute(torch.from_numpy(pts).cuda(), t, 1, semantics="reference")
+    assert cx.info["h1_stats"].get("replay_ok") is True
+    rp = cx.info["h1_replay"]
+    assert rp["status"] == 0
+    assert rp["xors"] == ref["stats"]["xor1"]
+    assert rp["held"] == ref["stats"]["store1"]
+    assert rp["held_columns"] == ref["stats"]["stored1"]
+    _eq("h0", _cpu(dg[0]), ref["h0"])
+    _eq("h1", _cpu(dg[1]), ref["h1"])
+
+
+def test_replay_detects_store_overflow():
+    """Where the reference's store drops columns, the replay rejects the
+    parallel pairs and the reference's own reducer runs (bit-exact); a cloud
+    whose rows exceed a wavefront's bitmap goes to that reducer directly."""
+    import oracle
+    from flash_ph import _pipeline
+    for idx in (0, 1):
+        x = _store_overflow_clouds()[idx]
+        t = oracle.auto_threshold(x)
+        print(f"[gpu] replay overflow case {idx} n={x.shape[0]} d={x.shape[1]}", flush=True)
+        ref = oracle.run(x, t, 1)
+        assert ref["stats"]["pool_drops"] > 0
+        dg, cx = _pipeline.compute(torch.from_numpy(x).cuda(), t, 1, semantics="reference")
+        assert "replay_ok" not in cx.info["h1_stats"]  # the serial reducer's stats
+        if ref["stats"]["T"] <= _pipeline._REPLAY_MAX_ROWS:
+            rp = cx.info["h1_replay"]
+            assert rp["status"] == 1 or rp["held"] > rp["cap"]
+        else:
+            assert "h1_replay" not in cx.info
+        _eq("h1", _cpu(dg[1]), ref["h1"])
+
+
 @pytest.mark.parametrize("variant", [0, 1])
 def test_serial_reducer_variants(variant):
     """Both forms of the working column (bitmap, sorted list) of the
@@ -152,8 +204,9 @@ def test_serial_reducer_variants(variant):
     import oracle
     from flash_ph import _pipeline
     print(f"[gpu] serial variant {variant}", flush=True)
-    old = _pipeline.SERIAL_VARIANT
+    old = _pipeline.SERIAL_VARIANT, _pipeline.REPLAY
     _pipeline.SERIAL_VARIANT = variant
+    _pipeline.REPLAY = False  # every cloud through the serial reducer
     try:
         for x, t in [(_store_overflow_clouds()[0], None), (cases.randn(400, 3, 2), 1.0),
                      (cases.clifford(600, 3, 0.02), 0.6)]:
@@ -162,7 +215,7 @@ def test_serial_reducer_variants(variant):
             dg, cx = _pipeline.compute(torch.from_numpy(x).cuda(), t, 1, semantics="reference")
             _eq("h1", _cpu(dg[1]), ref["h1"])
     finally:
-        _pipeline.SERIAL_VARIANT = old
+        _pipeline.SERIAL_VARIANT, _pipeline.REPLAY = old
 
 
 def test_exact_serial_and_parallel_agree():
