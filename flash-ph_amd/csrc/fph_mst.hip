@@ -115,15 +115,46 @@ __global__ void k_bvk_label(const int* __restrict__ comp, const unsigned* __rest
 
 // The whole round loop in one workgroup, its state in LDS (comp, nxt,
 // min_e: 12 bytes per vertex; gfx950 gives a workgroup up to 160 KB), for
-// clouds of up to kBvkLdsMaxN vertices.  Same steps as the kernels above --
+// clouds of up to kBvkLdsMaxN vertices.  Same rounds as the kernels above --
 // find, hook, pointer jumping with path halving -- separated by barriers
 // instead of launches, and it stops at the first round without a crossing
 // edge instead of running every round: the forest (unique: distinct ranks),
 // the labels and the convergence flag are those of the multi-launch form.
+// Hooking goes from the edge side: an edge that is the lightest crossing
+// edge of one of its two components hooks that component to the other (of a
+// mutual pair only the larger root), which is the root-side rule above
+// without reading the chosen edge's endpoints back from memory.  With
+// CACHED, each thread keeps its (at most kBvkEdgesPerThread) edges in
+// registers for all rounds, one word each (a << 16 | b: n < 2^16).
 constexpr int kBvkLdsThreads = 1024;
 constexpr int kBvkLdsMaxN = 12288;          // 12 * n bytes <= 144 KB
 constexpr long long kBvkLdsMaxE = 1 << 19;  // beyond, the edge sweeps favour the grid
+constexpr int kBvkEdgesPerThread = 32;      // register-resident edges (CACHED)
 
+// One edge (a, b) of rank e during find: offers e to both components.
+__device__ __forceinline__ void bvk_offer(const int* comp, unsigned* min_e, int a, int b,
+                                          unsigned e, int& cross) {
+    if (a < 0) return;  // unused slot of a capacity-sized edge list
+    const int cu = comp[a], cv = comp[b];
+    if (cu == cv) return;
+    atomicMin(&min_e[cu], e);
+    atomicMin(&min_e[cv], e);
+    cross = 1;
+}
+
+// One edge during hook: hooks a component that chose it (the forest edges
+// themselves are marked from the roots' side, mst[min_e[root]]).
+__device__ __forceinline__ void bvk_hook_edge(const int* comp, int* nxt, const unsigned* min_e,
+                                              int a, int b, unsigned e) {
+    if (a < 0) return;
+    const int cu = comp[a], cv = comp[b];
+    if (cu == cv) return;
+    const bool mu = min_e[cu] == e, mv = min_e[cv] == e;
+    if (mu && (!mv || cu > cv)) nxt[cu] = cv;
+    if (mv && (!mu || cv > cu)) nxt[cv] = cu;
+}
+
+template <bool CACHED>
 __global__ __launch_bounds__(kBvkLdsThreads) void k_bvk_lds(
     const int* __restrict__ ei, const int* __restrict__ ej, long long E, int n, int rounds,
     unsigned char* __restrict__ mst, long long* __restrict__ comp_label,
@@ -134,6 +165,15 @@ __global__ __launch_bounds__(kBvkLdsThreads) void k_bvk_lds(
     unsigned* min_e = reinterpret_cast<unsigned*>(lds + 2 * n);
     __shared__ int s_cross;
     const int tid = threadIdx.x;
+    unsigned ce[CACHED ? kBvkEdgesPerThread : 1];
+    if constexpr (CACHED) {
+#pragma unroll
+        for (int u = 0; u < kBvkEdgesPerThread; u++) {
+            const long long e = tid + (long long)u * kBvkLdsThreads;
+            const int a = e < E ? ei[e] : -1;
+            ce[u] = a < 0 ? 0xFFFFFFFFu : ((unsigned)a << 16 | (unsigned)ej[e]);
+        }
+    }
     for (int v = tid; v < n; v += kBvkLdsThreads) {
         comp[v] = v;
         nxt[v] = v;
@@ -145,32 +185,36 @@ __global__ __launch_bounds__(kBvkLdsThreads) void k_bvk_lds(
         if (tid == 0) s_cross = 0;
         __syncthreads();
         // find: every crossing edge offers itself to both components
-        for (long long e = tid; e < E; e += kBvkLdsThreads) {
-            const int a = ei[e];
-            if (a < 0) continue;  // unused slot of a capacity-sized edge list
-            const int cu = comp[a], cv = comp[ej[e]];
-            if (cu == cv) continue;
-            atomicMin(&min_e[cu], (unsigned)e);
-            atomicMin(&min_e[cv], (unsigned)e);
-            s_cross = 1;
+        int cross = 0;
+        if constexpr (CACHED) {
+#pragma unroll
+            for (int u = 0; u < kBvkEdgesPerThread; u++)
+                if (ce[u] != 0xFFFFFFFFu)
+                    bvk_offer(comp, min_e, (int)(ce[u] >> 16), (int)(ce[u] & 0xFFFFu),
+                              (unsigned)(tid + u * kBvkLdsThreads), cross);
+        } else {
+            for (long long e = tid; e < E; e += kBvkLdsThreads)
+                bvk_offer(comp, min_e, ei[e], ej[e], (unsigned)e, cross);
         }
+        if (cross) s_cross = 1;
         __syncthreads();
         if (!s_cross) {
             conv = 1;
             break;
         }
-        // hook: each root takes its lightest crossing edge; a mutual pair
-        // keeps the smaller root
-        for (int c = tid; c < n; c += kBvkLdsThreads) {
-            if (comp[c] != c) continue;
-            const unsigned e = min_e[c];
-            if (e == kNone) continue;
-            mst[e] = 1;
-            const int cu = comp[ei[e]], cv = comp[ej[e]];
-            const int other = (cu == c) ? cv : cu;
-            if (min_e[other] == e && other > c) continue;
-            nxt[c] = other;
+        // hook, from the edge side
+        if constexpr (CACHED) {
+#pragma unroll
+            for (int u = 0; u < kBvkEdgesPerThread; u++)
+                if (ce[u] != 0xFFFFFFFFu)
+                    bvk_hook_edge(comp, nxt, min_e, (int)(ce[u] >> 16), (int)(ce[u] & 0xFFFFu),
+                                  (unsigned)(tid + u * kBvkLdsThreads));
+        } else {
+            for (long long e = tid; e < E; e += kBvkLdsThreads)
+                bvk_hook_edge(comp, nxt, min_e, ei[e], ej[e], (unsigned)e);
         }
+        for (int c = tid; c < n; c += kBvkLdsThreads)
+            if (comp[c] == c && min_e[c] != kNone) mst[min_e[c]] = 1;
         __syncthreads();
         // jump: every vertex walks its root's hook chain to the new root
         for (int v = tid; v < n; v += kBvkLdsThreads) {
@@ -327,8 +371,12 @@ hipError_t fph_mst_prepare() {
     static std::once_flag once;
     static hipError_t rc = hipSuccess;
     std::call_once(once, [] {
-        rc = hipFuncSetAttribute((const void*)k_bvk_lds,
+        rc = hipFuncSetAttribute((const void*)k_bvk_lds<true>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 12 * kBvkLdsMaxN);
+        if (rc == hipSuccess)
+            rc = hipFuncSetAttribute((const void*)k_bvk_lds<false>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     12 * kBvkLdsMaxN);
     });
     return rc;
 }
@@ -357,8 +405,14 @@ extern "C" int fph_boruvka_mst(const int32_t* edge_i, const int32_t* edge_j, int
     if (n <= kBvkLdsMaxN && E <= kBvkLdsMaxE) {
         const int lds_b = 12 * (int)n;
         FPH_CHECK_HIP(fph_mst_prepare());
-        k_bvk_lds<<<1, kBvkLdsThreads, lds_b, s>>>(edge_i, edge_j, E, (int)n, R, mst_mask,
-                                                  (long long*)comp_label, converged);
+        if (E <= (long long)kBvkEdgesPerThread * kBvkLdsThreads)
+            k_bvk_lds<true><<<1, kBvkLdsThreads, lds_b, s>>>(edge_i, edge_j, E, (int)n, R,
+                                                            mst_mask, (long long*)comp_label,
+                                                            converged);
+        else
+            k_bvk_lds<false><<<1, kBvkLdsThreads, lds_b, s>>>(edge_i, edge_j, E, (int)n, R,
+                                                             mst_mask, (long long*)comp_label,
+                                                             converged);
         FPH_LAUNCH_CHECK();
         return FPH_OK;
     }
