@@ -36,7 +36,7 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 # the reference's published wall time for this workload (A100-80GB, README.md:102)
-BASELINE_MS = 49.0
+REF_A100_MS = 49.0
 
 
 def make_points():
@@ -144,7 +144,7 @@ def traffic_from_profile(kernel: str):
     return int(round(1024.0 * (fetch * fcal + write * wcal)))
 
 
-def bench_cfg4() -> dict:
+def bench_precision_n200k() -> dict:
     """n=200,000, d=50 N(0,1), t=6.0, H0+H1: the split-bf16 MFMA screen
     beside the direct FP32 screen (identical edges), the margin band, the
     FP64 error of the FP32 distances, and the whole H0+H1 call."""
@@ -227,7 +227,7 @@ def bench_graph_h0(pts, reps: int = 50) -> dict:
     return out
 
 
-def bench_cfg5() -> dict:
+def bench_autograd_clifford() -> dict:
     """64 Clifford-torus clouds (n=2000, d=4, noise 0.02, t=0.6), loss =
     total finite H1 persistence: forward + hand-written backward, and the
     gradient against central differences (h=1e-3, FP64) of the same loss on
@@ -376,7 +376,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         # published: 49 ms on an A100-80GB (reference README.md:102), lower is better
-        "vs_baseline": round(ms / BASELINE_MS, 4),
+        "vs_baseline": round(ms / REF_A100_MS, 4),
         "dtype": "f32",
         "data": "synthetic: numpy default_rng(0) N(0,1) float32",
         "config": {"workload": f"plain, n={N}, d={D}, t={T_EDGE} (strict <), H0+H1+H2",
@@ -412,8 +412,8 @@ def main():
     if not args.no_batch:
         out["batch"] = bench_batch(args.batch_steps)
     if not args.no_extra:
-        out["precision_n200k_d50"] = bench_cfg4()
-        out["autograd_clifford_64x2000"] = bench_cfg5()
+        out["precision_n200k_d50"] = bench_precision_n200k()
+        out["autograd_clifford_64x2000"] = bench_autograd_clifford()
         out["graph_h0"] = bench_graph_h0(pts)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pts_np)

@@ -80,7 +80,7 @@ MEDIUM = [
     ("o3_1024", lambda: o3(1024, 42), 1.8, 1),
     ("randn_d10_n2000", lambda: randn(2000, 10, 1), 2.2, 2),
     ("clifford_1000_noisy", lambda: clifford(1000, 3, 0.02), 0.45, 2),
-    ("randn_d10_n10000_cfg2", lambda: randn(10000, 10, 0), 1.6, 2),
+    ("randn_d10_n10000_t16", lambda: randn(10000, 10, 0), 1.6, 2),
 ]
 
 

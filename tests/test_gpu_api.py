@@ -115,11 +115,11 @@ def test_deterministic_bits():
         assert np.array_equal(_u32(x), _u32(y))
 
 
-def test_cfg2_invariants():
+def test_n10000_d10_invariants():
     """n=10,000, d=10, t=1.6, H0-H2: every vertex accounted for in H0, every
     column either cleared, paired or essential, diagrams sorted by birth."""
     from flash_ph import _pipeline
-    print("[gpu] cfg2 invariants", flush=True)
+    print("[gpu] n=10000 d=10 invariants", flush=True)
     pts = torch.from_numpy(cases.randn(10_000, 10, 0)).cuda()
     dg, cx = _pipeline.compute(pts, 1.6, 2)
     h0, h1, h2 = (d.cpu().numpy() for d in dg)

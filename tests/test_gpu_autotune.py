@@ -31,7 +31,7 @@ def _eq(name, got, want):
 
 
 # ragged tile edges (n not a multiple of 32/64/128), d beyond one LDS chunk,
-# a dense case and a cfg2-sized one
+# a dense case and an n=10,000, d=10 one
 SCREEN_CASES = [
     ("randn_n777_d3", lambda: cases.randn(777, 3, 5), 0.4),
     ("randn_n300_d50", lambda: cases.randn(300, 50, 6), 9.5),
@@ -80,7 +80,7 @@ QUEUE_CASES = [
     ("randn_d10_n2000", lambda: cases.randn(2000, 10, 1), 2.2, 2, "reference"),
     ("clifford_1000_noisy", lambda: cases.clifford(1000, 3, 0.02), 0.45, 2, "reference"),
     ("o3_1024_exact", lambda: cases.o3(1024, 42), 1.8, 1, "exact"),
-    ("randn_d10_n10000_cfg2", lambda: cases.randn(10000, 10, 0), 1.6, 2, "reference"),
+    ("randn_d10_n10000_t16", lambda: cases.randn(10000, 10, 0), 1.6, 2, "reference"),
 ]
 
 
