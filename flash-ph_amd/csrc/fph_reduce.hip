@@ -2172,6 +2172,37 @@ __global__ void k_order_keys(const int* __restrict__ res_list, const int* __rest
     if (r < K) keys[r] = r < *res_count ? (unsigned)res_list[r] : 0u;
 }
 
+// order[0..R) = the residual columns (kind 3) in descending index, by one
+// workgroup: thread t counts its run of columns (t = 0 the highest run), a
+// workgroup scan gives each run its place, and the run is written top down.
+// Replaces a descending radix sort of the residual list (several launches
+// of block sort and merge passes at these sizes).
+constexpr int kOrdThreads = 1024;
+constexpr long long kOrderCompactMax = 1LL << 21;  // beyond: the radix sort
+
+__global__ __launch_bounds__(kOrdThreads) void k_order_desc(const unsigned char* __restrict__ kind,
+                                                            long long K,
+                                                            unsigned* __restrict__ order) {
+    __shared__ int s_wave[kOrdThreads / FPH_WAVE];
+    const int t = threadIdx.x, lane = fph_lane(), wid = t / FPH_WAVE;
+    const long long chunk = (K + kOrdThreads - 1) / kOrdThreads;
+    const long long hi = K - (long long)t * chunk;
+    const long long lo = hi - chunk > 0 ? hi - chunk : 0;
+    int cnt = 0;
+    for (long long c = lo; c < hi; c++) cnt += kind[c] == 3;
+    int incl = cnt;
+    for (int o = 1; o < FPH_WAVE; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    if (lane == FPH_WAVE - 1) s_wave[wid] = incl;
+    __syncthreads();
+    int pos = incl - cnt;
+    for (int w = 0; w < wid; w++) pos += s_wave[w];
+    for (long long c = hi - 1; c >= lo; c--)
+        if (kind[c] == 3) order[pos++] = (unsigned)c;
+}
+
 inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
 size_t sort_bytes(int64_t K) {
@@ -2182,6 +2213,22 @@ size_t sort_bytes(int64_t K) {
 }
 
 }  // namespace
+
+// order[0..res_count) = the residual columns in descending index.
+int launch_order(const unsigned char* kind, const int* res_list, const int* res_count, int64_t K,
+                 unsigned* keys, unsigned* order, void* sort_tmp, size_t sort_sz,
+                 hipStream_t s) {
+    if (K <= kOrderCompactMax) {
+        k_order_desc<<<1, kOrdThreads, 0, s>>>(kind, K, order);
+        FPH_LAUNCH_CHECK();
+        return FPH_OK;
+    }
+    k_order_keys<<<fph_blocks(K, 256), 256, 0, s>>>(res_list, res_count, K, keys);
+    FPH_LAUNCH_CHECK();
+    FPH_CHECK_HIP(rocprim::radix_sort_keys_desc(sort_tmp, sort_sz, keys, order, (unsigned)K, 0,
+                                                32, s));
+    return FPH_OK;
+}
 
 extern "C" size_t fph_reduce_scratch_bytes(int64_t K) {
     const size_t k4 = align_up(4 * (size_t)std::max<int64_t>(K, 1));
@@ -2235,10 +2282,9 @@ extern "C" int fph_reduce_serial(int64_t K, const int32_t* col_off, const uint32
         const unsigned grid_w =
             (unsigned)std::min<int64_t>(1024, (K + kPhaseAWaves - 1) / kPhaseAWaves);
         k_publish_apparent<<<grid_w, kPhaseAWaves * FPH_WAVE, 0, s>>>(a, K, 0);
-        k_order_keys<<<fph_blocks(K, 256), 256, 0, s>>>(res_list, res_count, K, keys);
         FPH_LAUNCH_CHECK();
-        FPH_CHECK_HIP(rocprim::radix_sort_keys_desc(sort_tmp, sort_sz, keys, order, (unsigned)K,
-                                                    0, 32, s));
+        const int orc = launch_order(kind, res_list, res_count, K, keys, order, sort_tmp, sort_sz, s);
+        if (orc) return orc;
         FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_reduce_serial,
                                           hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)kSerLds));
@@ -2311,10 +2357,9 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
     const size_t lds_a = kPhaseAWaves * kArenaA * 4;
     if (!from_csr)
         k_reduce_chain<<<grid_w, kPhaseAWaves * FPH_WAVE, lds_a, s>>>(a, res_list, res_count);
-    k_order_keys<<<fph_blocks(K, 256), 256, 0, s>>>(res_list, res_count, K, keys);
     FPH_LAUNCH_CHECK();
-    FPH_CHECK_HIP(rocprim::radix_sort_keys_desc(sort_tmp, sort_sz, keys, order, (unsigned)K, 0,
-                                                32, s));
+    const int orc = launch_order(kind, res_list, res_count, K, keys, order, sort_tmp, sort_sz, s);
+    if (orc) return orc;
     if (!from_csr) {
         k_fill_u64<<<fph_blocks(K, 256), 256, 0, s>>>(ring, K, ~0ull);
         FPH_LAUNCH_CHECK();
@@ -2400,10 +2445,9 @@ extern "C" int fph_reduce_replay(int64_t K, const int32_t* col_off, const uint32
     k_fill_unwritten<<<1024, 256, 0, s>>>(pool, a.pool_ptr, pool_cap + 64);
     k_replay_apparent<<<(unsigned)std::min<int64_t>(1024, fph_blocks(K, 256)), 256, 0, s>>>(
         kind, col_off, K, status);
-    k_order_keys<<<fph_blocks(K, 256), 256, 0, s>>>(res_list, res_count, K, keys);
     FPH_LAUNCH_CHECK();
-    FPH_CHECK_HIP(rocprim::radix_sort_keys_desc(sort_tmp, sort_sz, keys, order, (unsigned)K, 0,
-                                                32, s));
+    const int orc = launch_order(kind, res_list, res_count, K, keys, order, sort_tmp, sort_sz, s);
+    if (orc) return orc;
     const size_t lds_b = kPhaseBWaves * kArenaB * 4;
     FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_replay,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
