@@ -41,6 +41,15 @@ _REPLAY_MAX_ROWS = 2496 * 32
 # A replayed column waits only for earlier columns being rebuilt (at most
 # milliseconds); past this bound the replay gives up and the serial reducer runs.
 _REPLAY_WAIT_SECONDS = 10
+# Largest replay pool (32-bit words); bigger complexes use the serial reducer.
+_REPLAY_MAX_WORDS = 1 << 28
+
+
+def _replay_words(cap: int, K: int) -> int:
+    """Replay pool words: rebuilt columns up to the store capacity (beyond
+    it the store overflows anyway), each rounded to 32-word lines, plus the
+    wavefronts' partly used allocation chunks."""
+    return cap + 32 * K + (1 << 18)
 # Dynamic LDS words of the serial reducer (working-column bitmap + summary).
 _BITMAP_LDS_WORDS = 32768
 # The exact reduction of a segment with more rows than this runs in serial
@@ -326,10 +335,12 @@ def build_h1(cx: Complex, max_triangles: int = MAX_TRIANGLES, semantics: str = "
     replay_cap = None
     if semantics == "reference":
         ref_sizes = shapes
-        if REPLAY and segs is None and T <= _REPLAY_MAX_ROWS:
+        cap = reference_pool_caps([(3 * T, E)])[0]
+        if (REPLAY and segs is None and T <= _REPLAY_MAX_ROWS
+                and _replay_words(cap, E) <= _REPLAY_MAX_WORDS):
             # the parallel reduction, kept when the replay shows the
             # reference's store holds every column (then its pairs are these)
-            replay_cap = reference_pool_caps([(3 * T, E)])[0]
+            replay_cap = cap
             ref_sizes = None
     elif semantics == "exact":
         # dense complexes (long columns): the serial complete reduction
@@ -462,10 +473,7 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
                                  ptr(pool), pool_words, ptr(scratch), scratch_bytes, status_p,
                                  _WAIT_SECONDS, s), "fph_reduce")
             if replay_cap is not None:
-                # rebuilt columns: at most the store capacity (else it
-                # overflows anyway), each rounded to 32-word lines, plus the
-                # wavefronts' partly used allocation chunks
-                words = replay_cap + 32 * K + (1 << 18)
+                words = _replay_words(replay_cap, K)
                 pool2 = _empty(torch.int32, device, words + 64)
                 check(lib.fph_reduce_replay(K, ptr(off), ptr(data), row_base, nrows, ptr(kind),
                                             ptr(own), ptr(pivot), ptr(res_list), rescnt_p,
