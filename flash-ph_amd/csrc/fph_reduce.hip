@@ -1727,23 +1727,31 @@ __device__ __forceinline__ void block_copy(const unsigned* src, int n, unsigned*
 // form, so the same results.
 constexpr int kBmLdsWords = 2 * kSerHalf + kSerStage;  // dynamic LDS words
 
+// A global bitmap belongs to one workgroup (its segment's), so its atomics,
+// loads and stores are workgroup-scoped: they stay in the L2 of the XCD the
+// workgroup runs on instead of going device-wide.
 template <bool G>
 __device__ __forceinline__ unsigned bm_load(const unsigned* p) {
-    if constexpr (G) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if constexpr (G) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     else return *p;
 }
 
 template <bool G>
 __device__ __forceinline__ void bm_zero(unsigned* p) {
-    if constexpr (G) __hip_atomic_store(p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if constexpr (G) __hip_atomic_store(p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     else *p = 0u;
 }
 
 // Flip local row x; returns +1 if the bit was set by it, -1 if cleared.
+template <bool G>
 __device__ __forceinline__ int bm_flip(unsigned* bits, unsigned* summ, int x) {
     const int w = x >> 5;
     const unsigned m = 1u << (x & 31);
-    const unsigned old = atomicXor(&bits[w], m);
+    unsigned old;
+    if constexpr (G)
+        old = __hip_atomic_fetch_xor(&bits[w], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else
+        old = atomicXor(&bits[w], m);
     if (old == 0u || old == m) atomicXor(&summ[w >> 5], 1u << (w & 31));
     return (old & m) ? -1 : 1;
 }
@@ -1781,7 +1789,7 @@ __device__ __forceinline__ void serial_bitmap(RedArgs& a, SerShared& sh, const u
         for (int e = t; e < n0; e += kSerThreads) {
             const long long x = (long long)a.data[lo_c + e] - row_lo;
             if (x < 0 || x >= rows) bad = 1;
-            else bm_flip(bits, summ, (int)x);
+            else bm_flip<G>(bits, summ, (int)x);
         }
         int len = n0;
         int p = (int)((long long)a.data[lo_c] - row_lo);
@@ -1810,7 +1818,7 @@ __device__ __forceinline__ void serial_bitmap(RedArgs& a, SerShared& sh, const u
                 if (x < 0 || x >= rows) {
                     bad = 1;
                 } else {
-                    delta += bm_flip(bits, summ, (int)x);
+                    delta += bm_flip<G>(bits, summ, (int)x);
                     lmax = max(lmax, (int)x);
                 }
             }

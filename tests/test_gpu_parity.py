@@ -196,6 +196,37 @@ def test_replay_detects_store_overflow():
         _eq("h1", _cpu(dg[1]), ref["h1"])
 
 
+def test_serial_global_bitmap():
+    """The serial reducer with its working-column bitmap in global memory
+    (clouds with more rows than LDS holds, as the 2,000-point Clifford
+    batch; forced here by a small LDS budget) gives the oracle's
+    reference-semantics diagrams, one cloud and a batch, with and without
+    store overflow."""
+    import oracle
+    from flash_ph import _pipeline, rips_persistence_batch
+    old = _pipeline._BITMAP_LDS_WORDS, _pipeline.REPLAY
+    _pipeline._BITMAP_LDS_WORDS = 64
+    _pipeline.REPLAY = False
+    try:
+        clouds = [(_store_overflow_clouds()[0], None), (cases.randn(400, 3, 2), 1.0),
+                  (cases.clifford(600, 3, 0.02), 0.6)]
+        refs = []
+        for x, t in clouds:
+            t = oracle.auto_threshold(x) if t is None else t
+            print(f"[gpu] serial global bitmap n={x.shape[0]} d={x.shape[1]}", flush=True)
+            ref = oracle.run(x, t, 1)
+            refs.append((t, ref))
+            dg, cx = _pipeline.compute(torch.from_numpy(x).cuda(), t, 1, semantics="reference")
+            _eq("h1", _cpu(dg[1]), ref["h1"])
+        print("[gpu] serial global bitmap batch", flush=True)
+        got = rips_persistence_batch([torch.from_numpy(x).cuda() for x, _ in clouds],
+                                     [t for t, _ in refs], max_dim=1)
+        for (t, ref), g in zip(refs, got):
+            _eq("batch h1", _cpu(g[1]), ref["h1"])
+    finally:
+        _pipeline._BITMAP_LDS_WORDS, _pipeline.REPLAY = old
+
+
 @pytest.mark.parametrize("variant", [0, 1])
 def test_serial_reducer_variants(variant):
     """Both forms of the working column (bitmap, sorted list) of the
