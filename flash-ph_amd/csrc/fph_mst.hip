@@ -125,21 +125,26 @@ __global__ void k_bvk_label(const int* __restrict__ comp, const unsigned* __rest
 // mutual pair only the larger root), which is the root-side rule above
 // without reading the chosen edge's endpoints back from memory.  With
 // CACHED, each thread keeps its (at most kBvkEdgesPerThread) edges in
-// registers for all rounds, one word each (a << 16 | b: n < 2^16).
+// registers for all rounds, one word each (a << 16 | b: n < 2^16), and drops
+// an edge once it lies inside a component, so later rounds touch LDS only
+// for crossing edges.
 constexpr int kBvkLdsThreads = 1024;
 constexpr int kBvkLdsMaxN = 12288;          // 12 * n bytes <= 144 KB
 constexpr long long kBvkLdsMaxE = 1 << 19;  // beyond, the edge sweeps favour the grid
 constexpr int kBvkEdgesPerThread = 32;      // register-resident edges (CACHED)
 
 // One edge (a, b) of rank e during find: offers e to both components.
-__device__ __forceinline__ void bvk_offer(const int* comp, unsigned* min_e, int a, int b,
+// Returns true when the edge lies inside one component (it stays inside:
+// components only merge).
+__device__ __forceinline__ bool bvk_offer(const int* comp, unsigned* min_e, int a, int b,
                                           unsigned e, int& cross) {
-    if (a < 0) return;  // unused slot of a capacity-sized edge list
+    if (a < 0) return true;  // unused slot of a capacity-sized edge list
     const int cu = comp[a], cv = comp[b];
-    if (cu == cv) return;
+    if (cu == cv) return true;
     atomicMin(&min_e[cu], e);
     atomicMin(&min_e[cv], e);
     cross = 1;
+    return false;
 }
 
 // One edge during hook: hooks a component that chose it (the forest edges
@@ -189,12 +194,13 @@ __global__ __launch_bounds__(kBvkLdsThreads) void k_bvk_lds(
         if constexpr (CACHED) {
 #pragma unroll
             for (int u = 0; u < kBvkEdgesPerThread; u++)
-                if (ce[u] != 0xFFFFFFFFu)
+                if (ce[u] != 0xFFFFFFFFu &&
                     bvk_offer(comp, min_e, (int)(ce[u] >> 16), (int)(ce[u] & 0xFFFFu),
-                              (unsigned)(tid + u * kBvkLdsThreads), cross);
+                              (unsigned)(tid + u * kBvkLdsThreads), cross))
+                    ce[u] = 0xFFFFFFFFu;
         } else {
             for (long long e = tid; e < E; e += kBvkLdsThreads)
-                bvk_offer(comp, min_e, ei[e], ej[e], (unsigned)e, cross);
+                (void)bvk_offer(comp, min_e, ei[e], ej[e], (unsigned)e, cross);
         }
         if (cross) s_cross = 1;
         __syncthreads();
