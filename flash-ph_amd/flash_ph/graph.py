@@ -10,8 +10,8 @@ new cloud: one graph launch and one count read per call.  When a cloud has
 more edges than the capacity, the graph is re-captured with room for them
 and replayed, so the result always equals the eager path's.  Without an
 explicit capacity, the first call screens the cloud once to count its
-edges and sizes the slots at 1.25x that count: every stage of the graph
-then runs over about as many slots as the eager path has edges.
+edges and sizes the slots at 1.0625x that count (+256): every stage of the
+graph then runs over about as many slots as the eager path has edges.
 
 Not in the reference, which drives every stage from Python on each call
 (persistence.py:140-186).
@@ -59,7 +59,10 @@ class RipsH0Graph:
         lib = _lib.load()
         dev, n, cap = self.device, self.n, self.cap
         e = lambda dt, *sh: torch.empty(*sh, dtype=dt, device=dev)
-        self.count = torch.zeros(2, dtype=torch.int64, device=dev)
+        # [0] edge count, [1] screen candidates, [2] low half: convergence
+        # flag -- read back together, one host copy per replay
+        self.state = torch.zeros(3, dtype=torch.int64, device=dev)
+        self.count = self.state[0:2]
         self.edge_i, self.edge_j = e(torch.int32, cap), e(torch.int32, cap)
         self.edge_dsq, self.edge_filt = e(torch.float32, cap), e(torch.float32, cap)
         self.adj_ptr = e(torch.int32, n + 1)
@@ -67,11 +70,10 @@ class RipsH0Graph:
         self.adj_dsq = e(torch.float32, 2 * cap)
         self.mst_mask = e(torch.uint8, cap)
         self.h0 = e(torch.float32, n, 2)
-        self.converged = torch.zeros(1, dtype=torch.int32, device=dev)
         p = _lib.ptr
         bufs = (p(self.count), p(self.edge_i), p(self.edge_j), p(self.edge_dsq),
                 p(self.edge_filt), p(self.adj_ptr), p(self.adj_idx), p(self.adj_rank),
-                p(self.adj_dsq), p(self.mst_mask), p(self.h0), p(self.converged))
+                p(self.adj_dsq), p(self.mst_mask), p(self.h0), p(self.state) + 16)
         head = (p(self.points), n, self.d, self.thr_sq, self.screen, cap)
         nb = ctypes.c_size_t(0)
         _lib.check(lib.fph_h0_pipeline(*head, *bufs, None, ctypes.byref(nb), None),
@@ -87,7 +89,7 @@ class RipsH0Graph:
 
     @staticmethod
     def _headroom(E: int) -> int:
-        return E + E // 4 + 1024
+        return E + E // 16 + 256
 
     def _count_edges(self) -> int:
         """Edge count of the current points (one eager screen)."""
@@ -110,16 +112,16 @@ class RipsH0Graph:
                 self.cap = self._headroom(self._count_edges())
             self._capture()
         self._launch()
-        E = int(self.count[0].item())
+        E, _, conv = self.state.tolist()
         if E > self.cap:
             # more edges than slots: re-capture with room and replay
             self.cap = self._headroom(E)
             self._capture()
             self._launch()
-            E = int(self.count[0].item())
+            E, _, conv = self.state.tolist()
             if E > self.cap:
                 raise _lib.FlashPHError("RipsH0Graph: edge count grew on replay")
-        if not int(self.converged.item()):
+        if not conv & 0xFFFFFFFF:
             raise _lib.FlashPHError("Boruvka did not converge")
         self.E = E
         return self.h0.clone()
