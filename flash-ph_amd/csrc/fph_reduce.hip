@@ -428,7 +428,7 @@ __device__ int spec_read_dev(const unsigned* src, unsigned& v, long long wait) {
     while (true) {
         v = first ? ld_fast(src + fph_lane()) : ld_coh(src + fph_lane());
         first = false;
-        const unsigned len = __shfl(v, 0);
+        const unsigned len = (unsigned)__builtin_amdgcn_readlane((int)v, 0);
         if (len != kUnwritten) return (int)len;
         if (wall_clock64() - t0 > wait) return -1;
         __builtin_amdgcn_s_sleep(1);
@@ -949,26 +949,25 @@ __device__ unsigned bm_flip_published(const RedArgs& a, BmWork& w, const unsigne
                                       unsigned v, int n, long long wait) {
     const int lane = fph_lane();
     const long long t0 = wall_clock64();
-    int delta = 0, hi = -1;
+    int delta = 0, hi = -1;  // wave-uniform
     unsigned first = 0;
     for (int base = 0; base <= n; base += FPH_WAVE) {
         const int i = base + lane;            // word index in [len, rows...]
         const bool need = i >= 1 && i <= n;
         unsigned x = base == 0 ? v : (need ? ld_fast(src + i) : 0u);
         if (!settle_word(src + i, need, x, t0, wait)) return 0xFFFFFFFFu;
-        if (base == 0) first = __shfl(x, 1);
+        if (base == 0) first = (unsigned)__builtin_amdgcn_readlane((int)x, 1);
+        const unsigned r = x - (unsigned)a.row_base;
+        const int wd = (int)(r >> 5);
+        bool set = false;
         if (need) {
-            const unsigned r = x - (unsigned)a.row_base;
-            const int wd = (int)(r >> 5);
             const unsigned bit = 1u << (r & 31);
-            const unsigned old = atomicXor(&w.bm[wd], bit);
-            delta += (old & bit) ? -1 : 1;
-            hi = max(hi, wd);
+            set = (atomicXor(&w.bm[wd], bit) & bit) == 0u;
         }
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        delta += __shfl_xor(delta, o);
-        hi = max(hi, __shfl_xor(hi, o));
+        // rows ascend: the chunk's last row holds its highest word
+        const unsigned long long mn = __ballot(need);
+        delta += 2 * __popcll(__ballot(set)) - __popcll(mn);
+        if (mn) hi = max(hi, __builtin_amdgcn_readlane(wd, 63 - __clzll((long long)mn)));
     }
     w.len += delta;
     w.hi_w = max(w.hi_w, hi);
@@ -979,18 +978,22 @@ __device__ unsigned bm_flip_published(const RedArgs& a, BmWork& w, const unsigne
 // Flips n rows of an immutable list (the CSR columns) into the bitmap;
 // returns the lowest row.
 __device__ unsigned bm_flip_list(const RedArgs& a, BmWork& w, const unsigned* rows, int n) {
-    int delta = 0, hi = -1;
-    for (int i = fph_lane(); i < n; i += FPH_WAVE) {
-        const unsigned r = rows[i] - (unsigned)a.row_base;
-        const int wd = (int)(r >> 5);
-        const unsigned bit = 1u << (r & 31);
-        const unsigned old = atomicXor(&w.bm[wd], bit);
-        delta += (old & bit) ? -1 : 1;
-        hi = max(hi, wd);
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        delta += __shfl_xor(delta, o);
-        hi = max(hi, __shfl_xor(hi, o));
+    int delta = 0, hi = -1;  // wave-uniform
+    for (int base = 0; base < n; base += FPH_WAVE) {
+        const int i = base + fph_lane();
+        const bool need = i < n;
+        int wd = 0;
+        bool set = false;
+        if (need) {
+            const unsigned r = rows[i] - (unsigned)a.row_base;
+            wd = (int)(r >> 5);
+            const unsigned bit = 1u << (r & 31);
+            set = (atomicXor(&w.bm[wd], bit) & bit) == 0u;
+        }
+        // rows ascend: the chunk's last row holds its highest word
+        const unsigned long long mn = __ballot(need);
+        delta += 2 * __popcll(__ballot(set)) - __popcll(mn);
+        hi = max(hi, __builtin_amdgcn_readlane(wd, 63 - __clzll((long long)mn)));
     }
     w.len += delta;
     w.hi_w = max(w.hi_w, hi);
@@ -1007,7 +1010,7 @@ __device__ void bm_next_pivot(const RedArgs& a, BmWork& w) {
         const unsigned long long b = __ballot(x != 0u);
         if (b) {
             const int l = __ffsll((long long)b) - 1;
-            const unsigned xw = __shfl(x, l);
+            const unsigned xw = (unsigned)__builtin_amdgcn_readlane((int)x, l);
             w.piv = (unsigned)a.row_base + 32u * (unsigned)(wd + l) + (unsigned)(__ffs(xw) - 1);
             return;
         }
@@ -1131,7 +1134,8 @@ __device__ int reduce_claim_dev_bm(const RedArgs& a, int c, unsigned src, bool c
         if (a.lines && (unsigned long long)v != kNoOwner) {
             const unsigned tag = (unsigned)(v >> 32);
             const unsigned lo32 = (unsigned)lw, hi32 = (unsigned)(lw >> 32);
-            const unsigned t0 = __shfl(hi32, 0), l0 = __shfl(lo32, 0);
+            const unsigned t0 = (unsigned)__builtin_amdgcn_readlane((int)hi32, 0);
+            const unsigned l0 = (unsigned)__builtin_amdgcn_readlane((int)lo32, 0);
             if (t0 == tag && l0 <= (unsigned)(kLineWords - 1)) {
                 const bool need = fph_lane() >= 1 && fph_lane() <= (int)l0;
                 if (!__ballot(need && hi32 != tag)) {
