@@ -1815,25 +1815,29 @@ __device__ __forceinline__ void serial_bitmap(RedArgs& a, SerShared& sh, const u
             const unsigned* ob = a.pool + (v & 0xFFFFFFFFull);
             const int nb = (int)ob[0];
             if (t == 0) sh.first[par] = 0x7FFFFFFF;
-            int delta = 0, lmax = 0;
-            bad = 0;
-            for (int e = t; e < nb; e += kSerThreads) {
-                const long long x = (long long)ob[1 + e] - row_lo;
-                if (x < 0 || x >= rows) {
-                    bad = 1;
-                } else {
-                    delta += bm_flip<G>(bits, summ, (int)x);
-                    lmax = max(lmax, (int)x);
+            // the wave's sums by ballot (wave-uniform): change in length =
+            // 2 x bits set - rows flipped; rows ascend, so the wave's last
+            // valid row is its largest
+            int delta = 0, lmax = 0, badw = 0;
+            for (int base = 0; base < nb; base += kSerThreads) {
+                const int e = base + t;
+                const bool need = e < nb;
+                long long x = 0;
+                bool ok = false, set = false;
+                if (need) {
+                    x = (long long)ob[1 + e] - row_lo;
+                    ok = x >= 0 && x < rows;
+                    if (ok) set = bm_flip<G>(bits, summ, (int)x) > 0;
                 }
-            }
-            for (int o = 32; o > 0; o >>= 1) {
-                delta += __shfl_xor(delta, o);
-                bad |= __shfl_xor(bad, o);
-                lmax = max(lmax, __shfl_xor(lmax, o));
+                const unsigned long long mok = __ballot(ok);
+                badw |= __ballot(need && !ok) != 0ull;
+                delta += 2 * __popcll(__ballot(set)) - __popcll(mok);
+                if (mok)
+                    lmax = max(lmax, __builtin_amdgcn_readlane((int)x, 63 - __clzll((long long)mok)));
             }
             if (lane == 0) {
                 sh.red[par][wid][0] = delta;
-                sh.red[par][wid][1] = bad;
+                sh.red[par][wid][1] = badw;
                 sh.red[par][wid][2] = lmax;
             }
             __syncthreads();
