@@ -1812,8 +1812,12 @@ __device__ __forceinline__ void serial_bitmap(RedArgs& a, SerShared& sh, const u
         while (true) {
             const unsigned long long v = a.own[row_lo + p - a.row_base];
             if (v == kNoOwner) break;
-            const unsigned* ob = a.pool + (v & 0xFFFFFFFFull);
+            const unsigned long long vo = v & 0xFFFFFFFFull;
+            const unsigned* ob = a.pool + vo;
+            // the length and each thread's first row in one round trip (a
+            // row past the column's end is a pool word, then ignored)
             const int nb = (int)ob[0];
+            const unsigned r0 = vo + 1 + t < a.pool_cap ? ob[1 + t] : 0u;
             if (t == 0) sh.first[par] = 0x7FFFFFFF;
             // the wave's sums by ballot (wave-uniform): change in length =
             // 2 x bits set - rows flipped; rows ascend, so the wave's last
@@ -1825,7 +1829,7 @@ __device__ __forceinline__ void serial_bitmap(RedArgs& a, SerShared& sh, const u
                 long long x = 0;
                 bool ok = false, set = false;
                 if (need) {
-                    x = (long long)ob[1 + e] - row_lo;
+                    x = (long long)(base == 0 ? r0 : ob[1 + e]) - row_lo;
                     ok = x >= 0 && x < rows;
                     if (ok) set = bm_flip<G>(bits, summ, (int)x) > 0;
                 }
