@@ -1614,7 +1614,7 @@ struct SerShared {
     int next_nb;                // that owner's length
     long long store_at;         // store offset when allocated from the pool
     int red[2][kSerWaves][3];   // bitmap form: per-wave (sum, bad, max), by parity
-    int first[2];               // bitmap form: lowest set word x 32 + bit, by parity
+    int wfirst[2][kSerWaves];   // bitmap form: per-wave lowest set row, by parity
 };
 
 // Exclusive prefix sum of v over the workgroup; *total = sum.  Contains
@@ -1818,7 +1818,7 @@ __device__ __forceinline__ void serial_bitmap(RedArgs& a, SerShared& sh, const u
             // row past the column's end is a pool word, then ignored)
             const int nb = (int)ob[0];
             const unsigned r0 = vo + 1 + t < a.pool_cap ? ob[1 + t] : 0u;
-            if (t == 0) sh.first[par] = 0x7FFFFFFF;
+
             // the wave's sums by ballot (wave-uniform): change in length =
             // 2 x bits set - rows flipped; rows ascend, so the wave's last
             // valid row is its largest
@@ -1863,26 +1863,38 @@ __device__ __forceinline__ void serial_bitmap(RedArgs& a, SerShared& sh, const u
                 return;
             }
             if (len == 0) break;
-            // lowest set bit at or above the old pivot's word: one strided
-            // sweep of the summary, each thread stopping at its first hit
-            // and offering that word's lowest set row.  The row, not the
-            // word, goes through LDS: after the barrier no wave reads the
-            // bitmap, which the next addition's flips may already change.
-            // (The summary mirrors the nonzero words exactly: each zero <->
-            // nonzero transition of a word is seen by exactly one flip.)
+            // lowest set bit at or above the old pivot's word and at or
+            // below the highest row flipped: each wave walks its interleaved
+            // 64-word chunks of the summary and stops at its first chunk
+            // with a nonzero word (ballot); that word's lowest set row is
+            // the wave's candidate, in the wave's own LDS slot.  The row,
+            // not the word, goes through LDS: after the barrier no wave
+            // reads the bitmap, which the next addition's flips may already
+            // change.  (The summary mirrors the nonzero words exactly: each
+            // zero <-> nonzero transition of a word is seen by one flip.)
             const int ws = p >> 5;
-            for (int i = (ws >> 5) + t; i < S; i += kSerThreads) {
-                unsigned val = summ[i];
-                if (i == (ws >> 5)) val &= ~0u << (ws & 31);
-                if (val) {
-                    const int w = i * 32 + __builtin_ctz(val);
-                    const unsigned bw = bm_load<G>(&bits[w]);
-                    if (bw) atomicMin(&sh.first[par], w * 32 + __builtin_ctz(bw));
+            const int s_lo = ws >> 5, s_hi = min(phigh >> 10, S - 1);
+            int cand = 0x7FFFFFFF;
+            for (int i0 = s_lo + wid * FPH_WAVE; i0 <= s_hi; i0 += kSerThreads) {
+                const int i = i0 + lane;
+                unsigned val = i <= s_hi ? summ[i] : 0u;
+                if (i == s_lo) val &= ~0u << (ws & 31);
+                const unsigned long long hit = __ballot(val != 0u);
+                if (hit) {
+                    const int l = __ffsll((long long)hit) - 1;
+                    if (lane == l) {
+                        const int w = i * 32 + __builtin_ctz(val);
+                        const unsigned bw = bm_load<G>(&bits[w]);
+                        if (bw) cand = w * 32 + __builtin_ctz(bw);
+                    }
+                    cand = __builtin_amdgcn_readlane(cand, l);
                     break;
                 }
             }
+            if (lane == 0) sh.wfirst[par][wid] = cand;
             __syncthreads();
-            const int found = sh.first[par];
+            int found = sh.wfirst[par][0];
+            for (int w2 = 1; w2 < kSerWaves; w2++) found = min(found, sh.wfirst[par][w2]);
             const int np = found != 0x7FFFFFFF ? found : -1;
             if (!(np > p)) {  // pivot must increase
                 if (t == 0) atomicExch(&a.status[0], 3);
