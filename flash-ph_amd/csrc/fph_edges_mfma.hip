@@ -35,6 +35,7 @@ typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
 constexpr int kMT = 128;       // points per tile side
 constexpr int kThreadsM = 256; // 4 waves, each a 64 x 64 quarter of the tile
+constexpr int kStageSteps = 2; // 16-dimension k-steps staged in LDS per chunk
 
 inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -94,6 +95,11 @@ __global__ __launch_bounds__(kThreadsM, 4) void k_edge_screen_mfma(
     unsigned long long* __restrict__ vals, long long cap,
     unsigned long long* __restrict__ count, unsigned long long* __restrict__ ncand) {
     __shared__ float s_nr[kMT], s_nc[kMT];
+    // operands of kStageSteps k-steps of the whole tile, [h/l][vector][point]:
+    // each row and column block is read from global memory once per tile,
+    // not once per wave that uses it
+    __shared__ bf16x8 sA[2][2 * kStageSteps][kMT];
+    __shared__ bf16x8 sB[2][2 * kStageSteps][kMT];
     // XCD-aware order: the 8 XCDs take blockIdx round robin; give each XCD
     // a contiguous run of tiles (shared row blocks stay in its L2).
     const long long nb = gridDim.x;
@@ -116,7 +122,8 @@ __global__ __launch_bounds__(kThreadsM, 4) void k_edge_screen_mfma(
     if (!live) return;
     const int w = tid / FPH_WAVE, lane = fph_lane();
     const int rsub = (w >> 1) * 64, csub = (w & 1) * 64;
-    if (I == J && rsub > csub) return;  // entirely below the diagonal
+    // entirely below the diagonal: stages operands with the others, then leaves
+    const bool below = I == J && rsub > csub;
     const int row0 = I * kMT + rsub, col0 = J * kMT + csub;
     const int vrow = dpad / 8;  // bf16x8 vectors per point
     const int r = lane & 31, kh = lane >> 5;
@@ -127,29 +134,46 @@ __global__ __launch_bounds__(kThreadsM, 4) void k_edge_screen_mfma(
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int v = 0; v < 16; v++) acc[a][b][v] = 0.0f;
-    const long long ra0 = (long long)min(row0 + r, n - 1) * vrow;
-    const long long ra1 = (long long)min(row0 + 32 + r, n - 1) * vrow;
-    const long long cb0 = (long long)min(col0 + r, n - 1) * vrow;
-    const long long cb1 = (long long)min(col0 + 32 + r, n - 1) * vrow;
-    for (int ks = 0; ks < dpad / 16; ks++) {
-        const int o = ks * 2 + kh;
-        const bf16x8 ah0 = xh[ra0 + o], al0 = xl[ra0 + o];
-        const bf16x8 ah1 = xh[ra1 + o], al1 = xl[ra1 + o];
-        const bf16x8 bh0 = xh[cb0 + o], bl0 = xl[cb0 + o];
-        const bf16x8 bh1 = xh[cb1 + o], bl1 = xl[cb1 + o];
-        acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bh0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bh1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bh0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bh1, acc[1][1], 0, 0, 0);
-        acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bl0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bl1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bl0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bl1, acc[1][1], 0, 0, 0);
-        acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al0, bh0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al0, bh1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al1, bh0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al1, bh1, acc[1][1], 0, 0, 0);
+    for (int kc = 0; kc < vrow; kc += 2 * kStageSteps) {
+        const int nv = min(2 * kStageSteps, vrow - kc);  // vectors in this chunk (even)
+        __syncthreads();  // the previous chunk's readers are done
+        for (int q = tid; q < 2 * nv * kMT; q += kThreadsM) {
+            const int side = q / (nv * kMT);
+            const int rem = q - side * nv * kMT;
+            const int vv = rem / kMT, pt = rem - vv * kMT;
+            const long long g = (long long)min((side ? J : I) * kMT + pt, n - 1);
+            const long long at = g * vrow + kc + vv;
+            if (side) {
+                sB[0][vv][pt] = xh[at];
+                sB[1][vv][pt] = xl[at];
+            } else {
+                sA[0][vv][pt] = xh[at];
+                sA[1][vv][pt] = xl[at];
+            }
+        }
+        __syncthreads();
+        if (below) continue;
+        for (int ks = 0; ks < nv / 2; ks++) {
+            const int o = ks * 2 + kh;
+            const bf16x8 ah0 = sA[0][o][rsub + r], al0 = sA[1][o][rsub + r];
+            const bf16x8 ah1 = sA[0][o][rsub + 32 + r], al1 = sA[1][o][rsub + 32 + r];
+            const bf16x8 bh0 = sB[0][o][csub + r], bl0 = sB[1][o][csub + r];
+            const bf16x8 bh1 = sB[0][o][csub + 32 + r], bl1 = sB[1][o][csub + 32 + r];
+            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bh0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bh1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bh0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bh1, acc[1][1], 0, 0, 0);
+            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bl0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bl1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bl0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bl1, acc[1][1], 0, 0, 0);
+            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al0, bh0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al0, bh1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al1, bh0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al1, bh1, acc[1][1], 0, 0, 0);
+        }
     }
+    if (below) return;
     // Epilogue: accumulator slot v of lane l is row 8 (v / 4) + 4 (l / 32)
     // + v % 4, column l % 32 of its 32 x 32 block.  Off the diagonal and
     // inside [0, n) every pair is valid (i < j < n).
