@@ -302,7 +302,22 @@ def main():
                     help="keep the default launch parameters (no autotuning pass)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 256 MiB three times (the PMC byte-counter calibration)")
+    ap.add_argument("--profile-mfma", action="store_true",
+                    help="only run the n=200K d=50 t=6.0 edge stage three times with the "
+                         "split-bf16 screen (for a kernel trace)")
     args = ap.parse_args()
+    if args.profile_mfma:
+        import numpy as np
+        import torch
+        from flash_ph import _pipeline
+        x = torch.from_numpy(np.random.default_rng(0).standard_normal(
+            (200_000, 50)).astype(np.float32)).cuda()
+        _pipeline.SCREEN = "mfma"
+        for _ in range(3):
+            cx = _pipeline.build_edges(x, 6.0)
+        torch.cuda.synchronize()
+        print(f"edge stage x3, E={cx.E}", flush=True)
+        return
     if args.calibrate_copy:
         import torch
         x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")  # 256 MiB
