@@ -34,8 +34,14 @@ typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
 constexpr int kMT = 128;       // points per tile side
+constexpr int kMTLog = 7;
 constexpr int kThreadsM = 256; // 4 waves, each a 64 x 64 quarter of the tile
 constexpr int kStageSteps = 2; // 16-dimension k-steps staged in LDS per chunk
+static_assert(kMT == 1 << kMTLog, "tile side is a power of two");
+// A chunk holds 2 or 4 vectors of 8 coordinates per point (dpad is a
+// multiple of 16), so a point's hi and lo vectors of one chunk are 4 or 8
+// contiguous bf16x8: a power of two, decoded with shifts.
+static_assert(kStageSteps == 2, "chunk decode assumes 2 or 4 vectors per point");
 
 inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -53,10 +59,12 @@ __device__ __forceinline__ float bf16_to_f32(unsigned short h) {
     return f;
 }
 
-// One thread per (point, coordinate) of the padded layout: xh, xl (bf16,
-// n x dpad, zero padded) and, per point, |x|^2 as an FP32 FMA chain.
+// One thread per (point, coordinate) of the padded coordinates (zero past
+// d).  Layout of xhl (bf16, 2 n dpad): per point, per vector of 8
+// coordinates, the hi vector then the lo vector, so one point's operands for
+// 4 k-steps of 16 coordinates are one 128-byte line.
 __global__ void k_split_bf16(const float* __restrict__ pts, long long n, int d, int dpad,
-                             unsigned short* __restrict__ xh, unsigned short* __restrict__ xl) {
+                             unsigned short* __restrict__ xhl) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * dpad) return;
     const long long i = t / dpad;
@@ -64,8 +72,9 @@ __global__ void k_split_bf16(const float* __restrict__ pts, long long n, int d, 
     const float x = k < d ? pts[i * d + k] : 0.0f;
     const unsigned short h = bf16_rne(x);
     const float r = x - bf16_to_f32(h);  // exact
-    xh[t] = h;
-    xl[t] = bf16_rne(r);
+    const long long at = 2 * (i * dpad + (k & ~7)) + (k & 7);
+    xhl[at] = h;
+    xhl[at + 8] = bf16_rne(r);
 }
 
 __global__ void k_norms(const float* __restrict__ pts, long long n, int d,
@@ -89,17 +98,19 @@ __device__ __forceinline__ void tile_coords(long long t, int* I, int* J) {
 }
 
 __global__ __launch_bounds__(kThreadsM, 4) void k_edge_screen_mfma(
-    const float* __restrict__ pts, const bf16x8* __restrict__ xh,
-    const bf16x8* __restrict__ xl, const float* __restrict__ nrm, int n, int d, int dpad,
+    const float* __restrict__ pts, const bf16x8* __restrict__ xhl,
+    const float* __restrict__ nrm, int n, int d, int dpad,
     float thr, float marg_a, float marg_b, long long ntiles, unsigned long long* __restrict__ keys,
     unsigned long long* __restrict__ vals, long long cap,
     unsigned long long* __restrict__ count, unsigned long long* __restrict__ ncand) {
     __shared__ float s_nr[kMT], s_nc[kMT];
-    // operands of kStageSteps k-steps of the whole tile, [h/l][vector][point]:
-    // each row and column block is read from global memory once per tile,
-    // not once per wave that uses it
-    __shared__ bf16x8 sA[2][2 * kStageSteps][kMT];
-    __shared__ bf16x8 sB[2][2 * kStageSteps][kMT];
+    // operands of kStageSteps k-steps of the whole tile, [2 vector + h/l]
+    // [point]: each row and column block is read from global memory once per
+    // tile, not once per wave that uses it.  Rows are padded by one vector:
+    // the 8 lanes that stage one point's line write 8 rows at distinct
+    // banks, and 8 lanes reading consecutive points of one row do too.
+    __shared__ bf16x8 sA[4 * kStageSteps][kMT + 1];
+    __shared__ bf16x8 sB[4 * kStageSteps][kMT + 1];
     // XCD-aware order: the 8 XCDs take blockIdx round robin; give each XCD
     // a contiguous run of tiles (shared row blocks stay in its L2).
     const long long nb = gridDim.x;
@@ -136,29 +147,29 @@ __global__ __launch_bounds__(kThreadsM, 4) void k_edge_screen_mfma(
             for (int v = 0; v < 16; v++) acc[a][b][v] = 0.0f;
     for (int kc = 0; kc < vrow; kc += 2 * kStageSteps) {
         const int nv = min(2 * kStageSteps, vrow - kc);  // vectors in this chunk (even)
+        // per point, 2 nv consecutive vectors (hi, lo of vectors kc..kc+nv-1):
+        // consecutive lanes read consecutive 16 bytes of one point's line
+        const int lgp = nv == 4 ? 3 : 2;  // log2(2 nv)
         __syncthreads();  // the previous chunk's readers are done
-        for (int q = tid; q < 2 * nv * kMT; q += kThreadsM) {
-            const int side = q / (nv * kMT);
-            const int rem = q - side * nv * kMT;
-            const int vv = rem / kMT, pt = rem - vv * kMT;
+        for (int q = tid; q < (2 * kMT) << lgp; q += kThreadsM) {
+            const int side = q >> (lgp + kMTLog);
+            const int rem = q & ((kMT << lgp) - 1);
+            const int pt = rem >> lgp, jv = rem & ((1 << lgp) - 1);
             const long long g = (long long)min((side ? J : I) * kMT + pt, n - 1);
-            const long long at = g * vrow + kc + vv;
-            if (side) {
-                sB[0][vv][pt] = xh[at];
-                sB[1][vv][pt] = xl[at];
-            } else {
-                sA[0][vv][pt] = xh[at];
-                sA[1][vv][pt] = xl[at];
-            }
+            const bf16x8 v = xhl[2 * (g * vrow + kc) + jv];
+            if (side)
+                sB[jv][pt] = v;
+            else
+                sA[jv][pt] = v;
         }
         __syncthreads();
         if (below) continue;
         for (int ks = 0; ks < nv / 2; ks++) {
-            const int o = ks * 2 + kh;
-            const bf16x8 ah0 = sA[0][o][rsub + r], al0 = sA[1][o][rsub + r];
-            const bf16x8 ah1 = sA[0][o][rsub + 32 + r], al1 = sA[1][o][rsub + 32 + r];
-            const bf16x8 bh0 = sB[0][o][csub + r], bl0 = sB[1][o][csub + r];
-            const bf16x8 bh1 = sB[0][o][csub + 32 + r], bl1 = sB[1][o][csub + 32 + r];
+            const int o = 2 * (ks * 2 + kh);  // hi row of this lane's vector; lo at o + 1
+            const bf16x8 ah0 = sA[o][rsub + r], al0 = sA[o + 1][rsub + r];
+            const bf16x8 ah1 = sA[o][rsub + 32 + r], al1 = sA[o + 1][rsub + 32 + r];
+            const bf16x8 bh0 = sB[o][csub + r], bl0 = sB[o + 1][csub + r];
+            const bf16x8 bh1 = sB[o][csub + 32 + r], bl1 = sB[o + 1][csub + 32 + r];
             acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bh0, acc[0][0], 0, 0, 0);
             acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, bh1, acc[0][1], 0, 0, 0);
             acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah1, bh0, acc[1][0], 0, 0, 0);
@@ -239,18 +250,17 @@ extern "C" int fph_edge_screen_mfma(const float* pts, int64_t n, int d, float th
     if (n < 0 || d < 1 || cap < 0 || !count_dev || !cand_dev || !ws_bytes) return FPH_ERR_ARG;
     if (n > 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
     const int dpad = (d + 15) / 16 * 16;
-    const size_t half = align_up(2 * (size_t)std::max<int64_t>(n, 1) * dpad);
-    const size_t need = 2 * half + align_up(4 * (size_t)std::max<int64_t>(n, 1));
+    const size_t split = align_up(4 * (size_t)std::max<int64_t>(n, 1) * dpad);
+    const size_t need = split + align_up(4 * (size_t)std::max<int64_t>(n, 1));
     if (!ws) { *ws_bytes = need; return FPH_OK; }
     if (*ws_bytes < need) return FPH_ERR_ARG;
     char* w = (char*)ws;
-    unsigned short* xh = (unsigned short*)w;
-    unsigned short* xl = (unsigned short*)(w + half);
-    float* nrm = (float*)(w + 2 * half);
+    unsigned short* xhl = (unsigned short*)w;
+    float* nrm = (float*)(w + split);
     k_zero2<<<1, 64, 0, s>>>((unsigned long long*)count_dev, (unsigned long long*)cand_dev);
     FPH_LAUNCH_CHECK();
     if (n < 2) return FPH_OK;
-    k_split_bf16<<<fph_blocks(n * dpad, 256), 256, 0, s>>>(pts, n, d, dpad, xh, xl);
+    k_split_bf16<<<fph_blocks(n * dpad, 256), 256, 0, s>>>(pts, n, d, dpad, xhl);
     FPH_LAUNCH_CHECK();
     k_norms<<<fph_blocks(n, 256), 256, 0, s>>>(pts, n, d, nrm);
     FPH_LAUNCH_CHECK();
@@ -265,7 +275,7 @@ extern "C" int fph_edge_screen_mfma(const float* pts, int64_t n, int d, float th
     // [0, 8 ceil(ntiles / 8)); blocks past ntiles exit.
     const long long nblk = (ntiles + 7) / 8 * 8;
     k_edge_screen_mfma<<<(unsigned)nblk, kThreadsM, 0, s>>>(
-        pts, (const bf16x8*)xh, (const bf16x8*)xl, nrm, (int)n, d, dpad, thr_sq, marg_a, marg_b,
+        pts, (const bf16x8*)xhl, nrm, (int)n, d, dpad, thr_sq, marg_a, marg_b,
         ntiles, (unsigned long long*)keys, (unsigned long long*)vals, cap,
         (unsigned long long*)count_dev, (unsigned long long*)cand_dev);
     FPH_LAUNCH_CHECK();
