@@ -97,6 +97,47 @@ __device__ __forceinline__ void tile_coords(long long t, int* I, int* J) {
     *I = (int)(t - j * (j + 1) / 2);
 }
 
+// Stages one chunk of the tile's operands: per point, 2^LGP consecutive
+// vectors (hi, lo of the chunk's vectors).  Thread tid takes vector slot
+// tid % 2^LGP of points tid / 2^LGP + P u (P = 256 / 2^LGP), so consecutive
+// lanes read consecutive 16 bytes of one point's line and the LDS addresses
+// differ by constant offsets.  Loads go out in groups of kStageGroup with
+// constant trip counts: a group's loads are all in flight before the first
+// LDS write waits (one memory latency per group, not per load).  Point
+// indices pass through an empty asm so that they are recomputed per chunk
+// rather than held across the k-loop beside the accumulators.
+constexpr int kStageGroup = 4;
+
+template <int LGP>
+__device__ __forceinline__ void stage_chunk(const bf16x8* __restrict__ xhl, int I, int J, int n,
+                                            int vrow, int kc, int tid,
+                                            bf16x8 (*sA)[kMT + 1], bf16x8 (*sB)[kMT + 1]) {
+    constexpr int kPts = kThreadsM >> LGP;   // points per pass
+    constexpr int kPasses = kMT / kPts;      // passes per side
+    constexpr int kLoads = 2 * kPasses;      // both sides
+    static_assert(kPasses * kPts == kMT && kLoads % kStageGroup == 0, "whole groups");
+    const int jv = tid & ((1 << LGP) - 1), p0 = tid >> LGP;
+    const bf16x8* src = xhl + 2 * kc + jv;
+    const long long stride = 2LL * vrow;
+#pragma unroll
+    for (int f0 = 0; f0 < kLoads; f0 += kStageGroup) {
+        bf16x8 v[kStageGroup];
+#pragma unroll
+        for (int u = 0; u < kStageGroup; u++) {
+            const int f = f0 + u;
+            int g = min((f < kPasses ? I : J) * kMT + p0 + (f % kPasses) * kPts, n - 1);
+            asm volatile("" : "+v"(g));
+            v[u] = src[(long long)g * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < kStageGroup; u++) {
+            const int f = f0 + u;
+            bf16x8(*dst)[kMT + 1] = f < kPasses ? sA : sB;
+            dst[jv][p0 + (f % kPasses) * kPts] = v[u];
+        }
+    }
+}
+
 __global__ __launch_bounds__(kThreadsM, 4) void k_edge_screen_mfma(
     const float* __restrict__ pts, const bf16x8* __restrict__ xhl,
     const float* __restrict__ nrm, int n, int d, int dpad,
@@ -147,21 +188,11 @@ __global__ __launch_bounds__(kThreadsM, 4) void k_edge_screen_mfma(
             for (int v = 0; v < 16; v++) acc[a][b][v] = 0.0f;
     for (int kc = 0; kc < vrow; kc += 2 * kStageSteps) {
         const int nv = min(2 * kStageSteps, vrow - kc);  // vectors in this chunk (even)
-        // per point, 2 nv consecutive vectors (hi, lo of vectors kc..kc+nv-1):
-        // consecutive lanes read consecutive 16 bytes of one point's line
-        const int lgp = nv == 4 ? 3 : 2;  // log2(2 nv)
         __syncthreads();  // the previous chunk's readers are done
-        for (int q = tid; q < (2 * kMT) << lgp; q += kThreadsM) {
-            const int side = q >> (lgp + kMTLog);
-            const int rem = q & ((kMT << lgp) - 1);
-            const int pt = rem >> lgp, jv = rem & ((1 << lgp) - 1);
-            const long long g = (long long)min((side ? J : I) * kMT + pt, n - 1);
-            const bf16x8 v = xhl[2 * (g * vrow + kc) + jv];
-            if (side)
-                sB[jv][pt] = v;
-            else
-                sA[jv][pt] = v;
-        }
+        if (nv == 4)
+            stage_chunk<3>(xhl, I, J, n, vrow, kc, tid, sA, sB);
+        else
+            stage_chunk<2>(xhl, I, J, n, vrow, kc, tid, sA, sB);
         __syncthreads();
         if (below) continue;
         for (int ks = 0; ks < nv / 2; ks++) {
