@@ -106,7 +106,7 @@ __device__ __forceinline__ void tile_coords(long long t, int* I, int* J) {
 // LDS write waits (one memory latency per group, not per load).  Point
 // indices pass through an empty asm so that they are recomputed per chunk
 // rather than held across the k-loop beside the accumulators.
-constexpr int kStageGroup = 4;
+constexpr int kStageGroup = 8;
 
 template <int LGP>
 __device__ __forceinline__ void stage_chunk(const bf16x8* __restrict__ xhl, int I, int J, int n,
@@ -115,22 +115,23 @@ __device__ __forceinline__ void stage_chunk(const bf16x8* __restrict__ xhl, int 
     constexpr int kPts = kThreadsM >> LGP;   // points per pass
     constexpr int kPasses = kMT / kPts;      // passes per side
     constexpr int kLoads = 2 * kPasses;      // both sides
-    static_assert(kPasses * kPts == kMT && kLoads % kStageGroup == 0, "whole groups");
+    constexpr int kGroup = kLoads < kStageGroup ? kLoads : kStageGroup;
+    static_assert(kPasses * kPts == kMT && kLoads % kGroup == 0, "whole groups");
     const int jv = tid & ((1 << LGP) - 1), p0 = tid >> LGP;
     const bf16x8* src = xhl + 2 * kc + jv;
     const long long stride = 2LL * vrow;
 #pragma unroll
-    for (int f0 = 0; f0 < kLoads; f0 += kStageGroup) {
-        bf16x8 v[kStageGroup];
+    for (int f0 = 0; f0 < kLoads; f0 += kGroup) {
+        bf16x8 v[kGroup];
 #pragma unroll
-        for (int u = 0; u < kStageGroup; u++) {
+        for (int u = 0; u < kGroup; u++) {
             const int f = f0 + u;
             int g = min((f < kPasses ? I : J) * kMT + p0 + (f % kPasses) * kPts, n - 1);
             asm volatile("" : "+v"(g));
             v[u] = src[(long long)g * stride];
         }
 #pragma unroll
-        for (int u = 0; u < kStageGroup; u++) {
+        for (int u = 0; u < kGroup; u++) {
             const int f = f0 + u;
             bf16x8(*dst)[kMT + 1] = f < kPasses ? sA : sB;
             dst[jv][p0 + (f % kPasses) * kPts] = v[u];
