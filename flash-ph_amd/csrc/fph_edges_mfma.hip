@@ -157,22 +157,16 @@ __global__ __launch_bounds__(kThreadsM, 4) void k_edge_screen_mfma(
     // a contiguous run of tiles (shared row blocks stay in its L2).
     const long long nb = gridDim.x;
     const long long per_xcd = (nb + 7) / 8;
-    long long t = (long long)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    const bool live = t < ntiles;
-    if (!live) t = 0;
+    const long long t = (long long)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (t >= ntiles) return;  // the whole workgroup, before any barrier
     int I, J;
     tile_coords(t, &I, &J);
     const int tid = threadIdx.x;
+    // this thread's scaled norm: read here, written to LDS with the first
+    // chunk's operands, so both loads are in flight together
     const float one_a = 1.0f - marg_a;
-    if (tid < kMT) {
-        const int i = min(I * kMT + tid, n - 1);
-        s_nr[tid] = one_a * nrm[i];
-    } else {
-        const int j = min(J * kMT + tid - kMT, n - 1);
-        s_nc[tid - kMT] = one_a * nrm[j];
-    }
-    __syncthreads();
-    if (!live) return;
+    const float my_nrm =
+        one_a * nrm[min(tid < kMT ? I * kMT + tid : J * kMT + tid - kMT, n - 1)];
     const int w = tid / FPH_WAVE, lane = fph_lane();
     const int rsub = (w >> 1) * 64, csub = (w & 1) * 64;
     // entirely below the diagonal: stages operands with the others, then leaves
@@ -194,6 +188,12 @@ __global__ __launch_bounds__(kThreadsM, 4) void k_edge_screen_mfma(
             stage_chunk<3>(xhl, I, J, n, vrow, kc, tid, sA, sB);
         else
             stage_chunk<2>(xhl, I, J, n, vrow, kc, tid, sA, sB);
+        if (kc == 0) {  // the loop runs at least once: vrow >= 2
+            if (tid < kMT)
+                s_nr[tid] = my_nrm;
+            else
+                s_nc[tid - kMT] = my_nrm;
+        }
         __syncthreads();
         if (below) continue;
         for (int ks = 0; ks < nv / 2; ks++) {
