@@ -223,11 +223,33 @@ __global__ __launch_bounds__(kThreadsM, 4) void k_edge_screen_mfma(
     const float lim = thr + marg_b;
     const bool interior = I < J && (J + 1) * kMT <= n;
     const float cn0 = s_nc[csub + r], cn1 = s_nc[csub + 32 + r];
+    // Candidates are rare: a first pass takes the minimum of the tested
+    // values (any value < lim makes the minimum < lim; NaN values, never
+    // hits, are skipped by fminf), and a wave with no hit leaves.  The second
+    // pass evaluates the same expression per pair.  The running minimum is
+    // materialised every 8 pairs so that the 64 values are never live at
+    // once beside the accumulators (register budget).
+    float vmin = __builtin_inff();
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int v = 0; v < 16; v++) {
             const float rv = s_nr[rsub + 32 * a + 8 * (v >> 2) + 4 * kh + (v & 3)];
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+                vmin = fminf(vmin, __builtin_fmaf(-2.0f, acc[a][b][v], rv + (b ? cn1 : cn0)));
+            if ((v & 3) == 3) asm volatile("" : "+v"(vmin));
+        }
+    if (__ballot(vmin < lim) == 0ull) return;
+    // an opaque copy of the row offset: the second pass reloads the norms
+    // instead of keeping the first pass's values live (register budget)
+    int rsub2 = rsub;
+    asm volatile("" : "+v"(rsub2));
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const float rv = s_nr[rsub2 + 32 * a + 8 * (v >> 2) + 4 * kh + (v & 3)];
 #pragma unroll
             for (int b = 0; b < 2; b++) {
                 const bool hit = __builtin_fmaf(-2.0f, acc[a][b][v], rv + (b ? cn1 : cn0)) < lim;
