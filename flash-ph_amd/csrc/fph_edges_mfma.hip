@@ -38,6 +38,7 @@ constexpr int kMTLog = 7;
 constexpr int kThreadsM = 256; // 4 waves, each a 64 x 64 quarter of the tile
 constexpr int kStageSteps = 2; // 16-dimension k-steps staged in LDS per chunk
 static_assert(kMT == 1 << kMTLog, "tile side is a power of two");
+constexpr int kSuperLog = 4;   // tiles are taken in 16 x 16 super-tiles
 // A chunk holds 2 or 4 vectors of 8 coordinates per point (dpad is a
 // multiple of 16), so a point's hi and lo vectors of one chunk are 4 or 8
 // contiguous bf16x8: a power of two, decoded with shifts.
@@ -142,7 +143,7 @@ __device__ __forceinline__ void stage_chunk(const bf16x8* __restrict__ xhl, int 
 __global__ __launch_bounds__(kThreadsM, 4) void k_edge_screen_mfma(
     const float* __restrict__ pts, const bf16x8* __restrict__ xhl,
     const float* __restrict__ nrm, int n, int d, int dpad,
-    float thr, float marg_a, float marg_b, long long ntiles, unsigned long long* __restrict__ keys,
+    float thr, float marg_a, float marg_b, long long nslots, unsigned long long* __restrict__ keys,
     unsigned long long* __restrict__ vals, long long cap,
     unsigned long long* __restrict__ count, unsigned long long* __restrict__ ncand) {
     __shared__ float s_nr[kMT], s_nc[kMT];
@@ -153,14 +154,22 @@ __global__ __launch_bounds__(kThreadsM, 4) void k_edge_screen_mfma(
     // banks, and 8 lanes reading consecutive points of one row do too.
     __shared__ bf16x8 sA[4 * kStageSteps][kMT + 1];
     __shared__ bf16x8 sB[4 * kStageSteps][kMT + 1];
-    // XCD-aware order: the 8 XCDs take blockIdx round robin; give each XCD
-    // a contiguous run of tiles (shared row blocks stay in its L2).
+    // XCD-aware order: the 8 XCDs take blockIdx round robin; each XCD takes
+    // a contiguous run of slots.  Slots enumerate the 16 x 16 super-tiles of
+    // the upper triangle, then their tiles: the tiles of one super-tile run
+    // together on one XCD and share 16 row and 16 column blocks in its L2
+    // (a plain triangular order re-reads a row block for almost every tile).
     const long long nb = gridDim.x;
     const long long per_xcd = (nb + 7) / 8;
     const long long t = (long long)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (t >= ntiles) return;  // the whole workgroup, before any barrier
-    int I, J;
-    tile_coords(t, &I, &J);
+    if (t >= nslots) return;  // the whole workgroup, before any barrier
+    int SI, SJ;
+    tile_coords(t >> (2 * kSuperLog), &SI, &SJ);
+    const int l = (int)(t & ((1 << (2 * kSuperLog)) - 1));
+    const int I = (SI << kSuperLog) + (l & ((1 << kSuperLog) - 1));
+    const int J = (SJ << kSuperLog) + (l >> kSuperLog);
+    // below the diagonal or past the last tile: the whole workgroup leaves
+    if (I > J || J >= (n + kMT - 1) / kMT) return;
     const int tid = threadIdx.x;
     // this thread's scaled norm: read here, written to LDS with the first
     // chunk's operands, so both loads are in flight together
@@ -323,14 +332,15 @@ extern "C" int fph_edge_screen_mfma(const float* pts, int64_t n, int d, float th
     const float marg_a = (float)(1.25 * (cg + gamma_n(d) + 6.0 * u));
     const float marg_b = (float)(1.1 * gamma_n(d) * (double)thr_sq);
     const long long nts = (n + kMT - 1) / kMT;
-    const long long ntiles = nts * (nts + 1) / 2;
-    if (ntiles > 0x7FFFFFF0LL) return FPH_ERR_LIMIT;
-    // 8 x ceil(ntiles / 8) blocks: the XCD remap is a bijection onto
-    // [0, 8 ceil(ntiles / 8)); blocks past ntiles exit.
-    const long long nblk = (ntiles + 7) / 8 * 8;
+    const long long nsup = (nts + (1 << kSuperLog) - 1) >> kSuperLog;
+    const long long nslots = (nsup * (nsup + 1) / 2) << (2 * kSuperLog);
+    if (nslots > 0x7FFFFFF0LL) return FPH_ERR_LIMIT;
+    // 8 x ceil(nslots / 8) blocks: the XCD remap is a bijection onto
+    // [0, 8 ceil(nslots / 8)); blocks past nslots exit.
+    const long long nblk = (nslots + 7) / 8 * 8;
     k_edge_screen_mfma<<<(unsigned)nblk, kThreadsM, 0, s>>>(
         pts, (const bf16x8*)xhl, nrm, (int)n, d, dpad, thr_sq, marg_a, marg_b,
-        ntiles, (unsigned long long*)keys, (unsigned long long*)vals, cap,
+        nslots, (unsigned long long*)keys, (unsigned long long*)vals, cap,
         (unsigned long long*)count_dev, (unsigned long long*)cand_dev);
     FPH_LAUNCH_CHECK();
     return FPH_OK;
