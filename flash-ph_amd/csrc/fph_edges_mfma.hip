@@ -83,6 +83,8 @@ __global__ void k_norms(const float* __restrict__ pts, long long n, int d,
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float acc = 0.0f;
+    // 8 coordinates' loads in flight together; the FMA chain keeps its order
+#pragma unroll 8
     for (int k = 0; k < d; k++) {
         const float x = pts[i * d + k];
         acc = __builtin_fmaf(x, x, acc);
@@ -271,6 +273,9 @@ __global__ __launch_bounds__(kThreadsM, 4) void k_edge_screen_mfma(
                     const float* xi = pts + (long long)i * d;
                     const float* xj = pts + (long long)j * d;
                     float e = 0.0f;
+                    // unrolled so that 8 coordinates' loads are in flight
+                    // together; the FMA chain keeps its order k = 0..d-1
+#pragma unroll 8
                     for (int k = 0; k < d; k++) {
                         const float diff = xi[k] - xj[k];
                         e = __builtin_fmaf(diff, diff, e);
