@@ -129,7 +129,14 @@ __global__ __launch_bounds__(kThreads) void k_edge_screen(
             const int g = (side ? j0 : i0) + p;
             float* dst = (side ? fb : fa) + p;
             const float* src = pts + (long long)g * d + k0;
-            for (int k = 0; k < kc; k++) dst[k * TILE] = g < n ? src[k] : 0.0f;
+            if (g < n) {
+                // unrolled so that 8 coordinates' loads are in flight
+                // before the first LDS write waits for its data
+#pragma unroll 8
+                for (int k = 0; k < kc; k++) dst[k * TILE] = src[k];
+            } else {
+                for (int k = 0; k < kc; k++) dst[k * TILE] = 0.0f;
+            }
         }
         __syncthreads();
         for (int k = 0; k < kc; k++) {
