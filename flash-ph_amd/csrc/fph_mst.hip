@@ -35,7 +35,11 @@ __global__ void k_bvk_init(BvkState st, int n, unsigned char* __restrict__ mst, 
 }
 
 // Every crossing edge offers itself to both endpoint components (atomicMin on
-// rank); the hook pointers of the previous round are reset here.
+// rank); the hook pointers of the previous round are reset here.  min_e
+// starts the round at kNone (written by the previous kernel) and only
+// decreases, so a plain read returns a value >= the current minimum: when it
+// is already <= this edge's rank, the atomic could not change anything and is
+// skipped.
 __global__ void k_bvk_find(BvkState st, const int* __restrict__ ei, const int* __restrict__ ej,
                            long long E, int n, int round) {
     if (st.flags[0]) return;
@@ -46,8 +50,9 @@ __global__ void k_bvk_find(BvkState st, const int* __restrict__ ei, const int* _
     if (a < 0) return;  // unused slot of a capacity-sized edge list
     const int cu = st.comp[a], cv = st.comp[ej[i]];
     if (cu == cv) return;
-    atomicMin(&st.min_e[cu], (unsigned)i);
-    atomicMin(&st.min_e[cv], (unsigned)i);
+    const unsigned e = (unsigned)i;
+    if (st.min_e[cu] > e) atomicMin(&st.min_e[cu], e);
+    if (st.min_e[cv] > e) atomicMin(&st.min_e[cv], e);
     st.flags[1 + round] = 1;
 }
 
