@@ -140,14 +140,17 @@ constexpr int kBvkEdgesPerThread = 32;      // register-resident edges (CACHED)
 
 // One edge (a, b) of rank e during find: offers e to both components.
 // Returns true when the edge lies inside one component (it stays inside:
-// components only merge).
+// components only merge).  min_e is reset before the find phase and only
+// decreases during it, so a read never shows less than the current minimum:
+// when it is already <= e the atomic is skipped (offers to one large
+// component would otherwise serialise on its LDS word).
 __device__ __forceinline__ bool bvk_offer(const int* comp, unsigned* min_e, int a, int b,
                                           unsigned e, int& cross) {
     if (a < 0) return true;  // unused slot of a capacity-sized edge list
     const int cu = comp[a], cv = comp[b];
     if (cu == cv) return true;
-    atomicMin(&min_e[cu], e);
-    atomicMin(&min_e[cv], e);
+    if (min_e[cu] > e) atomicMin(&min_e[cu], e);
+    if (min_e[cv] > e) atomicMin(&min_e[cv], e);
     cross = 1;
     return false;
 }
