@@ -22,8 +22,10 @@
 //   cG = 3.1 * 2^-16 + 1.02 gamma(3 dpad).
 //   The FP32 norms add gamma(d) (|x|^2 + |y|^2), the FP32 operations of the
 //   test below <= 6u (|x|^2 + |y|^2), and the direct form differs from the
-//   true distance by <= gamma(d) dsq ~ gamma(d) t^2 near the threshold:
-//   a = 1.25 (cG + gamma(d) + 6u),  b = 1.1 gamma(d) t^2.
+//   true distance by <= gamma(d) dsq ~ gamma(d) t^2 near the threshold.
+//   All of it runs on centred points x' = fl(x - c) (c = the coordinate
+//   mean; see k_centre), whose rounding adds 2.1u to a and 2u t^2 to b:
+//   a = 1.25 (cG + gamma(d) + 8.1u),  b = 1.1 (gamma(d) + 2u) t^2.
 // The test is evaluated as fma(-2, G, (1-a)|x|^2 + (1-a)|y|^2) < t^2 + b
 // (the scaled norms staged once per tile): three operations per pair.
 #include "fph_common.h"
@@ -64,13 +66,13 @@ __device__ __forceinline__ float bf16_to_f32(unsigned short h) {
 // d).  Layout of xhl (bf16, 2 n dpad): per point, per vector of 8
 // coordinates, the hi vector then the lo vector, so one point's operands for
 // 4 k-steps of 16 coordinates are one 128-byte line.
-__global__ void k_split_bf16(const float* __restrict__ pts, long long n, int d, int dpad,
-                             unsigned short* __restrict__ xhl) {
+__global__ void k_split_bf16(const float* __restrict__ pts, const float* __restrict__ ctr,
+                             long long n, int d, int dpad, unsigned short* __restrict__ xhl) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * dpad) return;
     const long long i = t / dpad;
     const int k = (int)(t - i * dpad);
-    const float x = k < d ? pts[i * d + k] : 0.0f;
+    const float x = k < d ? pts[i * d + k] - ctr[k] : 0.0f;
     const unsigned short h = bf16_rne(x);
     const float r = x - bf16_to_f32(h);  // exact
     const long long at = 2 * (i * dpad + (k & ~7)) + (k & 7);
@@ -78,18 +80,47 @@ __global__ void k_split_bf16(const float* __restrict__ pts, long long n, int d, 
     xhl[at + 8] = bf16_rne(r);
 }
 
-__global__ void k_norms(const float* __restrict__ pts, long long n, int d,
-                        float* __restrict__ nrm) {
+__global__ void k_norms(const float* __restrict__ pts, const float* __restrict__ ctr,
+                        long long n, int d, float* __restrict__ nrm) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float acc = 0.0f;
     // 8 coordinates' loads in flight together; the FMA chain keeps its order
 #pragma unroll 8
     for (int k = 0; k < d; k++) {
-        const float x = pts[i * d + k];
+        const float x = pts[i * d + k] - ctr[k];
         acc = __builtin_fmaf(x, x, acc);
     }
     nrm[i] = acc;
+}
+
+// Centre of the cloud, for the screen only: distances do not change under
+// translation, but the margin grows with |x|^2 + |y|^2, so the Gram
+// products are formed on x - c with c the coordinate mean (any c keeps the
+// bound valid; the mean keeps the norms, hence the margin band, small for a
+// cloud far from the origin).  Partial sums per block of rows in FP64, then
+// a fixed-order sum per coordinate: deterministic.
+constexpr int kCtrBlocks = 256;
+
+__global__ void k_centre_partial(const float* __restrict__ pts, long long n, int d,
+                                 double* __restrict__ part) {
+    const long long rows = (n + gridDim.x - 1) / gridDim.x;
+    const long long r0 = (long long)blockIdx.x * rows;
+    const long long r1 = r0 + rows < n ? r0 + rows : n;
+    for (int k = threadIdx.x; k < d; k += blockDim.x) {
+        double acc = 0.0;
+        for (long long i = r0; i < r1; i++) acc += (double)pts[i * d + k];
+        part[(long long)blockIdx.x * d + k] = acc;
+    }
+}
+
+__global__ void k_centre(const double* __restrict__ part, int nb, long long n, int d,
+                         float* __restrict__ ctr) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d) return;
+    double acc = 0.0;
+    for (int b = 0; b < nb; b++) acc += part[(long long)b * d + k];
+    ctr[k] = (float)(acc / (double)n);
 }
 
 __device__ __forceinline__ void tile_coords(long long t, int* I, int* J) {
@@ -319,23 +350,34 @@ extern "C" int fph_edge_screen_mfma(const float* pts, int64_t n, int d, float th
     if (n > 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
     const int dpad = (d + 15) / 16 * 16;
     const size_t split = align_up(4 * (size_t)std::max<int64_t>(n, 1) * dpad);
-    const size_t need = split + align_up(4 * (size_t)std::max<int64_t>(n, 1));
+    const size_t nrm_b = align_up(4 * (size_t)std::max<int64_t>(n, 1));
+    const size_t ctr_b = align_up(4 * (size_t)d);
+    const size_t part_b = align_up(8 * (size_t)kCtrBlocks * d);
+    const size_t need = split + nrm_b + ctr_b + part_b;
     if (!ws) { *ws_bytes = need; return FPH_OK; }
     if (*ws_bytes < need) return FPH_ERR_ARG;
     char* w = (char*)ws;
     unsigned short* xhl = (unsigned short*)w;
     float* nrm = (float*)(w + split);
+    float* ctr = (float*)(w + split + nrm_b);
+    double* part = (double*)(w + split + nrm_b + ctr_b);
     k_zero2<<<1, 64, 0, s>>>((unsigned long long*)count_dev, (unsigned long long*)cand_dev);
     FPH_LAUNCH_CHECK();
     if (n < 2) return FPH_OK;
-    k_split_bf16<<<fph_blocks(n * dpad, 256), 256, 0, s>>>(pts, n, d, dpad, xhl);
+    k_centre_partial<<<kCtrBlocks, 64, 0, s>>>(pts, n, d, part);
+    k_centre<<<fph_blocks(d, 64), 64, 0, s>>>(part, kCtrBlocks, n, d, ctr);
+    k_split_bf16<<<fph_blocks(n * dpad, 256), 256, 0, s>>>(pts, ctr, n, d, dpad, xhl);
     FPH_LAUNCH_CHECK();
-    k_norms<<<fph_blocks(n, 256), 256, 0, s>>>(pts, n, d, nrm);
+    k_norms<<<fph_blocks(n, 256), 256, 0, s>>>(pts, ctr, n, d, nrm);
     FPH_LAUNCH_CHECK();
+    // The bound of the header, on x' = fl(x - c): |x'_k - (x_k - c_k)| <=
+    // u' |x'_k| (u' = u / (1 - u)), so | |x'-y'|^2 - |x-y|^2 | <= u' |x-y|^2 +
+    // (2u' + 2u'^2)(|x'|^2 + |y'|^2): 2.1u more in a, and |x-y|^2's factor
+    // (1 + u') near t^2 adds u t^2 to b (2u t^2 is used).
     const double cg = 3.1 * std::ldexp(1.0, -16) + 1.02 * gamma_n(3 * dpad);
     const double u = std::ldexp(1.0, -24);
-    const float marg_a = (float)(1.25 * (cg + gamma_n(d) + 6.0 * u));
-    const float marg_b = (float)(1.1 * gamma_n(d) * (double)thr_sq);
+    const float marg_a = (float)(1.25 * (cg + gamma_n(d) + 8.1 * u));
+    const float marg_b = (float)(1.1 * (gamma_n(d) + 2.0 * u) * (double)thr_sq);
     const long long nts = (n + kMT - 1) / kMT;
     const long long nsup = (nts + (1 << kSuperLog) - 1) >> kSuperLog;
     const long long nslots = (nsup * (nsup + 1) / 2) << (2 * kSuperLog);

@@ -407,14 +407,12 @@ extern "C" size_t fph_boruvka_scratch_bytes(int64_t n) {
     return 4 * align_up(4 * (size_t)std::max<int64_t>(n, 1)) + align_up(4 * (size_t)(R + 2));
 }
 
-extern "C" int fph_boruvka_mst(const int32_t* edge_i, const int32_t* edge_j, int64_t E,
-                               int64_t n, uint8_t* mst_mask, int64_t* comp_label,
-                               int32_t* converged, void* scratch, size_t scratch_bytes,
-                               void* stream) {
-    hipStream_t s = (hipStream_t)stream;
-    if (E < 0 || n < 1 || !scratch) return FPH_ERR_ARG;
-    if (n > 0x7FFFFFFFLL || E >= 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
-    if (scratch_bytes < fph_boruvka_scratch_bytes(n)) return FPH_ERR_ARG;
+// Issues the whole Boruvka forest on a stream: the one-workgroup form for
+// small complexes (no scratch), else the round kernels with their state in
+// `w` (fph_boruvka_scratch_bytes(n) bytes, device memory).
+static int bvk_issue(const int32_t* edge_i, const int32_t* edge_j, int64_t E, int64_t n,
+                     uint8_t* mst_mask, int64_t* comp_label, int32_t* converged, char* w,
+                     hipStream_t s) {
     const int R = fph_boruvka_rounds(n);
     if (n <= kBvkLdsMaxN && E <= kBvkLdsMaxE) {
         const int lds_b = 12 * (int)n;
@@ -430,7 +428,6 @@ extern "C" int fph_boruvka_mst(const int32_t* edge_i, const int32_t* edge_j, int
         FPH_LAUNCH_CHECK();
         return FPH_OK;
     }
-    char* w = (char*)scratch;
     const size_t vb = align_up(4 * (size_t)n);
     BvkState st;
     st.comp = (int*)w;
@@ -438,6 +435,8 @@ extern "C" int fph_boruvka_mst(const int32_t* edge_i, const int32_t* edge_j, int
     st.min_e = (unsigned*)(w + 2 * vb);
     unsigned* lab = (unsigned*)(w + 3 * vb);
     st.flags = (int*)(w + 4 * vb);
+    // k_bvk_init sets comp, nxt, min_e and every round flag; lab is set by
+    // k_bvk_minlabel_init: nothing in `w` needs to start at zero.
     const long long span = std::max<long long>(std::max<long long>(E, n), R + 1);
     k_bvk_init<<<fph_blocks(span, 256), 256, 0, s>>>(st, (int)n, mst_mask, E, R);
     FPH_LAUNCH_CHECK();
@@ -456,6 +455,17 @@ extern "C" int fph_boruvka_mst(const int32_t* edge_i, const int32_t* edge_j, int
                                                   st.flags, R, converged);
     FPH_LAUNCH_CHECK();
     return FPH_OK;
+}
+
+extern "C" int fph_boruvka_mst(const int32_t* edge_i, const int32_t* edge_j, int64_t E,
+                               int64_t n, uint8_t* mst_mask, int64_t* comp_label,
+                               int32_t* converged, void* scratch, size_t scratch_bytes,
+                               void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (E < 0 || n < 1 || !scratch) return FPH_ERR_ARG;
+    if (n > 0x7FFFFFFFLL || E >= 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
+    if (scratch_bytes < fph_boruvka_scratch_bytes(n)) return FPH_ERR_ARG;
+    return bvk_issue(edge_i, edge_j, E, n, mst_mask, comp_label, converged, (char*)scratch, s);
 }
 
 extern "C" int fph_h0_diagram(const uint8_t* mst_mask, const float* edge_dsq, int64_t E,

@@ -1952,7 +1952,8 @@ __device__ __forceinline__ void serial_bitmap(RedArgs& a, SerShared& sh, const u
 
 __global__ __launch_bounds__(kSerThreads) void k_reduce_serial(
     RedArgs a, const unsigned* __restrict__ order, const int* __restrict__ res_count,
-    const int* __restrict__ seg_cols, const long long* __restrict__ seg_pool, int variant) {
+    const int* __restrict__ seg_cols, const long long* __restrict__ seg_pool, int variant,
+    int maxcol_ref) {
     extern __shared__ unsigned smem[];
     __shared__ SerShared sh;
     unsigned* half0 = smem;
@@ -2029,7 +2030,7 @@ __global__ __launch_bounds__(kSerThreads) void k_reduce_serial(
     const int rows = (int)sp[3];
     // variant bit 0: sorted-list working column; bit 1: the complete
     // reduction (no store capacity, no MAX_COL truncation)
-    const int maxcol = (variant & 2) ? 0x7FFFFFFF : kSerialMaxCol;
+    const int maxcol = (variant & 2) ? 0x7FFFFFFF : maxcol_ref;
     if ((variant & 1) == 0 && rows > 0) {
         const int W = (rows + 31) / 32, S = (W + 31) / 32;
         for (int i = t; i < S; i += kSerThreads) smem[i] = 0u;
@@ -2277,10 +2278,12 @@ extern "C" int fph_reduce_serial(int64_t K, const int32_t* col_off, const uint32
                                  const int32_t* res_list, const int32_t* res_count,
                                  uint32_t* pool, int64_t pool_cap, void* scratch,
                                  size_t scratch_bytes, int32_t* status, int variant,
-                                 void* stream) {
+                                 int64_t max_col, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     if (K < 0 || nrows < 0 || pool_cap < 0 || !scratch || !status || nseg < 1 || !seg_pool)
         return FPH_ERR_ARG;
+    if (max_col == 0) max_col = kSerialMaxCol;
+    if (max_col < 1 || max_col > 0x7FFFFFFFLL) return FPH_ERR_ARG;
     if (nseg > 1 && !seg_cols) return FPH_ERR_ARG;
     if (scratch_bytes < fph_reduce_scratch_bytes(K)) return FPH_ERR_ARG;
     if (pool_start < 0 || pool_start > pool_cap || pool_cap > 0xFFFFFFFFLL) return FPH_ERR_ARG;
@@ -2321,7 +2324,7 @@ extern "C" int fph_reduce_serial(int64_t K, const int32_t* col_off, const uint32
                                           hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)kSerLds));
         k_reduce_serial<<<(unsigned)nseg, kSerThreads, kSerLds, s>>>(
-            a, order, res_count, seg_cols, (const long long*)seg_pool, variant);
+            a, order, res_count, seg_cols, (const long long*)seg_pool, variant, (int)max_col);
         FPH_LAUNCH_CHECK();
     }
     k_reduce_fini<<<1, 64, 0, s>>>(status, a.pool_ptr);

@@ -28,10 +28,15 @@ _POOL_MIN_WORDS = 1 << 20
 # reduction's second phase gives up (status 5).
 _WAIT_SECONDS = 120
 # Entry cap of the reference's reduction store (_numba_utils.py:11).
-_REF_MAX_POOL = 100_000_000
+REF_MAX_POOL = 100_000_000
 # Rows of a working column beyond which the reference truncates it
-# (_numba_utils.py:10).
-_REF_MAX_COL = 1_048_576
+# (_numba_utils.py:10).  Both limits are module attributes so that tests can
+# lower them and reach the reference's store-drop and truncation branches
+# on small clouds; the reducers take them as arguments.
+REF_MAX_COL = 1_048_576
+# Largest reduction pool the retry loop may request (32-bit words, 64 GiB):
+# past it a reduction fails with FlashPHError instead of asking for more.
+MAX_POOL_WORDS = 1 << 34
 # Reference semantics, one cloud: run the parallel reduction and replay the
 # reference's additions from its final owners (fph_reduce_replay); its pairs
 # are used when the reference's store would hold every column, else the
@@ -189,6 +194,8 @@ class Complex:
 def build_edges(pts: torch.Tensor, max_edge_length: float) -> Complex:
     """Stage (a): screened edges in filtration order plus the vertex CSR."""
     lib = load()
+    if pts.dtype != torch.float32 or not pts.is_contiguous():
+        raise ValueError("build_edges needs a contiguous float32 (n, d) tensor")
     dev = pts.device
     s = stream_of(dev)
     n, d = pts.shape
@@ -379,7 +386,7 @@ def _set_h1(cx: Complex, res, batch: bool) -> None:
 def reference_pool_caps(sizes) -> list:
     """Store capacity of the reference's reducer per segment, from
     (total cofacet entries, columns) (reduce_h1_gpu.py:286-289)."""
-    return [min(max(2 * cof, 16 * k, 100_000), _REF_MAX_POOL) for cof, k in sizes]
+    return [min(max(2 * cof, 16 * k, 100_000), REF_MAX_POOL) for cof, k in sizes]
 
 
 def _round32(x: int) -> int:
@@ -489,7 +496,7 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
                                         nseg, ptr(seg_pool_t), pool_start, ptr(kind), ptr(own),
                                         ptr(pivot), ptr(res_list), rescnt_p, ptr(pool),
                                         pool_words, ptr(scratch), scratch_bytes, status_p,
-                                        state["variant"], s), "fph_reduce_serial")
+                                        state["variant"], REF_MAX_COL, s), "fph_reduce_serial")
         TIMER.stop(tok)
         call_ws("fph_diagram",
                 (K, ptr(kind), ptr(off), ptr(data), ptr(pivot), ptr(col_edge), row_base,
@@ -505,7 +512,12 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
             if st[0] == 0:
                 break
             if st[0] == 1:
-                state["pool_words"] = max(2 * state["pool_words"], 32 * st[5] + (32 * st[5]) // 4)
+                want = max(2 * state["pool_words"], 32 * st[5] + (32 * st[5]) // 4)
+                if want > MAX_POOL_WORDS:
+                    raise _lib.FlashPHError(
+                        f"reduction pool would need {want} words (> {MAX_POOL_WORDS}); "
+                        "reduce max_edge_length")
+                state["pool_words"] = want
             elif st[0] == 4 and ref_sizes is not None and (state["variant"] & 1) == 0:
                 state["variant"] |= 1  # a column would pass the reference's MAX_COL: list form
             else:
@@ -522,7 +534,7 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
             stats["replay"] = {"status": r[0], "xors": r[2], "longest": r[3], "held": held,
                                "held_columns": r[6], "cap": replay_cap}
             stats["replay_ok"] = (r[0] == 0 and (r[1] == 1 or K == 0) and held <= replay_cap
-                                  and r[3] <= _REF_MAX_COL and r[6] <= K)
+                                  and r[3] <= REF_MAX_COL and r[6] <= K)
         counts = st[12:12 + nseg]
         starts = [0] if segs is None else segs["cols_host"][:-1]
         dgms = [out[a:a + c] for a, c in zip(starts, counts)]

@@ -58,8 +58,10 @@ int fph_edge_screen(const float *pts, int64_t n, int d, float thr_sq, uint64_t *
  * t^2 plus a rigorous error margin; candidates are recomputed in the direct
  * FP32 form and kept iff < thr_sq.  Same outputs as fph_edge_screen (same
  * pairs, same dsq bits, unordered); *cand_dev = candidates recomputed
- * (>= edges; the difference is the margin band).  Workspace: the bf16
- * splits and norms (4 n ceil16(d) + 4 n bytes).
+ * (>= edges; the difference is the margin band).  The Gram products are
+ * formed on coordinates centred on the cloud's mean, so a common offset
+ * does not widen the band.  Workspace: the bf16 splits, norms, centre and
+ * its partial sums (about 4 n ceil16(d) + 4 n + 2 KiB d bytes).
  * Replaces kernels/edge_kernel.py:23-224 (the reference computes the direct
  * form for every pair).
  */
@@ -249,12 +251,14 @@ int fph_reduce_replay(int64_t K, const int32_t *col_off, const uint32_t *col_dat
  * column, or -1 when ceil(rows/32) + ceil(rows/1024) <= 32768 (kept in LDS).
  * pool_start: first pool word after the stores and bitmaps.  variant bit 0:
  * 0 = working column as a bitmap (status[0] = 4 when a column would exceed
- * 1048576 rows: rerun with bit 0 set), 1 = as a sorted list.  Bit 1 (with
+ * max_col rows: rerun with bit 0 set), 1 = as a sorted list.  Bit 1 (with
  * seg_pool[0] = -1, capacity ignored): the complete reduction instead --
  * every column stored (bump-allocated in the pool), no truncation; the
  * pairs of fph_reduce, in the reference's serial order (faster than
- * fph_reduce on dense complexes with long columns).  status as for
- * fph_reduce.
+ * fph_reduce on dense complexes with long columns).  max_col: rows kept by
+ * a column addition in the reference semantics, 0 = the reference's
+ * MAX_COL of 1048576 (a smaller value reproduces the truncation branch on
+ * small complexes, for tests).  status as for fph_reduce.
  * Replaces _general_cohomology_reduce_v2 (cohomology_general.py:202-371)
  * with the pool sizing of reduce_h1_gpu.py:282-289 and the prefill of
  * _numba_utils.py:68-104, including their capacity limits.
@@ -265,7 +269,7 @@ int fph_reduce_serial(int64_t K, const int32_t *col_off, const uint32_t *col_dat
                       uint64_t *own, int32_t *pivot, const int32_t *res_list,
                       const int32_t *res_count, uint32_t *pool, int64_t pool_cap,
                       void *scratch, size_t scratch_bytes, int32_t *status, int variant,
-                      void *stream);
+                      int64_t max_col, void *stream);
 
 /* ---- diagrams --------------------------------------------------------- */
 

@@ -353,7 +353,7 @@ static int64_t xor_merge(const int32_t *a, int64_t na, const int32_t *b, int64_t
 static void reduce_columns(int64_t K, const int32_t *col_rank, const int32_t *off,
                            const int32_t *data, const uint8_t *skip,
                            const float *r2f, int64_t total, uint8_t *apparent_out,
-                           ReduceOut *ro, int exact) {
+                           ReduceOut *ro, int exact, int64_t max_col, int64_t max_pool) {
     memset(ro, 0, sizeof(*ro));
     int32_t *cmf = xmalloc((size_t)nz(total) * 4);
     for (int64_t r = 0; r < total; r++) cmf[r] = -1;
@@ -366,7 +366,7 @@ static void reduce_columns(int64_t K, const int32_t *col_rank, const int32_t *of
     int64_t pool_cap = 2 * total_cof;
     if (K * 16 > pool_cap) pool_cap = K * 16;
     if (100000 > pool_cap) pool_cap = 100000;
-    if (pool_cap > ORC_MAX_POOL) pool_cap = ORC_MAX_POOL;
+    if (pool_cap > max_pool) pool_cap = max_pool;
     /* exact: a store without capacity limit (grown on demand) and no column
      * truncation -- the complete reduction (the pairs giotto-ph returns). */
     int64_t store_cap = pool_cap;
@@ -438,8 +438,8 @@ static void reduce_columns(int64_t K, const int32_t *col_rank, const int32_t *of
     int64_t maxrow = 16;
     for (int64_t k = 0; k < K; k++)
         if (off[k + 1] - off[k] > maxrow) maxrow = off[k + 1] - off[k];
-    int64_t lim = exact ? INT64_MAX : ORC_MAX_COL;
-    int64_t wcap = ORC_MAX_COL;
+    int64_t lim = exact ? INT64_MAX : max_col;
+    int64_t wcap = max_col > ORC_MAX_COL ? max_col : ORC_MAX_COL;
     if (exact && total + 1 > wcap) wcap = total + 1;
     int32_t *cur = xmalloc((size_t)wcap * 4), *tmp = xmalloc((size_t)wcap * 4);
     int32_t *carry = xmalloc((size_t)wcap * 4);
@@ -541,8 +541,24 @@ static int csr_lookup(const int32_t *ptr, const int32_t *idx, const int32_t *ran
  * store (reduce_h1_gpu.py:438-604); for max_dim = 2 the reference hands H1
  * and H2 to giotto-ph (persistence.py:197-206), whose complete reduction is
  * restated by the exact store.  semantics 1: exact everywhere. */
-OrcResult *orc_run(const float *pts, int64_t n, int d, float thr_sq, int max_dim,
-                   int semantics) {
+/*
+ * orc_run_ex: orc_run with the reference's two reduction limits as
+ * arguments (max_col rows per column addition, _numba_utils.py:10; max_pool
+ * store entries, _numba_utils.py:11; 0 = the reference's values), and
+ * optionally starting from a given edge set instead of the O(n^2 d)
+ * enumeration: in_E >= 0 edges (in_i < in_j, in_dsq) in any order.  They
+ * are first put into the reference's emission order (tile (i/32, j/32)
+ * row-major, then (i mod 32, j mod 32), kernels/edge_kernel.py:133-149,
+ * 199-210) -- the order the enumeration below produces -- so that the key
+ * sort's ties resolve as the reference's.  pts is still needed for
+ * nothing else than n; it may be NULL when in_E >= 0.
+ */
+OrcResult *orc_run_ex(const float *pts, int64_t n, int d, float thr_sq, int max_dim,
+                      int semantics, const int32_t *in_i, const int32_t *in_j,
+                      const float *in_dsq, int64_t in_E, int64_t max_col,
+                      int64_t max_pool) {
+    if (max_col <= 0) max_col = ORC_MAX_COL;
+    if (max_pool <= 0) max_pool = ORC_MAX_POOL;
     const int exact1 = semantics == 1 || max_dim >= 2;
     OrcResult *r = calloc(1, sizeof(OrcResult));
     if (!r) abort();
@@ -554,7 +570,25 @@ OrcResult *orc_run(const float *pts, int64_t n, int d, float thr_sq, int max_dim
     }
     /* --- edges: emission order, then stable sort by packed key --- */
     int32_t *ui, *uj; float *ud;
-    int64_t E = enumerate_edges(pts, n, d, thr_sq, &ui, &uj, &ud);
+    int64_t E;
+    if (in_E >= 0) {
+        E = in_E;
+        const uint64_t nb = (uint64_t)((n + 31) / 32);
+        uint64_t *ek = xmalloc((size_t)nz(E) * 8);
+        for (int64_t e = 0; e < E; e++) {
+            const uint64_t i = (uint64_t)in_i[e], j = (uint64_t)in_j[e];
+            ek[e] = (((i >> 5) * nb + (j >> 5)) << 10) | ((i & 31) << 5) | (j & 31);
+        }
+        int64_t *ep = stable_argsort_u64(ek, E);
+        ui = xmalloc((size_t)nz(E) * 4); uj = xmalloc((size_t)nz(E) * 4);
+        ud = xmalloc((size_t)nz(E) * 4);
+        for (int64_t e = 0; e < E; e++) {
+            ui[e] = in_i[ep[e]]; uj[e] = in_j[ep[e]]; ud[e] = in_dsq[ep[e]];
+        }
+        free(ek); free(ep);
+    } else {
+        E = enumerate_edges(pts, n, d, thr_sq, &ui, &uj, &ud);
+    }
     r->stats[S_E] = E;
     uint64_t *key = xmalloc((size_t)nz(E) * 8);
     for (int64_t e = 0; e < E; e++) key[e] = edge_sort_key(ud[e], ui[e], uj[e]);
@@ -729,7 +763,8 @@ OrcResult *orc_run(const float *pts, int64_t n, int d, float thr_sq, int max_dim
     for (int64_t e = 0; e < E; e++) colr[e] = (int32_t)e;
     uint8_t *app1 = xmalloc((size_t)E);
     ReduceOut ro1;
-    reduce_columns(E, colr, coff, cdat, mst_mask, r2f, total1, app1, &ro1, exact1);
+    reduce_columns(E, colr, coff, cdat, mst_mask, r2f, total1, app1, &ro1, exact1, max_col,
+                   max_pool);
     arr_set(r, A_APPARENT1, app1, E, 1);
     r->stats[S_RES1] = ro1.residual;
     r->stats[S_XOR1] = ro1.xors;
@@ -874,7 +909,8 @@ OrcResult *orc_run(const float *pts, int64_t n, int d, float thr_sq, int max_dim
     /* CSR rows are indexed by local triangle rank; reorder data accordingly */
     uint8_t *app2 = xmalloc((size_t)T);
     ReduceOut ro2;
-    reduce_columns(T, col2, c2off, c2dat, skip2, r2f2, total2, app2, &ro2, 1);
+    reduce_columns(T, col2, c2off, c2dat, skip2, r2f2, total2, app2, &ro2, 1, max_col,
+                   max_pool);
     arr_set(r, A_APPARENT2, app2, T, 1);
     r->stats[S_RES2] = ro2.residual;
     r->stats[S_XOR2] = ro2.xors;
@@ -893,6 +929,28 @@ OrcResult *orc_run(const float *pts, int64_t n, int d, float thr_sq, int max_dim
     free(tet_order);
     free(colr); free(r2f); free(edge_filt); free(mst_mask); free(tri_order);
     return r;
+}
+
+OrcResult *orc_run(const float *pts, int64_t n, int d, float thr_sq, int max_dim,
+                   int semantics) {
+    return orc_run_ex(pts, n, d, thr_sq, max_dim, semantics, NULL, NULL, NULL, -1, 0, 0);
+}
+
+/* Neighbours of vertex v (every u != v with FMA-chain dsq(min, max) < thr_sq,
+ * kernels/edge_kernel.py:112-125), ascending, with their dsq; returns the
+ * count (at most cap written).  For membership checks of one vertex's row
+ * on clouds too large for the full enumeration. */
+int64_t orc_vertex_neighbours(const float *pts, int64_t n, int d, float thr_sq, int64_t v,
+                              int32_t *out_idx, float *out_dsq, int64_t cap) {
+    int64_t m = 0;
+    for (int64_t u = 0; u < n; u++) {
+        if (u == v) continue;
+        const float dsq = u < v ? dist_sq_fma(pts, d, u, v) : dist_sq_fma(pts, d, v, u);
+        if (!(dsq < thr_sq)) continue;
+        if (m < cap) { out_idx[m] = (int32_t)u; out_dsq[m] = dsq; }
+        m++;
+    }
+    return m;
 }
 
 int64_t orc_len(const OrcResult *r, int id) {

@@ -62,6 +62,15 @@ def _load():
         lib.orc_run.restype = ctypes.c_void_p
         lib.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
                                 ctypes.c_float, ctypes.c_int, ctypes.c_int]
+        lib.orc_run_ex.restype = ctypes.c_void_p
+        lib.orc_run_ex.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                                   ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]
+        lib.orc_vertex_neighbours.restype = ctypes.c_int64
+        lib.orc_vertex_neighbours.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                                              ctypes.c_float, ctypes.c_int64, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_int64]
         lib.orc_len.restype = ctypes.c_int64
         lib.orc_len.argtypes = [ctypes.c_void_p, ctypes.c_int]
         lib.orc_get.restype = ctypes.c_int
@@ -88,10 +97,18 @@ def threshold_sq_f32(max_edge_length: float) -> np.float32:
     return np.float32(float(max_edge_length) * float(max_edge_length))
 
 
-def run(points, max_edge_length: float, max_dim: int = 1, exact: bool = False) -> dict:
+def run(points, max_edge_length: float, max_dim: int = 1, exact: bool = False,
+        edges=None, max_col: int = 0, max_pool: int = 0) -> dict:
     """exact=False: the reference's results (its capacity-limited H1 store
     for max_dim <= 1; the complete reduction giotto-ph performs for
-    max_dim = 2).  exact=True: the complete reduction everywhere."""
+    max_dim = 2).  exact=True: the complete reduction everywhere.
+
+    edges: optional (edge_i, edge_j, edge_dsq) arrays in any order -- the
+    pipeline then starts from this edge set (put into the reference's
+    emission order, then key-sorted) instead of enumerating all pairs, for
+    clouds too large for the O(n^2 d) enumeration.  max_col / max_pool: the
+    reference's MAX_COL and store-entry cap (_numba_utils.py:10-11; 0 = its
+    values), lowered by tests to reach the truncation and drop branches."""
     lib = _load()
     pts = np.ascontiguousarray(np.asarray(points, dtype=np.float32))
     if pts.ndim != 2:
@@ -104,8 +121,21 @@ def run(points, max_edge_length: float, max_dim: int = 1, exact: bool = False) -
         out["diagrams"] = [h0] + [np.empty((0, 2), np.float32) for _ in range(max_dim)]
         out["stats"] = {k: 0 for k in STATS}
         return out
-    h = lib.orc_run(pts.ctypes.data, n, d, float(threshold_sq_f32(max_edge_length)),
-                    int(max_dim), 1 if exact else 0)
+    keep = []
+    if edges is not None:
+        ei, ej, ed = (np.ascontiguousarray(np.asarray(a, dtype=t))
+                      for a, t in zip(edges, (np.int32, np.int32, np.float32)))
+        if not (ei.shape == ej.shape == ed.shape) or ei.ndim != 1:
+            raise ValueError("edges must be three 1-D arrays of one length")
+        if ei.size and not (ei < ej).all():
+            raise ValueError("edges must have edge_i < edge_j")
+        keep = [ei, ej, ed]
+        eargs = (ei.ctypes.data, ej.ctypes.data, ed.ctypes.data, ei.size)
+    else:
+        eargs = (None, None, None, -1)
+    h = lib.orc_run_ex(pts.ctypes.data, n, d, float(threshold_sq_f32(max_edge_length)),
+                       int(max_dim), 1 if exact else 0, *eargs, int(max_col), int(max_pool))
+    del keep
     try:
         for idx, name in enumerate(NAMES):
             ln = lib.orc_len(h, idx)
@@ -131,6 +161,21 @@ def dist_sq(points, i: int, j: int) -> np.float32:
     lib = _load()
     pts = np.ascontiguousarray(np.asarray(points, dtype=np.float32))
     return np.float32(lib.orc_dist_sq(pts.ctypes.data, pts.shape[1], i, j))
+
+
+def vertex_neighbours(points, max_edge_length: float, v: int):
+    """(neighbour ids ascending, their FMA-chain dsq) of vertex v: every u != v
+    with dsq < fp32(t^2).  O(n d) per call."""
+    lib = _load()
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float32))
+    n, d = pts.shape
+    thr = float(threshold_sq_f32(max_edge_length))
+    m = lib.orc_vertex_neighbours(pts.ctypes.data, n, d, thr, int(v), None, None, 0)
+    idx = np.empty(max(m, 1), dtype=np.int32)
+    dsq = np.empty(max(m, 1), dtype=np.float32)
+    lib.orc_vertex_neighbours(pts.ctypes.data, n, d, thr, int(v), idx.ctypes.data,
+                              dsq.ctypes.data, m)
+    return idx[:m], dsq[:m]
 
 
 def fmaf(a, b, c) -> np.float32:

@@ -136,3 +136,80 @@ def multiset(dgm) -> np.ndarray:
         return a
     idx = np.lexsort((a[:, 1], a[:, 0]))
     return a[idx]
+
+
+def reference_reduction(cof_off, cof_data, skip, r2f, max_col, max_pool):
+    """The reference's capacity-limited H1 reduction, written from its
+    source with Python containers (a dict for its hash map, lists for its
+    pool): apparent pre-detection and pool prefill (reduce_h1_gpu.py:159-223,
+    282-289; _numba_utils.py:68-104), then residual columns in descending
+    rank order, merged with the MAX_COL-limited sorted XOR
+    (_numba_utils.py:43-65) and stored only while the pool has room
+    (cohomology_general.py:255-366).  Columns are ranks 0..K-1 (edges),
+    rows are global ranks.  Returns (pairs [(birth, death)] in emission
+    order, essentials, drops, truncated merges)."""
+    K = len(cof_off) - 1
+    cols = [list(cof_data[cof_off[k]:cof_off[k + 1]]) for k in range(K)]
+    cmf = {}
+    for k in range(K):
+        if not skip[k]:
+            for r in cols[k]:
+                cmf[r] = max(cmf.get(r, -1), k)
+    total = cof_off[K]
+    cap = min(max(2 * total, 16 * K, 100_000), max_pool)
+    store = {}        # pivot row -> stored column
+    used = 0
+    pairs, ess = [], []
+    drops = truncs = 0
+    apparent = [not skip[k] and len(cols[k]) > 0 and cmf.get(cols[k][0]) == k
+                for k in range(K)]
+    for k in range(K):
+        if apparent[k]:
+            if r2f[cols[k][0]] > r2f[k]:
+                pairs.append((k, cols[k][0]))
+    for k in range(K):
+        if apparent[k]:
+            if used + len(cols[k]) <= cap and len(store) < K:
+                store[cols[k][0]] = list(cols[k])
+                used += len(cols[k])
+
+    def xor(a, b):
+        out, ia, ib = [], 0, 0
+        while ia < len(a) and ib < len(b):
+            if len(out) >= max_col:
+                break
+            if a[ia] < b[ib]:
+                out.append(a[ia]); ia += 1
+            elif a[ia] > b[ib]:
+                out.append(b[ib]); ib += 1
+            else:
+                ia += 1; ib += 1
+        # the tails, a's remainder first, each while room remains
+        tail = a[ia:] + b[ib:]
+        room = max_col - len(out)
+        out += tail[:max(room, 0)]
+        return out, len(tail) > max(room, 0)
+
+    for k in range(K - 1, -1, -1):
+        if skip[k] or apparent[k]:
+            continue
+        if not cols[k]:
+            ess.append(k)
+            continue
+        if len(cols[k]) > max_col:
+            truncs += 1
+        cur = cols[k][:max_col]
+        while cur and cur[0] in store:
+            cur, cut = xor(cur, store[cur[0]])
+            truncs += cut
+        if cur:
+            if used + len(cur) <= cap and len(store) < K:
+                store[cur[0]] = list(cur)
+                used += len(cur)
+            else:
+                drops += 1
+            if r2f[cur[0]] > r2f[k]:
+                pairs.append((k, cur[0]))
+        else:
+            ess.append(k)
+    return pairs, ess, drops, truncs

@@ -103,3 +103,76 @@ def batch_uneven(C, seed, nmin=200, nmax=5000, dims=(4, 10, 50), noise=0.05):
 def clifford_batch(C, n, seed, noise=0.02):
     """C Clifford-torus clouds (cos t, sin t, cos p, sin p)/sqrt(2) + noise."""
     return [clifford(n, seed + c, noise) for c in range(C)]
+
+
+def grid_points(m: int) -> np.ndarray:
+    """m^3 integer grid points in 3-D, x-major (exact in FP32, so every
+    squared distance is an exact small integer: massive dsq ties)."""
+    return np.ascontiguousarray(np.indices((m, m, m)).reshape(3, -1).T, dtype=np.float32)
+
+
+def grid_edges(pts: np.ndarray, t: float):
+    """(i, j, dsq) of every pair with dsq < fp32(t^2) on integer points, from
+    scipy's k-d tree (FP64 distances are exact here)."""
+    from scipy.spatial import cKDTree
+    pr = cKDTree(pts.astype(np.float64)).query_pairs(float(t) + 0.01, output_type="ndarray")
+    i = pr.min(axis=1).astype(np.int64)
+    j = pr.max(axis=1).astype(np.int64)
+    dsq = ((pts[i].astype(np.float64) - pts[j]) ** 2).sum(axis=1).astype(np.float32)
+    keep = dsq < np.float32(float(t) * float(t))
+    return i[keep].astype(np.int32), j[keep].astype(np.int32), dsq[keep]
+
+
+def reference_edge_order(i, j, dsq) -> np.ndarray:
+    """Permutation putting edges into the reference's filtration order,
+    written with numpy: emission order first -- tile (i // 32, j // 32)
+    row-major, then (i mod 32, j mod 32) (kernels/edge_kernel.py:133-149,
+    199-210) -- then a stable sort on the packed key
+    float_bits(dsq) << 32 | (j (j - 1) / 2 + i) mod 2^32 (rips.py:41-49,
+    189-195)."""
+    i = np.asarray(i, dtype=np.int64)
+    j = np.asarray(j, dtype=np.int64)
+    em = np.lexsort((j % 32, i % 32, j // 32, i // 32))
+    bits = np.asarray(dsq, dtype=np.float32).view(np.uint32).astype(np.int64)
+    key = (bits << 32) | ((j * (j - 1) // 2 + i) & 0xFFFFFFFF)
+    return em[np.argsort(key[em], kind="stable")]
+
+
+def wrapped_key_ties(i, j, dsq) -> int:
+    """Number of edges whose packed key equals another edge's (possible
+    only once the colex index passes 2^32, i.e. n > 92,682)."""
+    i = np.asarray(i, dtype=np.int64)
+    j = np.asarray(j, dtype=np.int64)
+    bits = np.asarray(dsq, dtype=np.float32).view(np.uint32).astype(np.int64)
+    key = (bits << 32) | ((j * (j - 1) // 2 + i) & 0xFFFFFFFF)
+    _, c = np.unique(key, return_counts=True)
+    return int(c[c > 1].sum())
+
+
+def near_threshold_pairs(m: int, d: int, t: float, seed: int, spread: int = 4):
+    """2m points: m N(0,1) points x_p and partners y_p placed so that the FP32
+    FMA-chain dsq(x_p, y_p) lies within a few ulp of fp32(t^2) on either side
+    (target offsets uniform in [-spread, spread] ulp, reached by rescaling
+    y_p - x_p a few times).  Returns (points, achieved ulp offsets)."""
+    import oracle
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((m, d)).astype(np.float32)
+    v = rng.standard_normal((m, d))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    t2 = np.float32(float(t) * float(t))
+    t2_bits = int(t2.view(np.int32))
+    targets = t2_bits + rng.integers(-spread, spread + 1, m)
+    y = np.empty_like(x)
+    offs = np.empty(m, dtype=np.int64)
+    for p in range(m):
+        s = float(t)
+        tgt = float(np.array(targets[p], dtype=np.int32).view(np.float32))
+        for _ in range(8):
+            yp = (x[p].astype(np.float64) + s * v[p]).astype(np.float32)
+            dsq = oracle.dist_sq(np.stack([x[p], yp]), 0, 1)
+            if float(dsq) == tgt:
+                break
+            s *= np.sqrt(tgt / float(dsq))
+        y[p] = yp
+        offs[p] = int(np.float32(dsq).view(np.int32)) - t2_bits
+    return np.concatenate([x, y]).astype(np.float32), offs

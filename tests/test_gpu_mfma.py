@@ -59,3 +59,41 @@ def test_mfma_pipeline_diagrams_and_fp64_error():
     d64 = ((x64[i] - x64[j]) ** 2).sum(1)
     rel = ((cx.edge_dsq[:cx.E].double() - d64).abs() / d64).max().item()
     assert rel < 50 * 2.0 ** -24
+
+
+@pytest.mark.parametrize("d", [50, 64, 70])
+def test_mfma_near_threshold_pairs(d):
+    """Pairs whose FP32 dsq lies within +-5 ulp of t^2 (both sides): the
+    split-bf16 screen keeps exactly the direct screen's and the oracle's
+    edges -- the error margin admits every pair the direct form keeps."""
+    import oracle
+    pts, offs = cases.near_threshold_pairs(300, d, 3.0, seed=d)
+    assert (offs < 0).sum() > 50 and (offs >= 0).sum() > 50
+    print(f"[gpu] near-threshold pairs d={d}", flush=True)
+    x = torch.from_numpy(pts).cuda()
+    a = _edges(x, 3.0, "direct")
+    b = _edges(x, 3.0, "mfma")
+    ref = oracle.run(pts, 3.0, 0)
+    assert a.E == b.E == ref["stats"]["E"]
+    for name in ("edge_i", "edge_j", "edge_dsq"):
+        got = getattr(b, name)[:b.E].cpu().numpy()
+        assert np.array_equal(got.view(np.int32), ref[name].view(np.int32)), name
+        assert torch.equal(getattr(a, name)[:a.E].view(torch.int32),
+                           getattr(b, name)[:b.E].view(torch.int32)), name
+
+
+def test_mfma_offset_cloud():
+    """A cloud far from the origin (N(0,1) + 1000 in every coordinate): the
+    screen works on centred coordinates, so the margin band stays small and
+    the edges equal the direct screen's."""
+    print("[gpu] mfma offset cloud", flush=True)
+    pts = (cases.randn(6000, 48, 21) + np.float32(1000.0)).astype(np.float32)
+    x = torch.from_numpy(pts).cuda()
+    a = _edges(x, 7.5, "direct")
+    b = _edges(x, 7.5, "mfma")
+    assert a.E == b.E and a.E > 1000
+    for name in ("edge_i", "edge_j", "edge_dsq"):
+        assert torch.equal(getattr(a, name)[:a.E].view(torch.int32),
+                           getattr(b, name)[:b.E].view(torch.int32)), name
+    cand = int(b.info["screen_candidates"].item())
+    assert b.E <= cand <= 2 * b.E + 1000, (cand, b.E)

@@ -354,3 +354,63 @@ def test_max_dim2_h1_is_complete_reduction():
     a = oracle.run(x, 1.1, 2)
     b = oracle.run(x, 1.1, 1, exact=True)
     assert np.array_equal(a["h1"].view(np.uint32), b["h1"].view(np.uint32))
+
+
+@pytest.mark.parametrize("max_col,max_pool", [(4, 0), (1 << 20, 60), (6, 80), (10, 300)])
+@pytest.mark.parametrize("seed", range(3))
+def test_reference_limits_match_python_restatement(seed, max_col, max_pool):
+    """The reference's MAX_COL truncation and store cap, lowered so that tiny
+    clouds reach them: the oracle's pairs and essentials equal a second,
+    independent restatement of the reference's loop (tests/brute.py)."""
+    rng = np.random.default_rng(seed)
+    pts = rng.standard_normal((60, 3)).astype(np.float32)
+    t = 1.3
+    o = oracle.run(pts, t, 1, max_col=max_col, max_pool=max_pool)
+    E, T = o["stats"]["E"], o["stats"]["T"]
+    mst = np.zeros(E, dtype=bool)
+    mst[o["mst_idx"]] = True
+    ef = oracle.sqrt_f32(o["edge_dsq"])
+    r2f = np.empty(E + T, dtype=np.float32)
+    r2f[:E] = ef
+    r2f[o["tri_global_rank"]] = ef[o["tri_max_rank"]]
+    pairs, ess, drops, truncs = brute.reference_reduction(
+        o["cof_off"], o["cof_data"], mst, r2f, max_col, max_pool or 100_000_000)
+    assert list(zip(o["h1_pb"].tolist(), o["h1_pd"].tolist())) == pairs
+    assert o["h1_eb"].tolist() == ess
+    if max_col < 1 << 20:
+        assert truncs > 0 and o["stats"]["truncations"] > 0
+    if max_pool:
+        assert drops > 0 and o["stats"]["pool_drops"] > 0
+
+
+@pytest.mark.parametrize("n,d,t,seed", [(700, 3, 0.5, 0), (300, 1, 0.05, 1), (2500, 2, 0.06, 2)])
+def test_edge_list_entry_matches_enumeration(n, d, t, seed):
+    """orc_run_ex from a shuffled edge list rebuilds the enumeration's
+    emission order: every output equals the full run's."""
+    rng = np.random.default_rng(seed)
+    pts = rng.standard_normal((n, d)).astype(np.float32)
+    if d == 1:
+        pts = np.round(pts * 50) / 50   # many equal distances
+    a = oracle.run(pts, t, 2)
+    p = rng.permutation(a["stats"]["E"])
+    b = oracle.run(pts, t, 2, edges=(a["edge_i"][p], a["edge_j"][p], a["edge_dsq"][p]))
+    for k in oracle.NAMES:
+        if k == "stats":
+            continue
+        assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k
+
+
+def test_wrapped_key_order_on_grid():
+    """n = 47^3 = 103,823 > 92,682: the packed colex field wraps, equal dsq
+    values are ordered by the wrapped colex, and fully tied keys by the
+    emission order.  The oracle's edge order equals numpy's restatement."""
+    pts = cases.grid_points(47)
+    i, j, dsq = cases.grid_edges(pts, 1.5)
+    assert cases.wrapped_key_ties(i, j, dsq) > 0
+    assert (j.astype(np.int64) * (j - 1) // 2 + i >= 2**32).any()
+    p = np.random.default_rng(0).permutation(i.size)
+    o = oracle.run(pts, 1.5, 0, edges=(i[p], j[p], dsq[p]))
+    order = cases.reference_edge_order(i, j, dsq)
+    assert np.array_equal(o["edge_i"], i[order])
+    assert np.array_equal(o["edge_j"], j[order])
+    assert np.array_equal(o["edge_dsq"].view(np.uint32), dsq[order].view(np.uint32))
