@@ -468,6 +468,62 @@ extern "C" int fph_boruvka_mst(const int32_t* edge_i, const int32_t* edge_j, int
     return bvk_issue(edge_i, edge_j, E, n, mst_mask, comp_label, converged, (char*)scratch, s);
 }
 
+// ---- allocation-free Boruvka ------------------------------------------------
+//
+// The round kernels' state for up to FPH_BORUVKA_NOALLOC_MAX_N vertices lives
+// in this file-scope array, one copy per GPU, held until the process ends.
+// Host code never passes `scratch` itself to a kernel (on the host it names a
+// stand-in, not device memory): each device's address comes from
+// hipGetSymbolAddress, taken once per device with that device current, and
+// only that address is used.
+namespace {
+constexpr size_t kNoallocBytes =
+    4 * (4 * (size_t)FPH_BORUVKA_NOALLOC_MAX_N) + 256;  // comp, nxt, min_e, lab + flags
+constexpr int kMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[kMaxDevices];
+
+// Device address of `scratch` on `dev` (looked up once per device).
+static int noalloc_scratch(int dev, char** out) {
+    if (dev < 0 || dev >= kMaxDevices) return FPH_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        int cur = -1;
+        FPH_CHECK_HIP(hipGetDevice(&cur));
+        if (cur != dev) FPH_CHECK_HIP(hipSetDevice(dev));
+        void* p = nullptr;
+        hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) {
+            hipError_t rc2 = hipSetDevice(cur);
+            if (rc == hipSuccess) rc = rc2;
+        }
+        if (rc != hipSuccess || !p) return FPH_ERR_HIP;
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = (char*)g_noalloc_ptr[dev];
+    return FPH_OK;
+}
+
+extern "C" int fph_boruvka_mst_noalloc(const int32_t* edge_i, const int32_t* edge_j, int64_t E,
+                                       int64_t n, uint8_t* mst_mask, int64_t* comp_label,
+                                       int32_t* converged, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (E < 0 || n < 1) return FPH_ERR_ARG;
+    if (n > FPH_BORUVKA_NOALLOC_MAX_N || E >= 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
+    if (fph_boruvka_scratch_bytes(n) > kNoallocBytes) return FPH_ERR_LIMIT;
+    int dev = -1;
+    if (s) FPH_CHECK_HIP(hipStreamGetDevice(s, &dev));
+    else FPH_CHECK_HIP(hipGetDevice(&dev));
+    char* w = nullptr;
+    const int rc = noalloc_scratch(dev, &w);
+    if (rc != FPH_OK) return rc;
+    return bvk_issue(edge_i, edge_j, E, n, mst_mask, comp_label, converged, w, s);
+}
+
 extern "C" int fph_h0_diagram(const uint8_t* mst_mask, const float* edge_dsq, int64_t E,
                               int64_t n, float* h0, int64_t* mst_idx, int64_t* n_mst,
                               void* ws, size_t* ws_bytes, void* stream) {

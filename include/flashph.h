@@ -110,6 +110,25 @@ int fph_boruvka_mst(const int32_t *edge_i, const int32_t *edge_j, int64_t E, int
                     void *scratch, size_t scratch_bytes, void *stream);
 
 /*
+ * fph_boruvka_mst without a scratch argument and without allocating: the
+ * round kernels' state lives in a static device array of the library
+ * (4 * 4 * FPH_BORUVKA_NOALLOC_MAX_N + 256 bytes = 4 MiB per GPU, held until
+ * the process ends; its address is looked up once per device).  Same
+ * outputs as fph_boruvka_mst.  FPH_ERR_LIMIT for n > FPH_BORUVKA_NOALLOC_MAX_N
+ * (checked before any HIP call).  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (two streams, or two host threads)
+ * race and must be serialised by the caller; the kernels initialise every
+ * word they read, so nothing of a previous call leaks into the next.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc).
+ */
+#define FPH_BORUVKA_NOALLOC_MAX_N 262144
+int fph_boruvka_mst_noalloc(const int32_t *edge_i, const int32_t *edge_j, int64_t E, int64_t n,
+                            uint8_t *mst_mask, int64_t *comp_label, int32_t *converged,
+                            void *stream);
+
+/*
  * H0 diagram: exactly n rows (n x 2 FP32): (0, sqrt(dsq)) for forest edges in
  * rank order, then (0, inf) per component.  mst_idx (int64, capacity n, may
  * be NULL) = forest edge indices ascending; *n_mst (device int64, may be NULL).
