@@ -320,6 +320,92 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(
     if (active) kth[vbase[c] + q] = n > 1 ? r : 0.0f;
 }
 
+// k-th smallest squared distance for k beyond the register lists
+// (k > FPH_KNN_MAX_K): one wavefront per query point selects it exactly by
+// radix on the FP32 bits (non-negative floats order as their bit patterns):
+// four passes over the cloud, each histogramming the next 8 bits of the
+// distances that carry the bits fixed so far (256 LDS counters per
+// wavefront) and fixing the byte where the cumulative count reaches the
+// remaining rank.  Distances are the same FMA chain as k_knn_batch, so the
+// result is the k-th entry of the same sorted list.
+constexpr int kSelWaves = 16;
+
+__global__ __launch_bounds__(kSelWaves * FPH_WAVE) void k_knn_select(
+    const float* __restrict__ pts, const long long* __restrict__ foff,
+    const int* __restrict__ dims, const long long* __restrict__ vbase, int C, long long N,
+    int k_req, float* __restrict__ kth) {
+    __shared__ unsigned hist[kSelWaves][256];
+    const int wv = threadIdx.x / FPH_WAVE, lane = fph_lane();
+    const long long g = (long long)blockIdx.x * kSelWaves + wv;  // global query id
+    if (g >= N) return;                                            // wave-uniform
+    const int c = find_seg(vbase, C, g);
+    const int n = (int)(vbase[c + 1] - vbase[c]);
+    const int q = (int)(g - vbase[c]);
+    const int d = dims[c];
+    const float* P = pts + foff[c];
+    const float* xq = P + (long long)q * d;
+    if (n <= 1) {
+        if (lane == 0) kth[g] = 0.0f;
+        return;
+    }
+    unsigned krem = (unsigned)max(1, min(k_req, n - 1));  // rank among the n-1 others
+    unsigned prefix = 0;
+    unsigned* h = hist[wv];
+    for (int pass = 0; pass < 4; pass++) {
+        const int shift = 24 - 8 * pass;
+        for (int b = lane; b < 256; b += FPH_WAVE) h[b] = 0u;
+        __builtin_amdgcn_wave_barrier();
+        for (int j = lane; j < n; j += FPH_WAVE) {
+            if (j == q) continue;
+            const float* y = P + (long long)j * d;
+            float acc = 0.0f;
+            for (int x = 0; x < d; x++) {
+                const float diff = xq[x] - y[x];
+                acc = __builtin_fmaf(diff, diff, acc);
+            }
+            unsigned bits;
+            memcpy(&bits, &acc, 4);
+            if (pass == 0 || (bits >> (shift + 8)) == prefix)
+                atomicAdd(&h[(bits >> shift) & 255u], 1u);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        // lane l holds bins 4l .. 4l+3; inclusive scan of their sums
+        unsigned c4[4], sum = 0;
+        for (int t = 0; t < 4; t++) {
+            c4[t] = h[4 * lane + t];
+            sum += c4[t];
+        }
+        unsigned incl = sum;
+        for (int o = 1; o < FPH_WAVE; o <<= 1) {
+            const unsigned y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        const unsigned excl = incl - sum;
+        // the lane whose range reaches rank krem picks the byte
+        const bool mine = excl < krem && incl >= krem;
+        const unsigned long long who = __ballot(mine);
+        const int src = who ? __ffsll((long long)who) - 1 : 0;  // always found
+        unsigned byte = 0, before = excl;
+        if (mine) {
+            for (int t = 0; t < 4; t++) {
+                if (before + c4[t] >= krem) { byte = 4 * lane + t; break; }
+                before += c4[t];
+            }
+        }
+        byte = (unsigned)__shfl((int)byte, src);
+        before = (unsigned)__shfl((int)before, src);
+        prefix = (prefix << 8) | byte;
+        krem -= before;
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane == 0) {
+        float v;
+        memcpy(&v, &prefix, 4);
+        kth[g] = v;
+    }
+}
+
 // Largest squared distance from each point to the rest of its cloud
 // (threshold.py:179-200 takes the minimum of these over the cloud); the
 // cloud's minimum goes to rmin[c] as FP32 bits (non-negative floats order
@@ -480,7 +566,7 @@ extern "C" int fph_knn_threshold_batch(const float* pts, const int64_t* foff, co
                                        float* t_out, float* thr_sq_out, void* ws,
                                        size_t* ws_bytes, void* stream) {
     hipStream_t s = (hipStream_t)stream;
-    if (C < 1 || N < 0 || k < 1 || k > FPH_KNN_MAX_K || !ws_bytes || max_dim_host < 1 ||
+    if (C < 1 || N < 0 || k < 1 || !ws_bytes || max_dim_host < 1 ||
         n_qtiles < 0 || n_qtiles > 0x7FFFFFFFLL || N >= 0x7FFFFFFFLL)
         return FPH_ERR_ARG;
     const size_t n4 = align_up(4 * (size_t)std::max<int64_t>(N, 1));
@@ -500,7 +586,12 @@ extern "C" int fph_knn_threshold_batch(const float* pts, const int64_t* foff, co
     if (N > 0 && n_qtiles > 0) {
         const size_t lds = 4 * (size_t)(max_dim_host > 64 ? kKnnLdsFloats : kKnnCand * 64);
         hipError_t e;
-        if (k <= 16) e = launch_knn<16>(pts, foff, dims, vbase, (int)C, k, qtile_off, n_qtiles, lds, kth, s);
+        if (k > FPH_KNN_MAX_K) {
+            k_knn_select<<<(unsigned)((N + kSelWaves - 1) / kSelWaves), kSelWaves * FPH_WAVE, 0,
+                           s>>>(pts, (const long long*)foff, dims, (const long long*)vbase,
+                                (int)C, N, k, kth);
+            e = hipGetLastError();
+        } else if (k <= 16) e = launch_knn<16>(pts, foff, dims, vbase, (int)C, k, qtile_off, n_qtiles, lds, kth, s);
         else if (k <= 32) e = launch_knn<32>(pts, foff, dims, vbase, (int)C, k, qtile_off, n_qtiles, lds, kth, s);
         else if (k <= 64) e = launch_knn<64>(pts, foff, dims, vbase, (int)C, k, qtile_off, n_qtiles, lds, kth, s);
         else e = launch_knn<128>(pts, foff, dims, vbase, (int)C, k, qtile_off, n_qtiles, lds, kth, s);

@@ -178,7 +178,9 @@ __global__ __launch_bounds__(kThreadsM, 4) void k_edge_screen_mfma(
     const float* __restrict__ nrm, int n, int d, int dpad,
     float thr, float marg_a, float marg_b, long long nslots, unsigned long long* __restrict__ keys,
     unsigned long long* __restrict__ vals, long long cap,
-    unsigned long long* __restrict__ count, unsigned long long* __restrict__ ncand) {
+    unsigned long long* __restrict__ count, unsigned long long* __restrict__ ncand,
+    unsigned long long* __restrict__ band, long long band_cap,
+    unsigned long long* __restrict__ nband) {
     __shared__ float s_nr[kMT], s_nc[kMT];
     // operands of kStageSteps k-steps of the whole tile, [2 vector + h/l]
     // [point]: each row and column block is read from global memory once per
@@ -321,17 +323,86 @@ __global__ __launch_bounds__(kThreadsM, 4) void k_edge_screen_mfma(
                             keys[at] = ((unsigned long long)bits << 32) | (colex & 0xFFFFFFFFull);
                             vals[at] = (uj << 32) | (unsigned long long)i;
                         }
+                    } else if (band) {  // validation mode: list the margin band
+                        const unsigned long long at = atomicAdd(nband, 1ull);
+                        if (at < (unsigned long long)band_cap)
+                            band[at] = ((unsigned long long)j << 32) | (unsigned long long)i;
                     }
                 }
             }
         }
 }
 
-__global__ void k_zero2(unsigned long long* a, unsigned long long* b) {
+__global__ void k_zero2(unsigned long long* a, unsigned long long* b, unsigned long long* c) {
     if (threadIdx.x == 0) {
         a[0] = 0ull;
         b[0] = 0ull;
+        if (c) c[0] = 0ull;
     }
+}
+
+// FP64 check of screened pairs (validation mode).  Per pair (val = j << 32 |
+// i): the FP32 FMA chain of the filtration and the FP64 distance (FP32
+// coordinates differ exactly in FP64; the FP64 FMA chain is within d 2^-53
+// relative of |x - y|^2).  st[0] pairs, [1] FP32 members (dsq32 < thr_sq),
+// [2] FP64 members (dsq64 < t^2 with t the caller's double), [3] FP32 in /
+// FP64 out, [4] FP32 out / FP64 in, [5] largest |dsq32 - dsq64| over FP32
+// members in 1/1024 ulp of dsq32.
+__global__ void k_pairs_fp64(const float* __restrict__ pts, int d,
+                             const unsigned long long* __restrict__ pv, long long m,
+                             float thr_sq, double t2, unsigned long long* __restrict__ st) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long f32in = 0, f64in = 0, a = 0, b = 0, err = 0;
+    const bool live = e < m;
+    if (live) {
+        const unsigned long long v = pv[e];
+        const long long i = (long long)(v & 0xFFFFFFFFull), j = (long long)(v >> 32);
+        const float* xi = pts + i * d;
+        const float* xj = pts + j * d;
+        float e32 = 0.0f;
+        double e64 = 0.0;
+        for (int k = 0; k < d; k++) {
+            const float diff = xi[k] - xj[k];
+            e32 = __builtin_fmaf(diff, diff, e32);
+            const double dd = (double)xi[k] - (double)xj[k];
+            e64 = __builtin_fma(dd, dd, e64);
+        }
+        const bool in32 = e32 < thr_sq, in64 = e64 < t2;
+        f32in = in32;
+        f64in = in64;
+        a = in32 && !in64;
+        b = !in32 && in64;
+        if (in32 && e32 > 0.0f) {
+            unsigned bits;
+            memcpy(&bits, &e32, 4);
+            const unsigned nb = bits + 1u;
+            float nxt;
+            memcpy(&nxt, &nb, 4);
+            const double ulp = (double)nxt - (double)e32;
+            err = (unsigned long long)(fabs((double)e32 - e64) / ulp * 1024.0);
+        }
+    }
+    // wave totals, one atomic per wave and counter
+    const unsigned long long m_live = __ballot(live);
+    const unsigned long long c1 = __popcll(__ballot(f32in)), c2 = __popcll(__ballot(f64in));
+    const unsigned long long c3 = __popcll(__ballot(a)), c4 = __popcll(__ballot(b));
+    unsigned long long mx = err;
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long y = __shfl_xor(mx, o);
+        mx = y > mx ? y : mx;
+    }
+    if (fph_lane() == 0 && m_live) {
+        atomicAdd(&st[0], (unsigned long long)__popcll(m_live));
+        if (c1) atomicAdd(&st[1], c1);
+        if (c2) atomicAdd(&st[2], c2);
+        if (c3) atomicAdd(&st[3], c3);
+        if (c4) atomicAdd(&st[4], c4);
+        if (mx) atomicMax(&st[5], mx);
+    }
+}
+
+__global__ void k_zero_n(unsigned long long* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
 double gamma_n(int m) {
@@ -341,12 +412,12 @@ double gamma_n(int m) {
 
 }  // namespace
 
-extern "C" int fph_edge_screen_mfma(const float* pts, int64_t n, int d, float thr_sq,
-                                    uint64_t* keys, uint64_t* vals, int64_t cap,
-                                    uint64_t* count_dev, uint64_t* cand_dev, void* ws,
-                                    size_t* ws_bytes, void* stream) {
-    hipStream_t s = (hipStream_t)stream;
+static int screen_mfma(const float* pts, int64_t n, int d, float thr_sq, uint64_t* keys,
+                       uint64_t* vals, int64_t cap, uint64_t* count_dev, uint64_t* cand_dev,
+                       uint64_t* band, int64_t band_cap, uint64_t* band_count, void* ws,
+                       size_t* ws_bytes, hipStream_t s) {
     if (n < 0 || d < 1 || cap < 0 || !count_dev || !cand_dev || !ws_bytes) return FPH_ERR_ARG;
+    if (band && (band_cap < 0 || !band_count)) return FPH_ERR_ARG;
     if (n > 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
     const int dpad = (d + 15) / 16 * 16;
     const size_t split = align_up(4 * (size_t)std::max<int64_t>(n, 1) * dpad);
@@ -361,7 +432,8 @@ extern "C" int fph_edge_screen_mfma(const float* pts, int64_t n, int d, float th
     float* nrm = (float*)(w + split);
     float* ctr = (float*)(w + split + nrm_b);
     double* part = (double*)(w + split + nrm_b + ctr_b);
-    k_zero2<<<1, 64, 0, s>>>((unsigned long long*)count_dev, (unsigned long long*)cand_dev);
+    k_zero2<<<1, 64, 0, s>>>((unsigned long long*)count_dev, (unsigned long long*)cand_dev,
+                             band ? (unsigned long long*)band_count : nullptr);
     FPH_LAUNCH_CHECK();
     if (n < 2) return FPH_OK;
     k_centre_partial<<<kCtrBlocks, 64, 0, s>>>(pts, n, d, part);
@@ -388,7 +460,42 @@ extern "C" int fph_edge_screen_mfma(const float* pts, int64_t n, int d, float th
     k_edge_screen_mfma<<<(unsigned)nblk, kThreadsM, 0, s>>>(
         pts, (const bf16x8*)xhl, nrm, (int)n, d, dpad, thr_sq, marg_a, marg_b,
         nslots, (unsigned long long*)keys, (unsigned long long*)vals, cap,
-        (unsigned long long*)count_dev, (unsigned long long*)cand_dev);
+        (unsigned long long*)count_dev, (unsigned long long*)cand_dev,
+        (unsigned long long*)band, band_cap, (unsigned long long*)band_count);
+    FPH_LAUNCH_CHECK();
+    return FPH_OK;
+}
+
+extern "C" int fph_edge_screen_mfma(const float* pts, int64_t n, int d, float thr_sq,
+                                    uint64_t* keys, uint64_t* vals, int64_t cap,
+                                    uint64_t* count_dev, uint64_t* cand_dev, void* ws,
+                                    size_t* ws_bytes, void* stream) {
+    return screen_mfma(pts, n, d, thr_sq, keys, vals, cap, count_dev, cand_dev, nullptr, 0,
+                       nullptr, ws, ws_bytes, (hipStream_t)stream);
+}
+
+extern "C" int fph_edge_screen_mfma_band(const float* pts, int64_t n, int d, float thr_sq,
+                                         uint64_t* keys, uint64_t* vals, int64_t cap,
+                                         uint64_t* count_dev, uint64_t* cand_dev,
+                                         uint64_t* band, int64_t band_cap,
+                                         uint64_t* band_count, void* ws, size_t* ws_bytes,
+                                         void* stream) {
+    if (!band) return FPH_ERR_ARG;
+    return screen_mfma(pts, n, d, thr_sq, keys, vals, cap, count_dev, cand_dev, band, band_cap,
+                       band_count, ws, ws_bytes, (hipStream_t)stream);
+}
+
+extern "C" int fph_pairs_fp64_check(const float* pts, int64_t n, int d, double max_edge_length,
+                                    float thr_sq, const uint64_t* pair_vals, int64_t m,
+                                    uint64_t* stats, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (n < 0 || d < 1 || m < 0 || !stats || (m > 0 && !pair_vals)) return FPH_ERR_ARG;
+    k_zero_n<<<1, 64, 0, s>>>((unsigned long long*)stats, 8);
+    FPH_LAUNCH_CHECK();
+    if (m == 0) return FPH_OK;
+    k_pairs_fp64<<<fph_blocks(m, 256), 256, 0, s>>>(
+        pts, d, (const unsigned long long*)pair_vals, m, thr_sq,
+        max_edge_length * max_edge_length, (unsigned long long*)stats);
     FPH_LAUNCH_CHECK();
     return FPH_OK;
 }

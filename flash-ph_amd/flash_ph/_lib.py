@@ -34,8 +34,12 @@ SIGNATURES = {
     "fph_version": (ctypes.c_char_p, []),
     "fph_edge_screen": (_I32, [_P, _I64, _I32, _F32, _P, _P, _I64, _P, _P]),
     "fph_edge_screen_mfma": (_I32, [_P, _I64, _I32, _F32, _P, _P, _I64, _P, _P, _P, _PSZ, _P]),
+    "fph_edge_screen_mfma_band": (_I32, [_P, _I64, _I32, _F32, _P, _P, _I64, _P, _P, _P, _I64,
+                                         _P, _P, _PSZ, _P]),
+    "fph_pairs_fp64_check": (_I32, [_P, _I64, _I32, _F64, _F32, _P, _I64, _P, _P]),
     "fph_edge_sort": (_I32, [_P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _PSZ, _P]),
     "fph_vertex_csr": (_I32, [_P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _PSZ, _P]),
+    "fph_coo_export": (_I32, [_P, _P, _P, _I64, _I64, _P, _P, _P, _P]),
     "fph_boruvka_rounds": (_I32, [_I64]),
     "fph_boruvka_scratch_bytes": (_SZ, [_I64]),
     "fph_boruvka_mst": (_I32, [_P, _P, _I64, _I64, _P, _P, _P, _P, _SZ, _P]),
@@ -46,6 +50,7 @@ SIGNATURES = {
     "fph_cofacets_h1": (_I32, [_P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _I64, _P, _P,
                                _PSZ, _P]),
     "fph_apparent": (_I32, [_I64, _I64, _P, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P]),
+    "fph_apparent_scan": (_I32, [_P, _P, _P, _I64, _P, _P, _P, _I64, _P, _P, _P]),
     "fph_reduce_scratch_bytes": (_SZ, [_I64]),
     "fph_reduce": (_I32, [_I64, _P, _P, _I64, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _P,
                           _SZ, _P, _I32, _P]),

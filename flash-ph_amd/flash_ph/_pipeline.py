@@ -247,6 +247,54 @@ def build_edges(pts: torch.Tensor, max_edge_length: float) -> Complex:
     return cx
 
 
+def fp64_screen_report(pts: torch.Tensor, max_edge_length: float) -> dict:
+    """Validation mode of the edge screen: screens (the split-bf16 screen
+    when use_mfma_screen picks it, listing its margin band), then re-examines
+    the edges and the band in FP64 on the device (fph_pairs_fp64_check).
+    Returns counts: edges, band (candidates the FP32 recompute rejected),
+    edges whose exact distance is not below t ("fp32_in_exact_out"), band
+    pairs whose exact distance is ("exact_in_fp32_out"), and the largest
+    FP32-vs-FP64 error of the edges' dsq in ulp.  Diagnostic only: the
+    filtration stays the FP32 one."""
+    lib = load()
+    if pts.dtype != torch.float32 or not pts.is_contiguous():
+        raise ValueError("fp64_screen_report needs a contiguous float32 (n, d) tensor")
+    dev = pts.device
+    s = stream_of(dev)
+    n, d = pts.shape
+    thr = threshold_sq_f32(max_edge_length)
+    mfma = use_mfma_screen(n, d)
+    cap = max(_EDGE_CAP_MIN, _EDGE_CAP_PER_POINT * n)
+    bcap = 1 << 16
+    cnt = _empty(torch.int64, dev, 3)
+    while True:
+        keys = _empty(torch.int64, dev, cap)
+        vals = _empty(torch.int64, dev, cap)
+        band = _empty(torch.int64, dev, bcap)
+        if mfma:
+            call_ws("fph_edge_screen_mfma_band",
+                    (ptr(pts), n, d, thr, ptr(keys), ptr(vals), cap, ptr(cnt),
+                     cnt.data_ptr() + 8, ptr(band), bcap, cnt.data_ptr() + 16), (s,), dev)
+            E, _, B = (int(x) for x in cnt.tolist())
+        else:
+            check(lib.fph_edge_screen(ptr(pts), n, d, thr, ptr(keys), ptr(vals), cap, ptr(cnt),
+                                      s), "fph_edge_screen")
+            E, B = int(cnt[0].item()), 0
+        if E <= cap and B <= bcap:
+            break
+        cap, bcap = max(cap, E), max(bcap, B)
+    st = _empty(torch.int64, dev, 16)
+    check(lib.fph_pairs_fp64_check(ptr(pts), n, d, float(max_edge_length), thr, ptr(vals), E,
+                                   ptr(st), s), "fph_pairs_fp64_check")
+    check(lib.fph_pairs_fp64_check(ptr(pts), n, d, float(max_edge_length), thr, ptr(band), B,
+                                   st.data_ptr() + 64, s), "fph_pairs_fp64_check")
+    e, b = st[:8].tolist(), st[8:].tolist()
+    return {"screen": "mfma" if mfma else "direct", "edges": E, "band": B,
+            "edges_fp32_members": e[1], "band_fp32_members": b[1],
+            "fp32_in_exact_out": e[3], "exact_in_fp32_out": b[4],
+            "max_ulp_fp32_vs_fp64": e[5] / 1024.0}
+
+
 def build_mst(cx: Complex, want_labels: bool = False) -> None:
     """Stage (b): Boruvka forest, component labels and the H0 diagram."""
     lib = load()

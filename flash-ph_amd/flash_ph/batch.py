@@ -62,10 +62,7 @@ def auto_threshold_batch(clouds, k: int = 20, percentile: float = 95.0) -> list[
     linear interpolation in FP32.  Distances are the filtration's own FP32
     values (the edge screen's FMA chain, correctly rounded sqrt); the
     reference's cdist-based values agree to within cdist's rounding.
-    k <= 128.
     """
-    if k > _pipeline.KNN_MAX_K:
-        raise ValueError(f"k must be <= {_pipeline.KNN_MAX_K} for the batched kNN, got {k}")
     pts, dev, _ = _prepare(clouds)
     if not pts:
         return []

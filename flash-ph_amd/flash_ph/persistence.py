@@ -20,32 +20,48 @@ _EXTRA_BACKENDS = ("exact",)
 
 
 def _rips_h1h2_giotto(filt, n, max_dim, max_edge_length, device):
-    """H1 (+H2) through giotto-ph, kept for ``backend='giotto'``
-    (reference persistence.py:27-61).  Raises ModuleNotFoundError when
-    giotto-ph is not installed."""
+    """H1 (and H2) from giotto-ph's reducer, for ``backend='giotto'``
+    (the reference's route, persistence.py:27-61).  The filtration leaves the
+    GPU as COO triplets written by fph_coo_export (one device-to-host copy);
+    giotto-ph then reduces with edge collapse on all host cores.  Raises
+    ModuleNotFoundError when giotto-ph is not installed."""
     from gph import ripser_parallel
+
+    dm = coo_distance_matrix(filt)
+    res = ripser_parallel(dm, maxdim=max_dim, thresh=max_edge_length, metric="precomputed",
+                          collapse_edges=True, n_threads=-1)
+    out = []
+    for dim in range(1, max_dim + 1):
+        rows = np.asarray(res["dgms"][dim], dtype=np.float32).reshape(-1, 2)
+        t = torch.from_numpy(rows).to(device)
+        out.append(t[torch.argsort(t[:, 0], stable=True)] if t.shape[0] else
+                   torch.empty(0, 2, dtype=torch.float32, device=device))
+    return out
+
+
+def coo_distance_matrix(filt):
+    """scipy (n, n) COO sparse distance matrix of a RipsFiltration: one entry
+    (i, j, |x_i - x_j|) per edge (i < j; FP32 correctly rounded sqrt of the
+    edge's dsq) and the zero diagonal -- the input the reference hands to
+    giotto-ph (persistence.py:36-45)."""
     import scipy.sparse
 
-    row = filt.edge_i.cpu().numpy().astype(np.int64)
-    col = filt.edge_j.cpu().numpy().astype(np.int64)
-    dist = torch.sqrt(filt.edge_dist_sq).cpu().numpy().astype(np.float32)
-    diag = np.arange(n, dtype=np.int64)
-    row = np.concatenate([row, diag])
-    col = np.concatenate([col, diag])
-    dist = np.concatenate([dist, np.zeros(n, dtype=np.float32)])
-    dm = scipy.sparse.coo_matrix((dist, (row, col)), shape=(n, n))
-    result = ripser_parallel(dm, maxdim=max_dim, thresh=max_edge_length,
-                             metric="precomputed", collapse_edges=True, n_threads=-1)
-    diagrams = []
-    for d in range(1, max_dim + 1):
-        dgm_np = result["dgms"][d]
-        if len(dgm_np) > 0:
-            t = torch.from_numpy(dgm_np).float().to(device)
-            t = t[t[:, 0].argsort(stable=True)]
-        else:
-            t = torch.empty(0, 2, dtype=torch.float32, device=device)
-        diagrams.append(t)
-    return diagrams
+    n = int(filt.n)
+    E = int(filt.edge_i.shape[0])
+    dev = filt.edge_i.device
+    if dev.type != "cuda":
+        raise ValueError("coo_distance_matrix needs a filtration on the GPU")
+    lib = _lib.load()
+    row = torch.empty(E + n, dtype=torch.int64, device=dev)
+    col = torch.empty(E + n, dtype=torch.int64, device=dev)
+    val = torch.empty(E + n, dtype=torch.float32, device=dev)
+    _lib.check(lib.fph_coo_export(_lib.ptr(filt.edge_i), _lib.ptr(filt.edge_j),
+                                  _lib.ptr(filt.edge_dist_sq), E, n, row.data_ptr(), col.data_ptr(),
+                                  val.data_ptr(), _lib.stream_of(dev)), "fph_coo_export")
+    host = torch.cat([row, col, val.view(torch.int32).long()]).cpu().numpy()
+    r, c = host[:E + n], host[E + n:2 * (E + n)]
+    v = host[2 * (E + n):].astype(np.int32).view(np.float32)
+    return scipy.sparse.coo_matrix((v, (r, c)), shape=(n, n))
 
 
 def rips_persistence(

@@ -70,6 +70,35 @@ int fph_edge_screen_mfma(const float *pts, int64_t n, int d, float thr_sq, uint6
                          void *ws, size_t *ws_bytes, void *stream);
 
 /*
+ * Validation mode of fph_edge_screen_mfma: the same screen, which also lists
+ * the margin band -- the candidates whose direct FP32 recompute is not below
+ * thr_sq -- as val = j << 32 | i in band (capacity band_cap; *band_count =
+ * their total, which may exceed band_cap: rerun with more room).  The band
+ * and the edges are the pairs fph_pairs_fp64_check re-examines in FP64.
+ * No counterpart in the reference (which has no matrix-core screen).
+ */
+int fph_edge_screen_mfma_band(const float *pts, int64_t n, int d, float thr_sq, uint64_t *keys,
+                              uint64_t *vals, int64_t cap, uint64_t *count_dev,
+                              uint64_t *cand_dev, uint64_t *band, int64_t band_cap,
+                              uint64_t *band_count, void *ws, size_t *ws_bytes, void *stream);
+
+/*
+ * FP64 re-examination of m screened pairs (pair_vals: j << 32 | i, as in
+ * fph_edge_screen's vals) on the device: per pair the filtration's FP32 FMA
+ * chain and the FP64 distance.  stats (device uint64[8], overwritten): [0]
+ * pairs, [1] FP32 members (dsq32 < thr_sq), [2] exact members (dsq64 <
+ * max_edge_length^2 in FP64), [3] FP32 members that are not exact members,
+ * [4] exact members that are not FP32 members, [5] the largest
+ * |dsq32 - dsq64| over FP32 members in 1/1024 ulp of dsq32.
+ * Used on the margin band and the edges in validation mode; the reference's
+ * filtration is defined by the FP32 values (kernels/edge_kernel.py:112-125),
+ * so this reports, and does not change, its membership.
+ */
+int fph_pairs_fp64_check(const float *pts, int64_t n, int d, double max_edge_length,
+                         float thr_sq, const uint64_t *pair_vals, int64_t m, uint64_t *stats,
+                         void *stream);
+
+/*
  * Sort the E screened pairs by key (ties, possible only for n > 92682, in
  * the reference's emission order) and unpack them into edge_i, edge_j,
  * edge_dsq and edge_filt = sqrt(edge_dsq) (correctly rounded; may be NULL).
@@ -90,6 +119,16 @@ int fph_vertex_csr(const int32_t *edge_i, const int32_t *edge_j, const float *ed
                    int64_t E, int64_t n, int32_t *adj_ptr, int32_t *adj_idx,
                    int32_t *adj_rank, float *adj_dsq, void *ws, size_t *ws_bytes,
                    void *stream);
+
+/*
+ * COO triplets of the filtration's sparse distance matrix for an external
+ * reducer: entries 0..E-1 = (edge_i[e], edge_j[e], sqrt(edge_dsq[e])) in
+ * rank order (correctly rounded FP32 sqrt), entries E..E+n-1 = (v, v, 0)
+ * for every vertex.  row / col int64, val FP32, E + n entries each.
+ * Replaces the COO assembly of _rips_h1h2_giotto, persistence.py:27-45.
+ */
+int fph_coo_export(const int32_t *edge_i, const int32_t *edge_j, const float *edge_dsq,
+                   int64_t E, int64_t n, int64_t *row, int64_t *col, float *val, void *stream);
 
 /* ---- stage (b): minimum spanning forest and H0 ------------------------ */
 
@@ -195,6 +234,24 @@ int fph_apparent(int64_t K, int64_t col_base, const int32_t *col_off, const uint
                  const uint8_t *skip_mask, const int32_t *row_cmf, int64_t row_base,
                  int64_t nrows, uint8_t *kind, uint64_t *own, int32_t *pivot, int32_t *res_list,
                  int32_t *res_count, void *stream);
+
+/*
+ * Ripser-style apparent-pair scan of K non-forest edges (edge ranks in
+ * nonmst_ranks; edges indexed by rank): is_apparent[k] = 1 when a common
+ * neighbour w of the edge's endpoints has both connecting edges of lower
+ * rank, and pivot_colex[k] = C(c,3) + C(b,2) + a of the triangle with the
+ * smallest such w (sorted a < b < c; 0 when not apparent).  max_degree > 0
+ * reproduces the reference's GPU loop, which examines at most 2 md steps of
+ * the neighbour-list merge (md = max_degree rounded up to a power of two in
+ * [64, 2048]); max_degree <= 0 scans completely (the reference's CPU path).
+ * The set is a subset of fph_apparent's forest-cleared one.
+ * Replaces apparent_pair_scan, kernels/apparent_pair_kernel.py:101-190,
+ * 200-313 (not wired into the reference's pipeline).
+ */
+int fph_apparent_scan(const int32_t *edge_i, const int32_t *edge_j,
+                      const int32_t *nonmst_ranks, int64_t K, const int32_t *adj_ptr,
+                      const int32_t *adj_idx, const int32_t *adj_rank, int64_t max_degree,
+                      int32_t *is_apparent, int64_t *pivot_colex, void *stream);
 
 /* ---- stage (e): residual reduction ------------------------------------ */
 
@@ -394,7 +451,9 @@ int fph_cofacets_h2(int64_t T, int64_t E, int64_t Q, const int32_t *edge_i,
  * Distances are the filtration's FP32 FMA-chain values, correctly rounded
  * sqrt.  Query tiles of 256 points: qtile_off (C+1, exclusive scan of
  * ceil(n_c / 256)); max_dim_host = max_c dims[c].  Optional outputs:
- * kth_dist (N floats), thr_sq_out[c] = fp32(double(t)^2).  k <= 128.
+ * kth_dist (N floats), thr_sq_out[c] = fp32(double(t)^2).  k <= 128 keeps
+ * the k smallest distances in registers; larger k selects the k-th one by
+ * radix over the FP32 bits (four passes over the cloud per query).
  * Replaces _tiled_knn (threshold.py:133-176) and the quantile at :77.
  */
 int fph_knn_threshold_batch(const float *pts, const int64_t *foff, const int32_t *dims,

@@ -135,3 +135,28 @@ def test_graph_argument_errors_without_gpu():
         RipsH0Graph(100, 0, 1.0)
     with pytest.raises(RuntimeError):
         RipsH0Graph(100, 3, 1.0)
+
+
+def test_complex_accessors_compare_in_fp32(monkeypatch):
+    """RipsComplex's counts equal the reference's torch formulation
+    (complex.py:96-130: FP32 tensors compared with Python scalars, i.e. in
+    FP32), including bounds that are not FP32 numbers."""
+    import flash_ph.complex as cm
+    dg = [torch.tensor([[0.0, 0.1], [0.0, 0.30000001192092896], [0.0, float("inf")]]),
+          torch.tensor([[0.1, 0.2], [0.2, float("inf")], [0.30000001192092896, 0.5]]),
+          torch.empty(0, 2)]
+    monkeypatch.setattr(cm, "rips_persistence", lambda *a, **k: [d.clone() for d in dg])
+    rc = cm.RipsComplex(torch.zeros(5, 2), 1.0)
+    rc.compute_persistence(max_dim=2)
+    for lo, hi in [(0.1, 0.1), (0.3, 0.2), (0.0, 0.30000000000000004), (1.0, 0.0)]:
+        want = [int(((d[:, 0] <= lo) & (d[:, 1] > hi)).sum()) if d.numel() else 0 for d in dg]
+        assert rc.persistent_betti_numbers(lo, hi) == want, (lo, hi)
+    assert rc.betti_numbers() == [int((d[:, 1] == float("inf")).sum()) for d in dg]
+    got = rc.persistence(max_dim=2)
+    want = sorted([(k, (float(b), float(e))) for k, d in enumerate(dg) for b, e in d.tolist()],
+                  key=lambda r: (r[0], r[1][0], r[1][1]))
+    assert got == want
+    iv = rc.persistence_intervals_in_dimension(1)
+    assert iv.dtype == np.float64 and np.array_equal(iv, dg[1].double().numpy())
+    assert rc.persistence_intervals_in_dimension(2).shape == (0, 2)
+    assert rc.persistence_intervals_in_dimension(-1).shape == (0, 2)

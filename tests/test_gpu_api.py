@@ -133,3 +133,36 @@ def test_n10000_d10_invariants():
     # Euler characteristic bookkeeping of the columns: E = forest + H1 columns
     kind1 = cx.kind1[: cx.E].cpu().numpy()
     assert int((kind1 == 0).sum()) == n_mst
+
+
+def test_coo_export_and_giotto_route():
+    """fph_coo_export: the sparse distance matrix the reference hands to
+    giotto-ph (persistence.py:36-45) -- one (i, j, sqrt(dsq)) entry per edge
+    plus the zero diagonal -- equal to the oracle's edges; its minimum
+    spanning forest gives the H0 deaths.  Without giotto-ph installed,
+    backend='giotto' raises ModuleNotFoundError as the reference does."""
+    import importlib.util
+
+    import oracle
+    from scipy.sparse.csgraph import minimum_spanning_tree
+
+    from flash_ph import rips_persistence
+    from flash_ph.persistence import coo_distance_matrix
+    from flash_ph.rips import rips_filtration
+    print("[gpu] api coo export", flush=True)
+    pts = cases.randn(400, 3, 8)
+    ref = oracle.run(pts, 0.8, 0)
+    f = rips_filtration(torch.from_numpy(pts).cuda(), 0.8)
+    dm = coo_distance_matrix(f)
+    E = ref["stats"]["E"]
+    assert dm.shape == (400, 400) and dm.nnz == E + 400
+    assert np.array_equal(dm.row[:E], ref["edge_i"]) and np.array_equal(dm.col[:E], ref["edge_j"])
+    assert np.array_equal(dm.data[:E].view(np.uint32),
+                          oracle.sqrt_f32(ref["edge_dsq"]).view(np.uint32))
+    assert np.array_equal(dm.row[E:], np.arange(400)) and not dm.data[E:].any()
+    fin = np.sort(minimum_spanning_tree(dm.tocsr()).data)
+    h0 = ref["h0"][np.isfinite(ref["h0"][:, 1]), 1]
+    assert np.array_equal(fin.astype(np.float32), h0)
+    if importlib.util.find_spec("gph") is None:
+        with pytest.raises(ModuleNotFoundError):
+            rips_persistence(torch.from_numpy(pts).cuda(), 0.8, max_dim=1, backend="giotto")

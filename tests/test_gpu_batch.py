@@ -40,7 +40,7 @@ def _mixed_clouds():
     return out
 
 
-@pytest.mark.parametrize("k", [1, 5, 20, 33, 100])
+@pytest.mark.parametrize("k", [1, 5, 20, 33, 100, 128, 129, 200, 1000])
 def test_knn_threshold_batch_vs_oracle(k):
     from flash_ph import _pipeline
     print(f"[gpu] knn batch k={k}", flush=True)
@@ -125,8 +125,6 @@ def test_batch_errors():
         rips_persistence_batch([cases.randn(10, 2, 0)], [1.0, 2.0])
     with pytest.raises(ValueError):
         rips_persistence_batch([cases.randn(10, 2, 0)], -1.0)
-    with pytest.raises(ValueError):
-        auto_threshold_batch([cases.randn(10, 2, 0)], k=129)
     assert rips_persistence_batch([], 1.0) == []
 
 
@@ -163,8 +161,9 @@ def test_auto_threshold_and_enclosing_radius_api():
         assert np.float32(enclosing_radius(torch.from_numpy(x))) == np.float32(
             oracle.enclosing_radius(x))
     x = cases.randn(400, 4, 9)
-    big_k = auto_threshold(x, k=200, percentile=90.0)
-    assert abs(big_k - oracle.auto_threshold(x, 200, 90.0)) < 1e-5 * big_k
+    for k in (129, 200, 399, 5000):   # the radix-select path, bit-exact
+        big_k = auto_threshold(x, k=k, percentile=90.0)
+        assert np.float32(big_k) == np.float32(oracle.auto_threshold(x, k, 90.0)), k
     r_all = enclosing_radius(x)
     r_sub = enclosing_radius(x, subsample=100)
     assert 0 < r_sub and abs(r_sub - r_all) / r_all < 0.5

@@ -97,3 +97,33 @@ def test_mfma_offset_cloud():
                            getattr(b, name)[:b.E].view(torch.int32)), name
     cand = int(b.info["screen_candidates"].item())
     assert b.E <= cand <= 2 * b.E + 1000, (cand, b.E)
+
+
+def test_fp64_validation_mode():
+    """Validation mode: the band listed by the split-bf16 screen is exactly
+    candidates minus edges; in FP64 every edge is an FP32 member and no band
+    pair is; on the near-threshold cloud some pairs change membership in
+    exact arithmetic (what the report is for), within a few ulp."""
+    from flash_ph import _pipeline
+    pts, offs = cases.near_threshold_pairs(300, 50, 3.0, seed=50)
+    x = torch.from_numpy(pts).cuda()
+    old = _pipeline.SCREEN
+    try:
+        _pipeline.SCREEN = "mfma"
+        print("[gpu] fp64 validation near-threshold", flush=True)
+        rep = _pipeline.fp64_screen_report(x, 3.0)
+        cx = _edges(x, 3.0, "mfma")
+    finally:
+        _pipeline.SCREEN = old
+    assert rep["screen"] == "mfma" and rep["edges"] == cx.E
+    assert rep["edges"] + rep["band"] == int(cx.info["screen_candidates"].item())
+    assert rep["edges_fp32_members"] == rep["edges"] and rep["band_fp32_members"] == 0
+    assert rep["band"] > 0
+    assert 0.0 < rep["max_ulp_fp32_vs_fp64"] < 64.0
+    # the planted pairs within +-5 ulp of t^2: exact membership differs for some
+    assert rep["fp32_in_exact_out"] + rep["exact_in_fp32_out"] >= 1
+    print("[gpu] fp64 validation n=20000 d=40", flush=True)
+    y = torch.from_numpy(cases.randn(20_000, 40, 3)).cuda()
+    rep2 = _pipeline.fp64_screen_report(y, 5.2)
+    assert rep2["screen"] == "mfma" and rep2["edges"] > 1000
+    assert rep2["edges_fp32_members"] == rep2["edges"] and rep2["band_fp32_members"] == 0
