@@ -386,6 +386,21 @@ __device__ __forceinline__ void st_coh(unsigned* p, unsigned v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Bounded wait whose clock starts at the first retry: the fast path (the
+// data already there) reads no clock.  Reading the device real-time clock
+// is a scalar-memory operation, and LDS operations wait on the same counter,
+// so a read on every column addition stalls the wavefront's next LDS access
+// for as long as the clock read takes -- long when hundreds of polling
+// wavefronts read the clock too.
+struct Deadline {
+    long long t0 = -1;
+    __device__ bool expired(long long wait) {
+        const long long now = wall_clock64();
+        if (t0 < 0) t0 = now;
+        return now - t0 > wait;
+    }
+};
+
 __device__ bool fail_wait(const RedArgs& a) {
     if (fph_lane() == 0) atomicExch(&a.status[0], 5);
     return false;
@@ -395,7 +410,7 @@ __device__ bool fail_wait(const RedArgs& a) {
 // waiting for words not yet written; false after `wait` ticks.
 __device__ bool copy_rows_dev(const unsigned* src, int n, unsigned* dst, long long wait) {
     const int lane = fph_lane();
-    const long long t0 = wall_clock64();
+    Deadline dl;
     for (int base = 0; base < n; base += FPH_WAVE) {
         const int i = base + lane;
         bool miss = i < n;
@@ -410,7 +425,7 @@ __device__ bool copy_rows_dev(const unsigned* src, int n, unsigned* dst, long lo
             }
             first = false;
             if (!__ballot(miss)) break;
-            if (wall_clock64() - t0 > wait) return false;
+            if (dl.expired(wait)) return false;
             __builtin_amdgcn_s_sleep(1);
         }
     }
@@ -423,14 +438,14 @@ __device__ bool copy_rows_dev(const unsigned* src, int n, unsigned* dst, long lo
 // -1 after `wait` ticks.  The read may pass the column's end (the pool has
 // 64 words of slack past its capacity); those words are ignored.
 __device__ int spec_read_dev(const unsigned* src, unsigned& v, long long wait) {
-    const long long t0 = wall_clock64();
+    Deadline dl;
     bool first = true;
     while (true) {
         v = first ? ld_fast(src + fph_lane()) : ld_coh(src + fph_lane());
         first = false;
         const unsigned len = (unsigned)__builtin_amdgcn_readlane((int)v, 0);
         if (len != kUnwritten) return (int)len;
-        if (wall_clock64() - t0 > wait) return -1;
+        if (dl.expired(wait)) return -1;
         __builtin_amdgcn_s_sleep(1);
     }
 }
@@ -440,7 +455,7 @@ __device__ int spec_read_dev(const unsigned* src, unsigned& v, long long wait) {
 __device__ bool spec_finish_dev(const unsigned* src, unsigned v, int len, unsigned* dst,
                                 long long wait) {
     const int lane = fph_lane();
-    const long long t0 = wall_clock64();
+    Deadline dl;
     bool miss = lane >= 1 && lane <= len;
     while (true) {
         if (miss) {
@@ -452,7 +467,7 @@ __device__ bool spec_finish_dev(const unsigned* src, unsigned v, int len, unsign
             }
         }
         if (!__ballot(miss)) break;
-        if (wall_clock64() - t0 > wait) return false;
+        if (dl.expired(wait)) return false;
         __builtin_amdgcn_s_sleep(1);
     }
     if (len > FPH_WAVE - 1)
@@ -754,10 +769,12 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_queue(
             // ticks without progress (no change of the pending count or
             // the ring tail) -- other wavefronts may legitimately work on a
             // long chain meanwhile
-            unsigned long long t_last = wall_clock64();
+            // (the clock is read every 16th round; see Deadline)
+            unsigned long long t_last = 0;
+            bool timing = false;
             int last_pend = -1;
             unsigned last_tail = 0xFFFFFFFFu;
-            while (true) {
+            for (unsigned it = 0;; it++) {
                 unsigned long long v = 0;
                 int pend = 0, ab = 0;
                 unsigned tail = 0;
@@ -780,18 +797,21 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_queue(
                     break;
                 }
                 if (pend == 0 || ab) break;
-                const unsigned long long now = wall_clock64();
-                if (pend != last_pend || tail != last_tail) {
-                    last_pend = pend;
-                    last_tail = tail;
-                    t_last = now;
-                }
-                if (now - t_last > (unsigned long long)spin_limit) {
-                    if (lane == 0) {
-                        atomicExch(&a.status[0], 5);
-                        atomicExch(&s_abort, 1);
+                if ((it & 15) == 15) {
+                    const unsigned long long now = wall_clock64();
+                    if (!timing || pend != last_pend || tail != last_tail) {
+                        timing = true;
+                        last_pend = pend;
+                        last_tail = tail;
+                        t_last = now;
                     }
-                    break;
+                    if (now - t_last > (unsigned long long)spin_limit) {
+                        if (lane == 0) {
+                            atomicExch(&a.status[0], 5);
+                            atomicExch(&s_abort, 1);
+                        }
+                        break;
+                    }
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
@@ -929,10 +949,10 @@ __device__ __forceinline__ unsigned long long load_own_fast(const RedArgs& a, un
 // Waits until every lane with `need` holds a written word in x (first read
 // already issued by the caller); false after `wait` ticks.
 __device__ __forceinline__ bool settle_word(const unsigned* p, bool need, unsigned& x,
-                                            long long t0, long long wait) {
+                                            Deadline& dl, long long wait) {
     bool miss = need && x == kUnwritten;
     while (__ballot(miss)) {
-        if (wall_clock64() - t0 > wait) return false;
+        if (dl.expired(wait)) return false;
         __builtin_amdgcn_s_sleep(1);
         if (miss) {
             x = ld_coh(p);
@@ -948,14 +968,14 @@ __device__ __forceinline__ bool settle_word(const unsigned* p, bool need, unsign
 __device__ unsigned bm_flip_published(const RedArgs& a, BmWork& w, const unsigned* src,
                                       unsigned v, int n, long long wait) {
     const int lane = fph_lane();
-    const long long t0 = wall_clock64();
+    Deadline dl;
     int delta = 0, hi = -1;  // wave-uniform
     unsigned first = 0;
     for (int base = 0; base <= n; base += FPH_WAVE) {
         const int i = base + lane;            // word index in [len, rows...]
         const bool need = i >= 1 && i <= n;
         unsigned x = base == 0 ? v : (need ? ld_fast(src + i) : 0u);
-        if (!settle_word(src + i, need, x, t0, wait)) return 0xFFFFFFFFu;
+        if (!settle_word(src + i, need, x, dl, wait)) return 0xFFFFFFFFu;
         if (base == 0) first = (unsigned)__builtin_amdgcn_readlane((int)x, 1);
         const unsigned r = x - (unsigned)a.row_base;
         const int wd = (int)(r >> 5);
@@ -1169,10 +1189,14 @@ __device__ int reduce_claim_dev_bm(const RedArgs& a, int c, unsigned src, bool c
 
 // qs: [0] next residual index, [1] next ring slot to read, [2] ring tail,
 // [3] pending columns, [4] abort, [5] XCD taking part (-1: none yet).
+// BITMAP: working columns as bitmaps in the wavefront's arena (rows fit
+// it), else sorted lists; two instantiations, so that the bitmap kernel
+// carries none of the list merges' code and registers.
+template <bool BITMAP>
 __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_queue_dev(
     RedArgs a, const unsigned* __restrict__ order, const int* __restrict__ res_count,
     unsigned long long* __restrict__ ring, int* __restrict__ qs, long long spin_limit,
-    int bitmap, int from_csr) {
+    int from_csr) {
     extern __shared__ unsigned smem[];
     __shared__ int s_part;
     a.lines = lines_of(a);
@@ -1194,7 +1218,7 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_queue_dev(
     const unsigned cap = (unsigned)a.K;  // >= pending columns
     int nxor = 0;
     Alloc al = make_alloc(512);  // many wavefronts: small private chunks
-    if (bitmap) {  // the bitmap starts (and is left) all zero
+    if (BITMAP) {  // the bitmap starts (and is left) all zero
         for (int i = lane; i < kArenaB; i += FPH_WAVE) arena[i] = 0u;
         wave_sync();
     }
@@ -1213,7 +1237,8 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_queue_dev(
             // poll the own slot every round and the shared counters (one
             // cache line that every waiting wavefront reads) every 8th, so
             // that idle wavefronts leave the memory path to the working ones
-            long long t_last = wall_clock64();
+            // the clock is read with the counters, every 64th round
+            long long t_last = -1;
             int last_pend = -1;
             unsigned last_tail = 0xFFFFFFFFu;
             for (unsigned it = 0;; it++) {
@@ -1238,8 +1263,12 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_queue_dev(
                     ab = __shfl(ab, 0);
                     tail = __shfl(tail, 0);
                     if (pend == 0 || ab) break;
+                    if ((it & 63) != 63) {
+                        __builtin_amdgcn_s_sleep(2);
+                        continue;
+                    }
                     const long long now = wall_clock64();
-                    if (pend != last_pend || tail != last_tail) {
+                    if (t_last < 0 || pend != last_pend || tail != last_tail) {
                         last_pend = pend;
                         last_tail = tail;
                         t_last = now;
@@ -1263,8 +1292,10 @@ __global__ __launch_bounds__(kPhaseBWaves * FPH_WAVE) void k_reduce_queue_dev(
         if (from_csr || a.pivot[c] >= 0) {
             const unsigned src = ld_coh(&a.desc[c]);
             const bool csr = from_csr && src == kUnwritten;
-            d = bitmap ? reduce_claim_dev_bm(a, c, src, csr, arena, &nxor, al, spin_limit)
-                       : reduce_claim_dev(a, c, src, arena, &nxor, al, spin_limit);
+            if constexpr (BITMAP)
+                d = reduce_claim_dev_bm(a, c, src, csr, arena, &nxor, al, spin_limit);
+            else
+                d = reduce_claim_dev(a, c, src, arena, &nxor, al, spin_limit);
             if (d == -2) {
                 if (lane == 0) {
                     atomicExch(qs + 4, 1);
@@ -1392,10 +1423,10 @@ __global__ void k_fill_u64(unsigned long long* __restrict__ p, long long n, unsi
 // (status 5).
 __device__ unsigned replay_wait_desc(const RedArgs& a, int k, const int* qs, long long wait) {
     unsigned src = __shfl(ld_fast(&a.desc[k]), 0);
-    const long long t0 = wall_clock64();
+    Deadline dl;
     while (src == kUnwritten) {
         if (__shfl(ld_coh(reinterpret_cast<const unsigned*>(qs + 4)), 0)) return kUnwritten;
-        if (wall_clock64() - t0 > wait) {
+        if (dl.expired(wait)) {
             fail_wait(a);
             return kUnwritten;
         }
@@ -2415,13 +2446,21 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
             k_queue_state_init<<<1, 64, 0, s>>>(qs, res_count, status);
             FPH_LAUNCH_CHECK();
         }
-        FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_reduce_queue_dev,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          (int)lds_b));
-        // 8 XCDs receive workgroups in turn: 8 x qwgs puts about qwgs on each
+        // 8 XCDs receive workgroups in turn: 8 x qwgs puts about qwgs on each;
         // bitmap working columns when the rows fit a wavefront's arena
-        k_reduce_queue_dev<<<(unsigned)(8 * qwgs), kPhaseBWaves * FPH_WAVE, lds_b, s>>>(
-            a, order, res_count, ring, qs, limit_ticks, bitmap, from_csr);
+        if (bitmap) {
+            FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_reduce_queue_dev<true>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)lds_b));
+            k_reduce_queue_dev<true><<<(unsigned)(8 * qwgs), kPhaseBWaves * FPH_WAVE, lds_b, s>>>(
+                a, order, res_count, ring, qs, limit_ticks, from_csr);
+        } else {
+            FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_reduce_queue_dev<false>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)lds_b));
+            k_reduce_queue_dev<false><<<(unsigned)(8 * qwgs), kPhaseBWaves * FPH_WAVE, lds_b, s>>>(
+                a, order, res_count, ring, qs, limit_ticks, from_csr);
+        }
         FPH_LAUNCH_CHECK();
         if (!from_csr) k_final_reset<<<fph_blocks(K, 256), 256, 0, s>>>(a);
         k_final_own<<<fph_blocks(std::max<int64_t>(nrows, 1), 256), 256, 0, s>>>(a);
