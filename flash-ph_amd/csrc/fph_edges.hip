@@ -81,7 +81,7 @@ struct ScreenSel {
                         memcpy(&bits, &acc[r][c], 4);
                         const unsigned long long colex = j * (j - 1) / 2 + i;
                         keys[at] = ((unsigned long long)bits << 32) | (colex & 0xFFFFFFFFull);
-                        vals[at] = (j << 32) | i;
+                        if (vals) vals[at] = (j << 32) | i;
                     }
                 }
                 pos += __popcll(b);

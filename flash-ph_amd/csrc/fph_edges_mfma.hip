@@ -321,7 +321,7 @@ __global__ __launch_bounds__(kThreadsM, 4) void k_edge_screen_mfma(
                             const unsigned long long uj = (unsigned long long)j;
                             const unsigned long long colex = uj * (uj - 1) / 2 + (unsigned)i;
                             keys[at] = ((unsigned long long)bits << 32) | (colex & 0xFFFFFFFFull);
-                            vals[at] = (uj << 32) | (unsigned long long)i;
+                            if (vals) vals[at] = (uj << 32) | (unsigned long long)i;
                         }
                     } else if (band) {  // validation mode: list the margin band
                         const unsigned long long at = atomicAdd(nband, 1ull);

@@ -11,7 +11,7 @@
 // / emit over edges, with no global sort:
 //     rank(triangle) = E     + tri_off[e] + (index of w in e's lower link)
 //     rank(tetra)    = E + T + tet_off[e] + (index of (y, x) in e's list)
-// Vertex lists per edge are stored sorted (tri_w: w; tet_key: y << 16 | x),
+// Vertex lists per edge are stored sorted (tri_w: w; tet_key: y << 32 | x),
 // so any simplex's rank is found with one binary search.
 #include "fph_common.h"
 
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(1024) void k_tet_emit(
     const int* __restrict__ tri_off, const int* __restrict__ tri_w,
     const int* __restrict__ tri_ru, const int* __restrict__ tri_rv,
     const unsigned char* __restrict__ cleared, const int* __restrict__ tet_off,
-    unsigned* __restrict__ tet_key, int* __restrict__ tet_e, int* __restrict__ tet_cmf) {
+    unsigned long long* __restrict__ tet_key, int* __restrict__ tet_e, int* __restrict__ tet_cmf) {
     const long long e = (long long)blockIdx.x * (blockDim.x / FPH_WAVE) + threadIdx.x / FPH_WAVE;
     if (e >= E) return;
     const int lane = fph_lane();
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(1024) void k_tet_emit(
                 if (!cleared[f_vxy] && f_vxy > cmf) cmf = f_vxy;
             }
             const int at = q + __popcll(m & lt);
-            tet_key[at] = ((unsigned)y << 16) | (unsigned)x;
+            tet_key[at] = ((unsigned long long)(unsigned)y << 32) | (unsigned)x;
             tet_e[at] = (int)e;
             tet_cmf[at] = cmf < 0 ? -1 : (int)(E + cmf);
         }
@@ -340,10 +340,12 @@ __global__ __launch_bounds__(1024) void k_tet_emit(
 // probed against the other two (staged in LDS when they fit), one probe per
 // lane.  kCount writes the column lengths, kEmit the tetrahedra's global
 // ranks (unsorted; sorted per column afterwards).
-__device__ __forceinline__ long long tet_rank_in(const int* tet_off, const unsigned* tet_key,
-                                                 int m, int p, int q) {
-    const unsigned key = p > q ? (((unsigned)p << 16) | (unsigned)q)
-                               : (((unsigned)q << 16) | (unsigned)p);
+__device__ __forceinline__ long long tet_rank_in(const int* tet_off,
+                                                 const unsigned long long* tet_key, int m, int p,
+                                                 int q) {
+    const unsigned long long key = p > q
+        ? (((unsigned long long)(unsigned)p << 32) | (unsigned)q)
+        : (((unsigned long long)(unsigned)q << 32) | (unsigned)p);
     const int lo = tet_off[m];
     return lo + fph_lower_bound(tet_key + lo, tet_off[m + 1] - lo, key);
 }
@@ -354,7 +356,8 @@ __global__ __launch_bounds__(1024) void k_tri_link(
     const int* __restrict__ aptr, const int* __restrict__ aidx, const int* __restrict__ arank,
     const int* __restrict__ tri_w, const int* __restrict__ tri_e,
     const unsigned char* __restrict__ cleared, const int* __restrict__ tet_off,
-    const unsigned* __restrict__ tet_key, int* __restrict__ cnt, const int* __restrict__ cof_off,
+    const unsigned long long* __restrict__ tet_key, int* __restrict__ cnt,
+    const int* __restrict__ cof_off,
     unsigned* __restrict__ cof_raw) {
     extern __shared__ int stage[];
     const int wid = threadIdx.x / FPH_WAVE, lane = fph_lane();
@@ -633,7 +636,7 @@ extern "C" int fph_tet_emit(int64_t E, const int32_t* edge_i, const int32_t* edg
                             const int32_t* adj_ptr, const int32_t* adj_idx,
                             const int32_t* adj_rank, const int32_t* tri_off,
                             const int32_t* tri_w, const int32_t* tri_ru, const int32_t* tri_rv,
-                            const uint8_t* cleared, const int32_t* tet_off, uint32_t* tet_key,
+                            const uint8_t* cleared, const int32_t* tet_off, uint64_t* tet_key,
                             int32_t* tet_e, int32_t* tet_cmf, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     if (E < 0) return FPH_ERR_ARG;
@@ -641,7 +644,7 @@ extern "C" int fph_tet_emit(int64_t E, const int32_t* edge_i, const int32_t* edg
     const int nw = link_waves(E);
     k_tet_emit<<<fph_blocks(E, nw), nw * FPH_WAVE, 0, s>>>(
         E, edge_i, edge_j, adj_ptr, adj_idx, adj_rank, tri_off, tri_w, tri_ru, tri_rv, cleared,
-        tet_off, tet_key, tet_e, tet_cmf);
+        tet_off, (unsigned long long*)tet_key, tet_e, tet_cmf);
     FPH_LAUNCH_CHECK();
     return FPH_OK;
 }
@@ -673,7 +676,7 @@ extern "C" int fph_cofacets_h2(int64_t T, int64_t E, int64_t Q, const int32_t* e
                                const int32_t* adj_idx, const int32_t* adj_rank,
                                const int32_t* tri_w, const int32_t* tri_e,
                                const uint8_t* cleared, const int32_t* tet_off,
-                               const uint32_t* tet_key, const int32_t* cof_off, int64_t C,
+                               const uint64_t* tet_key, const int32_t* cof_off, int64_t C,
                                uint32_t* cof_data, void* ws, size_t* ws_bytes, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     if (T < 0 || C < 0 || !ws_bytes) return FPH_ERR_ARG;
@@ -687,7 +690,7 @@ extern "C" int fph_cofacets_h2(int64_t T, int64_t E, int64_t Q, const int32_t* e
     const int nw = link_waves(T);
     k_tri_link<1><<<fph_blocks(T, nw), nw * FPH_WAVE, nw * kLinkStage * 4, s>>>(
         T, E, edge_i, edge_j, adj_ptr, adj_idx, adj_rank, tri_w, tri_e, cleared, tet_off,
-        tet_key, nullptr, cof_off, raw);
+        (const unsigned long long*)tet_key, nullptr, cof_off, raw);
     FPH_LAUNCH_CHECK();
     size_t t = tmp;
     const int end_bit = std::max(1, bit_width((unsigned long long)(E + T + Q)));

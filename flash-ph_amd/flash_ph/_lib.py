@@ -44,6 +44,7 @@ SIGNATURES = {
     "fph_boruvka_scratch_bytes": (_SZ, [_I64]),
     "fph_boruvka_mst": (_I32, [_P, _P, _I64, _I64, _P, _P, _P, _P, _SZ, _P]),
     "fph_boruvka_mst_noalloc": (_I32, [_P, _P, _I64, _I64, _P, _P, _P, _P]),
+    "fph_h0_from_keys": (_I32, [_P, _I64, _I64, _P, _P, _P, _PSZ, _P]),
     "fph_h0_diagram": (_I32, [_P, _P, _I64, _I64, _P, _P, _P, _P, _PSZ, _P]),
     "fph_triangle_count": (_I32, [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _PSZ, _P]),
     "fph_triangle_emit": (_I32, [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),

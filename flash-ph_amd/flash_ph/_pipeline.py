@@ -295,6 +295,50 @@ def fp64_screen_report(pts: torch.Tensor, max_edge_length: float) -> dict:
             "max_ulp_fp32_vs_fp64": e[5] / 1024.0}
 
 
+# Largest cloud whose packed edge keys are unique and decode back to (i, j)
+# (the colex field j (j - 1) / 2 + i stays below 2^32).
+MAX_KEY_DECODE_N = 92682
+
+
+def h0_direct(pts: torch.Tensor, max_edge_length: float) -> torch.Tensor:
+    """H0 diagram straight from the screened edge keys (fph_h0_from_keys):
+    Boruvka on the keys, only the forest sorted -- no edge sort, no CSR.
+    Same rows as build_edges + build_mst (persistence.py:140-186), and the
+    edge count may pass 2^31 (8 bytes per edge in HBM).  n <= 92,682."""
+    lib = load()
+    if pts.dtype != torch.float32 or not pts.is_contiguous():
+        raise ValueError("h0_direct needs a contiguous float32 (n, d) tensor")
+    dev = pts.device
+    s = stream_of(dev)
+    n, d = pts.shape
+    if n > MAX_KEY_DECODE_N:
+        raise ValueError(f"h0_direct needs n <= {MAX_KEY_DECODE_N}, got {n}")
+    thr = threshold_sq_f32(max_edge_length)
+    cap = max(_EDGE_CAP_MIN, _EDGE_CAP_PER_POINT * n)
+    cnt = _empty(torch.int64, dev, 2)
+    mfma = use_mfma_screen(n, d)
+    while True:
+        keys = _empty(torch.int64, dev, cap)
+        if mfma:
+            call_ws("fph_edge_screen_mfma",
+                    (ptr(pts), n, d, thr, ptr(keys), None, cap, ptr(cnt), cnt.data_ptr() + 8),
+                    (s,), dev)
+        else:
+            check(lib.fph_edge_screen(ptr(pts), n, d, thr, ptr(keys), None, cap, ptr(cnt), s),
+                  "fph_edge_screen")
+        E = int(cnt[0].item())
+        if E <= cap:
+            break
+        del keys
+        cap = E
+    h0 = _empty(torch.float32, dev, n, 2)
+    conv = _empty(torch.int32, dev, 1)
+    call_ws("fph_h0_from_keys", (ptr(keys), E, n, ptr(h0), ptr(conv)), (s,), dev)
+    if not int(conv.item()):
+        raise _lib.FlashPHError("Boruvka did not converge")
+    return h0
+
+
 def build_mst(cx: Complex, want_labels: bool = False) -> None:
     """Stage (b): Boruvka forest, component labels and the H0 diagram."""
     lib = load()
@@ -640,7 +684,7 @@ def build_h2(cx: Complex) -> None:
     if E + T + Q >= 2**31 - 1:
         raise ValueError("simplex count exceeds the int32 rank range")
     cx.Q, cx.C2 = Q, C2
-    cx.tet_key = _empty(torch.int32, dev, max(Q, 1))
+    cx.tet_key = _empty(torch.int64, dev, max(Q, 1))
     cx.tet_e = _empty(torch.int32, dev, max(Q, 1))
     cx.tet_cmf = _empty(torch.int32, dev, max(Q, 1))
     cx.cof2_data = _empty(torch.int32, dev, max(C2, 1))

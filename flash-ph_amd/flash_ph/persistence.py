@@ -69,6 +69,8 @@ def rips_persistence(
     max_edge_length: float,
     max_dim: int = 1,
     backend: str = 'auto',
+    *,
+    large_h2: bool = False,
 ) -> list[Tensor]:
     """Exact Vietoris-Rips persistent homology (H0 + H1 + optional H2).
 
@@ -92,6 +94,12 @@ def rips_persistence(
         complete reduction (equal to ``'gpu'`` whenever the reference's store
         does not overflow); ``'giotto'`` hands H1 to giotto-ph.  For
         ``max_dim=2`` H1 and H2 are complete reductions, as giotto-ph's.
+
+    large_h2 : bool, keyword only
+        Allow ``max_dim=2`` for n >= 65536.  The reference packs tetrahedron
+        vertices in 16 bits and raises ValueError there
+        (persistence.py:125-129), which stays the default; this build packs
+        them in 32 bits.
 
     Returns
     -------
@@ -119,7 +127,7 @@ def rips_persistence(
         )
     if max_dim not in (0, 1, 2):
         raise ValueError(f"max_dim must be 0, 1, or 2, got {max_dim}")
-    if max_dim == 2 and points.shape[0] >= (1 << _VERTEX_PACK_BITS):
+    if max_dim == 2 and points.shape[0] >= (1 << _VERTEX_PACK_BITS) and not large_h2:
         raise ValueError(
             f"H2 requires n < {1 << _VERTEX_PACK_BITS} (16-bit vertex packing), "
             f"got n={points.shape[0]}. Use max_dim=1 or reduce n."
@@ -165,6 +173,10 @@ def rips_persistence(
                               cx.adj_rank, cx.adj_dsq, n)
         h1 = _rips_h1h2_giotto(filt, n, 1, max_edge_length, pts.device)[0]
         out = [cx.h0, h1]
+    elif max_dim == 0 and n <= _pipeline.MAX_KEY_DECODE_N:
+        # H0 needs only the forest: Boruvka on the screened keys, no edge
+        # sort or CSR (and no 2^31 edge limit)
+        out = [_pipeline.h0_direct(pts, max_edge_length)]
     else:
         semantics = "exact" if backend == "exact" else "reference"
         out, _ = _pipeline.compute(pts, max_edge_length, max_dim, semantics=semantics)

@@ -42,7 +42,8 @@ const char *fph_version(void);
  * Screen all pairs i < j of `pts` (n x d, row-major FP32) and append those
  * with dsq < thr_sq, where dsq = fma-chain sum_k (x_i[k]-x_j[k])^2 in FP32,
  * as unordered (key, val) pairs:
- *   key = float_bits(dsq) << 32 | (j(j-1)/2 + i) mod 2^32,  val = j << 32 | i.
+ *   key = float_bits(dsq) << 32 | (j(j-1)/2 + i) mod 2^32,  val = j << 32 | i
+ * (vals may be NULL: keys only).
  * *count_dev receives the total number of qualifying pairs even when it
  * exceeds `cap` (only the first `cap` are written).
  * Replaces kernels/edge_kernel.py:23-149 (count + emit kernels) and
@@ -176,6 +177,21 @@ int fph_boruvka_mst_noalloc(const int32_t *edge_i, const int32_t *edge_j, int64_
 int fph_h0_diagram(const uint8_t *mst_mask, const float *edge_dsq, int64_t E, int64_t n,
                    float *h0, int64_t *mst_idx, int64_t *n_mst, void *ws, size_t *ws_bytes,
                    void *stream);
+
+/*
+ * H0 (n x 2 FP32, the rows of fph_h0_diagram) straight from E screened keys
+ * in any order (fph_edge_screen / fph_edge_screen_mfma keys; vals may be
+ * NULL there): Boruvka runs on the keys themselves -- unique, and decoded
+ * back to (i, j), while n <= 92682 -- with 64-bit minima per component;
+ * only the forest (at most n - 1 keys) is sorted.  No edge sort and no CSR,
+ * so E may exceed 2^31 (the keys stay in HBM, 8 bytes per edge).
+ * *converged as for fph_boruvka_mst.  FPH_ERR_LIMIT for n > 92682.
+ * Replaces, for max_dim = 0, rips_filtration + gpu_boruvka_mst + the H0
+ * assembly (rips.py:98-218, boruvka.py:17-113, persistence.py:140-186),
+ * without the reference's E < 2^31 assertion (kernels/edge_kernel.py:203).
+ */
+int fph_h0_from_keys(const uint64_t *keys, int64_t E, int64_t n, float *h0, int32_t *converged,
+                     void *ws, size_t *ws_bytes, void *stream);
 
 /* ---- stage (d): triangles and H1 cofacet columns ---------------------- */
 
@@ -394,9 +410,10 @@ int fph_tet_count(int64_t E, const int32_t *adj_ptr, const int32_t *adj_idx,
 
 /*
  * Tetrahedra in (max_rank, colex4) order (global rank E + T + q):
- * tet_key = y << 16 | x for the two vertices off the largest edge (requires
- * n < 65536), tet_e = largest edge, tet_cmf = largest uncleared triangle
- * face (global rank) or -1.
+ * tet_key = y << 32 | x for the two vertices off the largest edge (any n;
+ * the reference packs 16-bit fields and so needs n < 65536,
+ * persistence.py:125-129), tet_e = largest edge, tet_cmf = largest
+ * uncleared triangle face (global rank) or -1.
  * Replaces _tet_emit_kernel (kernels/triangle_kernel.py:206-270) and the
  * colex4 order of kernels/apparent_pair_h2_kernel.py:52-58.
  */
@@ -404,7 +421,7 @@ int fph_tet_emit(int64_t E, const int32_t *edge_i, const int32_t *edge_j,
                  const int32_t *adj_ptr, const int32_t *adj_idx, const int32_t *adj_rank,
                  const int32_t *tri_off, const int32_t *tri_w, const int32_t *tri_ru,
                  const int32_t *tri_rv, const uint8_t *cleared, const int32_t *tet_off,
-                 uint32_t *tet_key, int32_t *tet_e, int32_t *tet_cmf, void *stream);
+                 uint64_t *tet_key, int32_t *tet_e, int32_t *tet_cmf, void *stream);
 
 /*
  * cof_off (T+1) = exclusive scan of the number of tetrahedra containing each
@@ -425,7 +442,7 @@ int fph_cofacets_h2_count(int64_t T, const int32_t *edge_i, const int32_t *edge_
 int fph_cofacets_h2(int64_t T, int64_t E, int64_t Q, const int32_t *edge_i,
                     const int32_t *edge_j, const int32_t *adj_ptr, const int32_t *adj_idx,
                     const int32_t *adj_rank, const int32_t *tri_w, const int32_t *tri_e,
-                    const uint8_t *cleared, const int32_t *tet_off, const uint32_t *tet_key,
+                    const uint8_t *cleared, const int32_t *tet_off, const uint64_t *tet_key,
                     const int32_t *cof_off, int64_t C, uint32_t *cof_data, void *ws,
                     size_t *ws_bytes, void *stream);
 

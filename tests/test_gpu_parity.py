@@ -82,10 +82,10 @@ def _check_case(pts, t, max_dim, semantics="reference"):
     assert Q == ref["stats"]["Q"]
     if Q:
         qe = _cpu(cx.tet_e)[:Q]
-        key = _cpu(cx.tet_key)[:Q].view(np.uint32)
+        key = _cpu(cx.tet_key)[:Q].view(np.uint64)
         ours = np.sort(np.stack([_cpu(cx.edge_i)[qe], _cpu(cx.edge_j)[qe],
-                                 (key >> 16).astype(np.int32), (key & 0xFFFF).astype(np.int32)],
-                                1), axis=1)
+                                 (key >> np.uint64(32)).astype(np.int32),
+                                 (key & np.uint64(0xFFFFFFFF)).astype(np.int32)], 1), axis=1)
         order = np.argsort(ref["tet_global_rank"])
         theirs = np.stack([ref["tet_v0"], ref["tet_v1"], ref["tet_v2"], ref["tet_v3"]], 1)[order]
         _eq("tetrahedra_by_rank", ours, theirs)
