@@ -185,6 +185,25 @@ __global__ void k_emit_order(const unsigned long long* __restrict__ vals, long l
     idx[e] = (unsigned)e;
 }
 
+// Tie key of the exact order: the full colex index j (j - 1) / 2 + i.
+__global__ void k_colex_order(const unsigned long long* __restrict__ vals, long long E,
+                              unsigned long long* __restrict__ ord, unsigned* __restrict__ idx) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const unsigned long long v = vals[e];
+    const unsigned long long i = v & 0xFFFFFFFFull, j = v >> 32;
+    ord[e] = j * (j - 1) / 2 + i;
+    idx[e] = (unsigned)e;
+}
+
+// Exact order: the key's low half becomes the edge's position in the colex
+// pre-order, so one full-width sort gives (dsq, colex) with unique keys.
+__global__ void k_rank_low(unsigned long long* __restrict__ keys, long long E) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    keys[e] = (keys[e] & 0xFFFFFFFF00000000ull) | (unsigned long long)e;
+}
+
 __global__ void k_gather_kv(const unsigned long long* __restrict__ keys,
                             const unsigned long long* __restrict__ vals,
                             const unsigned* __restrict__ idx, long long E,
@@ -326,11 +345,12 @@ extern "C" int fph_edge_screen(const float* pts, int64_t n, int d, float thr_sq,
     }
 }
 
-extern "C" int fph_edge_sort(const uint64_t* keys_in, const uint64_t* vals_in, int64_t E,
-                             int64_t n, uint64_t* keys_tmp, uint64_t* vals_tmp,
-                             int32_t* edge_i, int32_t* edge_j, float* edge_dsq,
-                             float* edge_filt, void* ws, size_t* ws_bytes, void* stream) {
-    hipStream_t s = (hipStream_t)stream;
+// exact_colex = 0: the reference's order (packed key, ties in emission
+// order); 1: (dsq, full colex) -- the two differ only for n > 92682.
+static int edge_sort(const uint64_t* keys_in, const uint64_t* vals_in, int64_t E, int64_t n,
+                     uint64_t* keys_tmp, uint64_t* vals_tmp, int32_t* edge_i, int32_t* edge_j,
+                     float* edge_dsq, float* edge_filt, void* ws, size_t* ws_bytes,
+                     hipStream_t s, int exact_colex) {
     if (E < 0 || !ws_bytes) return FPH_ERR_ARG;
     if (E >= 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
     const bool tie_order = n > 92682;
@@ -364,7 +384,10 @@ extern "C" int fph_edge_sort(const uint64_t* keys_in, const uint64_t* vals_in, i
         unsigned* idx = (unsigned*)(w + tmp + 2 * align_up(8 * (size_t)E));
         unsigned* idx2 = (unsigned*)(w + tmp + 2 * align_up(8 * (size_t)E) + align_up(4 * (size_t)E));
         const long long nb32 = (n + 31) / 32;
-        k_emit_order<<<fph_blocks(E, 256), 256, 0, s>>>(v_src, E, nb32, ord, idx);
+        if (exact_colex)
+            k_colex_order<<<fph_blocks(E, 256), 256, 0, s>>>(v_src, E, ord, idx);
+        else
+            k_emit_order<<<fph_blocks(E, 256), 256, 0, s>>>(v_src, E, nb32, ord, idx);
         FPH_LAUNCH_CHECK();
         size_t t2 = tmp;
         FPH_CHECK_HIP(rocprim::radix_sort_pairs(ws, t2, ord, ord2, idx, idx2, N, 0, 64, s));
@@ -372,7 +395,12 @@ extern "C" int fph_edge_sort(const uint64_t* keys_in, const uint64_t* vals_in, i
         // sorted back into ord/idx space reinterpreted as u64 pairs below.
         k_gather_kv<<<fph_blocks(E, 256), 256, 0, s>>>(k_src, v_src, idx2, E, kk, vv);
         FPH_LAUNCH_CHECK();
-        // Final stable sort on the packed key: emission order breaks ties.
+        // Final stable sort on the packed key (emission order breaks ties),
+        // or, exact order, on dsq bits << 32 | colex position (unique keys).
+        if (exact_colex) {
+            k_rank_low<<<fph_blocks(E, 256), 256, 0, s>>>(kk, E);
+            FPH_LAUNCH_CHECK();
+        }
         unsigned long long* kout = ord;   // reuse (E u64 each)
         unsigned long long* vout = ord2;
         size_t t3 = tmp;
@@ -388,6 +416,22 @@ extern "C" int fph_edge_sort(const uint64_t* keys_in, const uint64_t* vals_in, i
                                                      edge_filt);
     FPH_LAUNCH_CHECK();
     return FPH_OK;
+}
+
+extern "C" int fph_edge_sort(const uint64_t* keys_in, const uint64_t* vals_in, int64_t E,
+                             int64_t n, uint64_t* keys_tmp, uint64_t* vals_tmp,
+                             int32_t* edge_i, int32_t* edge_j, float* edge_dsq,
+                             float* edge_filt, void* ws, size_t* ws_bytes, void* stream) {
+    return edge_sort(keys_in, vals_in, E, n, keys_tmp, vals_tmp, edge_i, edge_j, edge_dsq,
+                     edge_filt, ws, ws_bytes, (hipStream_t)stream, 0);
+}
+
+extern "C" int fph_edge_sort_colex64(const uint64_t* keys_in, const uint64_t* vals_in, int64_t E,
+                                     int64_t n, uint64_t* keys_tmp, uint64_t* vals_tmp,
+                                     int32_t* edge_i, int32_t* edge_j, float* edge_dsq,
+                                     float* edge_filt, void* ws, size_t* ws_bytes, void* stream) {
+    return edge_sort(keys_in, vals_in, E, n, keys_tmp, vals_tmp, edge_i, edge_j, edge_dsq,
+                     edge_filt, ws, ws_bytes, (hipStream_t)stream, 1);
 }
 
 extern "C" int fph_vertex_csr(const int32_t* edge_i, const int32_t* edge_j,

@@ -38,6 +38,7 @@ SIGNATURES = {
                                          _P, _P, _PSZ, _P]),
     "fph_pairs_fp64_check": (_I32, [_P, _I64, _I32, _F64, _F32, _P, _I64, _P, _P]),
     "fph_edge_sort": (_I32, [_P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _PSZ, _P]),
+    "fph_edge_sort_colex64": (_I32, [_P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _PSZ, _P]),
     "fph_vertex_csr": (_I32, [_P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _PSZ, _P]),
     "fph_coo_export": (_I32, [_P, _P, _P, _I64, _I64, _P, _P, _P, _P]),
     "fph_boruvka_rounds": (_I32, [_I64]),

@@ -191,8 +191,11 @@ class Complex:
     h2_edges: object = None
 
 
-def build_edges(pts: torch.Tensor, max_edge_length: float) -> Complex:
-    """Stage (a): screened edges in filtration order plus the vertex CSR."""
+def build_edges(pts: torch.Tensor, max_edge_length: float, colex64: bool = False) -> Complex:
+    """Stage (a): screened edges in filtration order plus the vertex CSR.
+    colex64: ties in dsq by the full colex index (fph_edge_sort_colex64)
+    instead of the reference's truncated key and emission order; the same
+    order for n <= 92,682."""
     lib = load()
     if pts.dtype != torch.float32 or not pts.is_contiguous():
         raise ValueError("build_edges needs a contiguous float32 (n, d) tensor")
@@ -232,7 +235,7 @@ def build_edges(pts: torch.Tensor, max_edge_length: float) -> Complex:
     if E > 0:
         ktmp = _empty(torch.int64, dev, E)
         vtmp = _empty(torch.int64, dev, E)
-        call_ws("fph_edge_sort",
+        call_ws("fph_edge_sort_colex64" if colex64 else "fph_edge_sort",
                 (ptr(keys), ptr(vals), E, n, ptr(ktmp), ptr(vtmp), ptr(cx.edge_i),
                  ptr(cx.edge_j), ptr(cx.edge_dsq), ptr(cx.edge_filt)), (s,), dev)
         del ktmp, vtmp
@@ -711,9 +714,10 @@ def build_h2(cx: Complex) -> None:
 
 
 def compute(pts: torch.Tensor, max_edge_length: float, max_dim: int,
-            semantics: str = "exact", want_edges: bool = False) -> tuple[list, Complex]:
+            semantics: str = "exact", want_edges: bool = False,
+            colex64: bool = False) -> tuple[list, Complex]:
     """Diagrams [H0, ..., H_max_dim] of a CUDA point cloud with n >= 2."""
-    cx = build_edges(pts, max_edge_length)
+    cx = build_edges(pts, max_edge_length, colex64=colex64)
     cx.want_edges = want_edges
     build_mst(cx)
     out = [cx.h0]

@@ -71,6 +71,7 @@ def rips_persistence(
     backend: str = 'auto',
     *,
     large_h2: bool = False,
+    exact_colex: bool = False,
 ) -> list[Tensor]:
     """Exact Vietoris-Rips persistent homology (H0 + H1 + optional H2).
 
@@ -100,6 +101,11 @@ def rips_persistence(
         vertices in 16 bits and raises ValueError there
         (persistence.py:125-129), which stays the default; this build packs
         them in 32 bits.
+    exact_colex : bool, keyword only
+        For n > 92,682, break ties between equal edge lengths by the full
+        colex index j(j-1)/2 + i instead of the reference's 32-bit
+        truncation (rips.py:36-49) and emission order.  Changes the row
+        order within equal births, never the intervals.
 
     Returns
     -------
@@ -179,7 +185,8 @@ def rips_persistence(
         out = [_pipeline.h0_direct(pts, max_edge_length)]
     else:
         semantics = "exact" if backend == "exact" else "reference"
-        out, _ = _pipeline.compute(pts, max_edge_length, max_dim, semantics=semantics)
+        out, _ = _pipeline.compute(pts, max_edge_length, max_dim, semantics=semantics,
+                                   colex64=exact_colex)
     if device.type != "cuda" or out[0].device != device:
         out = [d.to(device) for d in out]
     return out

@@ -112,6 +112,19 @@ int fph_edge_sort(const uint64_t *keys_in, const uint64_t *vals_in, int64_t E, i
                   float *edge_dsq, float *edge_filt, void *ws, size_t *ws_bytes, void *stream);
 
 /*
+ * fph_edge_sort with ties in dsq broken by the full 64-bit colex index
+ * j (j - 1) / 2 + i instead of the reference's 32-bit truncation and
+ * emission order: the order (dsq, j, i).  Identical to fph_edge_sort for
+ * n <= 92682; beyond, an opt-in departure from the reference's order
+ * (rips.py:36-49 notes the truncation) that keeps every diagram's multiset
+ * of intervals.
+ */
+int fph_edge_sort_colex64(const uint64_t *keys_in, const uint64_t *vals_in, int64_t E, int64_t n,
+                          uint64_t *keys_tmp, uint64_t *vals_tmp, int32_t *edge_i,
+                          int32_t *edge_j, float *edge_dsq, float *edge_filt, void *ws,
+                          size_t *ws_bytes, void *stream);
+
+/*
  * Per-vertex CSR of the sorted edges: adj_ptr (n+1), and per half-edge the
  * neighbour (ascending within a row), its edge rank and dsq (2E each).
  * Replaces _build_vertex_csr, rips.py:53-95.

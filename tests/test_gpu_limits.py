@@ -93,3 +93,35 @@ def test_h2_beyond_65536_points():
         got = dg[k].cpu().numpy()
         want = ref["diagrams"][k]
         assert got.shape == want.shape and np.array_equal(_u32(got), _u32(want)), f"H{k}"
+
+
+def test_exact_colex_order_and_invariance():
+    """exact_colex=True (n > 92,682): edges in (dsq, j, i) order; the
+    diagrams' intervals are those of the reference's order (a filtration's
+    diagram does not depend on how equal values are ordered), which itself
+    matches the oracle bit for bit, row order included."""
+    import oracle
+    from flash_ph import _pipeline, rips_persistence
+    pts = cases.grid_points(47)
+    print("[gpu] exact colex order on the grid", flush=True)
+    cx = _pipeline.build_edges(torch.from_numpy(pts).cuda(), 1.5, colex64=True)
+    i, j, dsq = cases.grid_edges(pts, 1.5)
+    order = np.lexsort((i, j, dsq.view(np.uint32)))
+    assert np.array_equal(cx.edge_i.cpu().numpy(), i[order])
+    assert np.array_equal(cx.edge_j.cpu().numpy(), j[order])
+    assert not np.array_equal(order, cases.reference_edge_order(i, j, dsq))
+    rng = np.random.default_rng(13)
+    flat = rng.choice(1000 * 1000, 93_000, replace=False)
+    pts2 = np.column_stack([flat // 1000, flat % 1000]).astype(np.float32)
+    x = torch.from_numpy(pts2).cuda()
+    print("[gpu] exact colex invariance n=93000", flush=True)
+    ref_dg = rips_persistence(x, 2.9, max_dim=1)
+    ex_dg = rips_persistence(x, 2.9, max_dim=1, exact_colex=True)
+    c2 = _pipeline.build_edges(x, 2.9)
+    ei, ej, ed = (a.cpu().numpy() for a in (c2.edge_i, c2.edge_j, c2.edge_dsq))
+    ref = oracle.run(pts2, 2.9, 1, edges=(ei, ej, ed))
+    for k in range(2):
+        a, b = ref_dg[k].cpu().numpy(), ex_dg[k].cpu().numpy()
+        assert np.array_equal(_u32(a), _u32(ref["diagrams"][k])), f"H{k} vs oracle"
+        key = lambda r: r[np.lexsort((r[:, 1], r[:, 0]))]
+        assert np.array_equal(_u32(key(a)), _u32(key(b))), f"H{k} multiset"
