@@ -1,7 +1,9 @@
-"""Build libflashph.so (gfx950) in-tree from flash-ph_amd/csrc.
+"""Build libflashph.so (gfx950) in-tree from flash-ph_amd/csrc, and the host
+pipeline libflashph_cpu.so from flash-ph_amd/cpu.
 
 Each .hip file is compiled to an object with hipcc, in parallel, then linked
-into flash-ph_amd/flash_ph/_lib/libflashph.so.  Always recompiles.
+into flash-ph_amd/flash_ph/_lib/libflashph.so; the CPU library is one g++
+invocation (OpenMP).  Always recompiles.
 """
 from __future__ import annotations
 
@@ -36,6 +38,26 @@ def _compile(src: str) -> str:
     return obj
 
 
+CPU_SRC = os.path.join(HERE, "cpu", "fph_cpu.cpp")
+CPU_LIB = os.path.join(OUT_DIR, "libflashph_cpu.so")
+CXX = os.environ.get("CXX", "g++")
+# -ffp-contract=off: the only fused operations are the explicit fmaf calls
+# of the distance chain (hardware FMA with -mfma, single rounding either way)
+CPU_FLAGS = ["-O3", "-std=c++17", "-fopenmp", "-ffp-contract=off", "-mfma", "-mavx2",
+             "-fPIC", "-shared", "-Wall", "-Wno-unused-function"]
+
+
+def build_cpu() -> str:
+    """libflashph_cpu.so: the host pipeline for clouds in host memory."""
+    os.makedirs(OUT_DIR, exist_ok=True)
+    tmp = CPU_LIB + ".tmp"
+    r = subprocess.run([CXX, *CPU_FLAGS, "-o", tmp, CPU_SRC], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for {CPU_SRC}:\n{r.stderr[-6000:]}")
+    os.replace(tmp, CPU_LIB)
+    return CPU_LIB
+
+
 def build(jobs: int | None = None) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
     os.makedirs(OBJ_DIR, exist_ok=True)
@@ -49,6 +71,7 @@ def build(jobs: int | None = None) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(tmp, LIB)
+    build_cpu()
     return LIB
 
 

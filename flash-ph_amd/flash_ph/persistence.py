@@ -39,6 +39,35 @@ def _rips_h1h2_giotto(filt, n, max_dim, max_edge_length, device):
     return out
 
 
+def _rips_persistence_cpu(points, max_edge_length, max_dim, backend, exact_colex):
+    """Diagrams of a cloud in host memory with libflashph_cpu (OpenMP).  H1
+    and H2 are complete reductions, the pairs giotto-ph returns on the
+    reference's CPU route (persistence.py:197-206); backend='giotto' hands
+    them to giotto-ph itself."""
+    from flash_ph import _cpu
+    device = points.device
+    pts = points.detach().float().contiguous().numpy()
+    thr = _pipeline.threshold_sq_f32(max_edge_length)
+    if max_dim >= 1 and backend == "giotto":
+        out = _cpu.rips(pts, thr, 0, exact_colex=exact_colex)
+        f = _cpu.filtration(pts, thr, exact_colex=exact_colex)
+        filt = _HostFiltration(f, pts.shape[0])
+        h = _rips_h1h2_giotto(filt, pts.shape[0], max_dim, max_edge_length, device)
+        return [torch.from_numpy(out[0]).to(device)] + h
+    dg = _cpu.rips(pts, thr, max_dim, exact_h1=True, exact_colex=exact_colex)
+    return [torch.from_numpy(d).to(device) for d in dg]
+
+
+class _HostFiltration:
+    """The fields coo_distance_matrix reads, from the host filtration."""
+
+    def __init__(self, f, n):
+        self.n = n
+        self.edge_i = torch.from_numpy(f["edge_i"])
+        self.edge_j = torch.from_numpy(f["edge_j"])
+        self.edge_dist_sq = torch.from_numpy(f["edge_dsq"])
+
+
 def coo_distance_matrix(filt):
     """scipy (n, n) COO sparse distance matrix of a RipsFiltration: one entry
     (i, j, |x_i - x_j|) per edge (i < j; FP32 correctly rounded sqrt of the
@@ -49,8 +78,14 @@ def coo_distance_matrix(filt):
     n = int(filt.n)
     E = int(filt.edge_i.shape[0])
     dev = filt.edge_i.device
-    if dev.type != "cuda":
-        raise ValueError("coo_distance_matrix needs a filtration on the GPU")
+    if dev.type != "cuda":   # host filtration: the same triplets with numpy
+        i = filt.edge_i.numpy().astype(np.int64)
+        j = filt.edge_j.numpy().astype(np.int64)
+        v = np.sqrt(filt.edge_dist_sq.numpy().astype(np.float64)).astype(np.float32)
+        diag = np.arange(n, dtype=np.int64)
+        return scipy.sparse.coo_matrix(
+            (np.concatenate([v, np.zeros(n, np.float32)]),
+             (np.concatenate([i, diag]), np.concatenate([j, diag]))), shape=(n, n))
     lib = _lib.load()
     row = torch.empty(E + n, dtype=torch.int64, device=dev)
     col = torch.empty(E + n, dtype=torch.int64, device=dev)
@@ -79,8 +114,10 @@ def rips_persistence(
     ----------
     points : (n, d) array-like
         Point cloud coordinates (any d >= 1).  NumPy arrays are converted to
-        a float32 tensor on the GPU.  CPU tensors are computed on the GPU and
-        the diagrams returned on the CPU.
+        a float32 tensor on the GPU when one is visible.  CUDA tensors run on
+        the GPU (gfx950 HIP library); CPU tensors run on the host pipeline
+        (C++/OpenMP, the same arithmetic: bit-identical diagrams), as the
+        reference computes CPU tensors on the CPU.
     max_edge_length : float
         Maximum edge length (strict < inequality).
     max_dim : int
@@ -160,10 +197,13 @@ def rips_persistence(
             f"backend must be one of {_VALID_BACKENDS}, got {backend!r}"
         )
 
+    if device.type != "cuda":
+        # host memory: the host pipeline, as the reference computes CPU
+        # tensors on the CPU (rips.py:221-233, persistence.py:197-206)
+        return _rips_persistence_cpu(points, max_edge_length, max_dim, backend, exact_colex)
+
     _lib.require_gpu()
     pts = points.float().contiguous()
-    if pts.device.type != "cuda":
-        pts = pts.cuda()
 
     if torch.is_grad_enabled() and points.requires_grad and backend != "giotto":
         from flash_ph.autograd import rips_persistence_diff

@@ -80,7 +80,8 @@ def _build_vertex_csr(edge_i: Tensor, edge_j: Tensor, edge_rank: Tensor,
 
 
 def rips_filtration(points: Tensor, max_edge_length: float) -> RipsFiltration:
-    """Construct a Vietoris-Rips filtration on the GPU (reference rips.py:98-218).
+    """Construct a Vietoris-Rips filtration (reference rips.py:98-218): on the
+    GPU for CUDA tensors, on the host pipeline for CPU tensors.
 
     Parameters
     ----------
@@ -102,6 +103,19 @@ def rips_filtration(points: Tensor, max_edge_length: float) -> RipsFiltration:
     device = points.device
     if n <= 1:
         return _empty_filtration(n, device)
+    if device.type != "cuda":
+        # host memory: the host pipeline (the reference's CPU route,
+        # rips.py:221-233, with the GPU's FMA-form distances)
+        from flash_ph import _cpu
+        h = _cpu.filtration(points.detach().float().contiguous().numpy(),
+                            _pipeline.threshold_sq_f32(max_edge_length))
+        if h["edge_i"].size == 0:
+            return _empty_filtration(n, device)
+        t = {k: torch.from_numpy(v) for k, v in h.items()}
+        return RipsFiltration(edge_i=t["edge_i"], edge_j=t["edge_j"],
+                              edge_dist_sq=t["edge_dsq"], vert_adj_ptr=t["adj_ptr"],
+                              vert_adj_idx=t["adj_idx"], vert_adj_rank=t["adj_rank"],
+                              vert_adj_dist_sq=t["adj_dsq"], n=n)
     _lib.require_gpu()
     pts = points.float().contiguous()
     if pts.device.type != "cuda":

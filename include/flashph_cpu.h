@@ -1,0 +1,92 @@
+/*
+ * flashph_cpu.h -- C ABI of the host (CPU) Vietoris-Rips pipeline
+ * (libflashph_cpu.so, built from flash-ph_amd/cpu, OpenMP).
+ *
+ * For point clouds held in host memory: the reference computes CPU tensors
+ * on the CPU (rips.py:221-233, persistence.py:197-206).  Same filtration,
+ * orders and pairing as libflashph.so (flashph.h), so the diagrams are
+ * bit-identical to the GPU's.  All pointers are host pointers; nothing here
+ * touches a GPU.  Reference citations are relative to src/flash_ph/.
+ */
+#ifndef FLASHPH_CPU_H
+#define FLASHPH_CPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FPHC_OK 0
+#define FPHC_ERR_ARG (-1)   /* invalid argument */
+#define FPHC_ERR_LIMIT (-3) /* ranks beyond the int32 range */
+#define FPHC_ERR_MEM (-5)   /* host memory exhausted */
+
+/* stats slots of fphc_stats */
+#define FPHC_STAT_E 0
+#define FPHC_STAT_T 1
+#define FPHC_STAT_Q 2
+#define FPHC_STAT_XOR1 3
+#define FPHC_STAT_XOR2 4
+#define FPHC_STAT_DROPS 5
+#define FPHC_STAT_TRUNC 6
+#define FPHC_NSTATS 8
+
+typedef struct fphc_result fphc_result;
+
+const char *fphc_version(void);
+
+/*
+ * Diagrams H0..H_max_dim (max_dim <= 2) of n points (n x d row-major FP32)
+ * under the strict threshold thr_sq = fp32(t^2).  exact_h1 = 0: H1 with the
+ * reference's capacity-limited reduction (its store, MAX_COL = max_col,
+ * store cap = max_pool; 0 = its values 1048576 and 1e8); 1: the complete
+ * reduction (also used for max_dim = 2, H1 and H2).  exact_colex: ties in
+ * dsq by the full colex index for n > 92682 (as fph_edge_sort_colex64).
+ * threads <= 0: OpenMP's default.  *out receives the result (free with
+ * fphc_free).
+ * Replaces rips_filtration's CPU route (rips.py:221-233), gpu_boruvka_mst
+ * on CPU tensors (boruvka.py:17-113) and the giotto-ph H1/H2 route the
+ * reference takes for CPU tensors (persistence.py:197-206), with the
+ * pipeline of reduce_h1_gpu.py:438-604 and cohomology_general.py:202-371.
+ */
+int fphc_rips(const float *pts, int64_t n, int d, float thr_sq, int max_dim, int exact_h1,
+              int exact_colex, int64_t max_col, int64_t max_pool, int threads,
+              fphc_result **out);
+/* Rows of dimension dim (-1 for a bad argument); copy them as (K, 2) FP32. */
+int64_t fphc_rows(const fphc_result *res, int dim);
+int fphc_copy(const fphc_result *res, int dim, float *dst);
+/* FPHC_NSTATS counters (edges, triangles, tetrahedra, column additions, ...). */
+int fphc_stats(const fphc_result *res, int64_t *dst);
+void fphc_free(fphc_result *res);
+
+/*
+ * The filtration alone (rips_filtration on a CPU tensor, rips.py:98-233):
+ * E edges in filtration order and the per-vertex CSR (neighbours
+ * ascending).  fphc_filtration_copy fills any non-NULL array: edge_i,
+ * edge_j, edge_dsq (E each), adj_ptr (n + 1), adj_idx, adj_rank, adj_dsq
+ * (2E each).
+ */
+typedef struct fphc_filtration_t fphc_filtration_t;
+int fphc_filtration(const float *pts, int64_t n, int d, float thr_sq, int exact_colex,
+                    int threads, fphc_filtration_t **out);
+int64_t fphc_filtration_edges(const fphc_filtration_t *h);
+int fphc_filtration_copy(const fphc_filtration_t *h, int32_t *edge_i, int32_t *edge_j,
+                         float *edge_dsq, int32_t *adj_ptr, int32_t *adj_idx, int32_t *adj_rank,
+                         float *adj_dsq);
+void fphc_filtration_free(fphc_filtration_t *h);
+
+/*
+ * k-th smallest FMA-chain dsq from every point to the others (k clamped to
+ * [1, n - 1]) -- the per-row top-k of threshold.py:133-176 in the
+ * filtration's arithmetic; and the largest dsq from every point
+ * (threshold.py:179-200).  n >= 2.
+ */
+int fphc_knn_kth(const float *pts, int64_t n, int d, int k, float *kth_dsq, int threads);
+int fphc_far_dsq(const float *pts, int64_t n, int d, float *far_dsq, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLASHPH_CPU_H */

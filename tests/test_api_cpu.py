@@ -85,9 +85,29 @@ def test_numpy_single_point():
 
 
 @pytest.mark.skipif(torch.cuda.is_available(), reason="checks the no-GPU error")
-def test_no_gpu_fails_loudly():
+def test_gpu_library_path_fails_loudly_without_a_gpu():
+    """CUDA work needs the GPU: require_gpu raises where none is visible
+    (there is no CPU stand-in for the HIP library)."""
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
     with pytest.raises(RuntimeError, match="ROCm GPU"):
-        rips_persistence(torch.randn(10, 3), 1.0)
+        _lib.require_gpu()
+    from flash_ph import rips_persistence_batch
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        rips_persistence_batch([torch.randn(10, 3)], 1.0)
+
+
+def test_cpu_tensors_run_on_the_host_pipeline():
+    """CPU tensors are computed on the CPU (as the reference's CPU route),
+    by libflashph_cpu: the oracle's diagrams bit for bit, output on the
+    CPU, no GPU needed."""
+    import oracle
+    pts = np.random.default_rng(0).standard_normal((80, 3)).astype(np.float32)
+    dg = rips_persistence(torch.from_numpy(pts), 1.0, max_dim=2)
+    ref = oracle.run(pts, 1.0, 2)["diagrams"]
+    for a, b in zip(dg, ref):
+        assert a.device.type == "cpu"
+        assert np.array_equal(a.numpy().view(np.uint32), b.view(np.uint32))
 
 
 def test_complex_errors():
@@ -109,18 +129,19 @@ def test_complex_single_point():
     assert rc.persistence() == [(0, (0.0, float("inf")))]
 
 
-def test_thresholds_need_gpu_except_trivial():
-    """n <= 1 returns 0.0 on any device (threshold.py:59-61, 106-108); larger
-    inputs run on the GPU and raise without one (no CPU fallback)."""
-    pts = torch.from_numpy(np.random.default_rng(0).standard_normal((30, 3)).astype(np.float32))
+def test_thresholds_on_cpu_tensors():
+    """n <= 1 returns 0.0 (threshold.py:59-61, 106-108); CPU tensors run on
+    the host pipeline with the filtration's FP32 arithmetic and torch's FP32
+    quantile: the oracle's values bit for bit, k beyond 128 included."""
+    import oracle
+    pts = torch.from_numpy(np.random.default_rng(0).standard_normal((300, 3)).astype(np.float32))
     assert auto_threshold(pts[:1]) == 0.0
     assert auto_threshold(pts[:0]) == 0.0
     assert enclosing_radius(pts[:1]) == 0.0
-    if not torch.cuda.is_available():
-        with pytest.raises((RuntimeError, ImportError)):
-            auto_threshold(pts)
-        with pytest.raises((RuntimeError, ImportError)):
-            enclosing_radius(pts)
+    x = pts.numpy()
+    for k, p in [(1, 50.0), (20, 95.0), (150, 90.0), (299, 100.0), (5000, 0.0)]:
+        assert np.float32(auto_threshold(pts, k, p)) == np.float32(oracle.auto_threshold(x, k, p))
+    assert np.float32(enclosing_radius(pts)) == np.float32(oracle.enclosing_radius(x))
 
 
 def test_graph_argument_errors_without_gpu():

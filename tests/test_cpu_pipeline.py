@@ -1,0 +1,73 @@
+"""The host (CPU) pipeline, libflashph_cpu, against the oracle on the CPU:
+diagrams bit for bit on the parity families (both reduction semantics,
+max_dim 0..2), the reference's lowered limits, the filtration and its CSR,
+and thread-count independence."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import cases
+import oracle
+from flash_ph import _cpu
+
+
+def _eq(a, b):
+    return a.shape == b.shape and np.array_equal(np.ascontiguousarray(a).view(np.uint32),
+                                                 np.ascontiguousarray(b).view(np.uint32))
+
+
+@pytest.mark.parametrize("name,make,t,md", cases.SMALL + cases.MEDIUM[:4],
+                         ids=[c[0] for c in cases.SMALL + cases.MEDIUM[:4]])
+@pytest.mark.parametrize("exact", [False, True])
+def test_cpu_pipeline_matches_oracle(name, make, t, md, exact):
+    pts = make()
+    thr = oracle.threshold_sq_f32(t)
+    for max_dim in sorted({0, 1, md}):
+        got = _cpu.rips(pts, thr, max_dim, exact_h1=exact)
+        ref = oracle.run(pts, t, max_dim, exact=exact)["diagrams"]
+        for k in range(max_dim + 1):
+            assert _eq(got[k], ref[k]), (name, max_dim, k)
+
+
+@pytest.mark.parametrize("max_col,max_pool", [(4, 0), (1 << 20, 60), (12, 2000)])
+def test_cpu_reference_limits(max_col, max_pool):
+    pts = cases.randn(300, 4, 2)
+    got = _cpu.rips(pts, oracle.threshold_sq_f32(1.2), 1, max_col=max_col, max_pool=max_pool)
+    ref = oracle.run(pts, 1.2, 1, max_col=max_col, max_pool=max_pool)
+    assert _eq(got[1], ref["h1"])
+
+
+def test_cpu_filtration_and_api():
+    from flash_ph.rips import rips_filtration
+    pts = cases.randn(400, 3, 8)
+    f = rips_filtration(torch.from_numpy(pts), 0.8)
+    ref = oracle.run(pts, 0.8, 0)
+    for k, r in [("edge_i", "edge_i"), ("edge_j", "edge_j"), ("edge_dist_sq", "edge_dsq"),
+                 ("vert_adj_ptr", "adj_ptr"), ("vert_adj_idx", "adj_idx"),
+                 ("vert_adj_rank", "adj_rank"), ("vert_adj_dist_sq", "adj_dsq")]:
+        assert _eq(getattr(f, k).numpy(), ref[r]), k
+    assert f.edge_i.device.type == "cpu"
+
+
+def test_cpu_threads_do_not_change_results():
+    pts = cases.randn(2000, 10, 1)
+    thr = oracle.threshold_sq_f32(2.2)
+    a = _cpu.rips(pts, thr, 2, threads=1)
+    b = _cpu.rips(pts, thr, 2, threads=8)
+    for x, y in zip(a, b):
+        assert _eq(x, y)
+
+
+def test_cpu_wrapped_key_order():
+    """n > 92,682: the emission-order tie break of the wrapped key, and the
+    exact colex order, on the 47^3 grid."""
+    pts = cases.grid_points(47)
+    i, j, dsq = cases.grid_edges(pts, 1.5)
+    f = _cpu.filtration(pts, oracle.threshold_sq_f32(1.5))
+    order = cases.reference_edge_order(i, j, dsq)
+    assert np.array_equal(f["edge_i"], i[order]) and np.array_equal(f["edge_j"], j[order])
+    g = _cpu.filtration(pts, oracle.threshold_sq_f32(1.5), exact_colex=True)
+    o2 = np.lexsort((i, j, dsq.view(np.uint32)))
+    assert np.array_equal(g["edge_i"], i[o2]) and np.array_equal(g["edge_j"], j[o2])
