@@ -67,6 +67,100 @@ def cpu_baseline(pts, budget_s: float = 12.0):
     }
 
 
+def host_info() -> dict:
+    """Cores visible to this process and the CPU model (nproc / lscpu)."""
+    info = {"nproc": len(os.sched_getaffinity(0)), "omp_num_threads": os.environ.get(
+        "OMP_NUM_THREADS")}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket",
+                             "Thread(s) per core", "CPU(s)"):
+                info[k.strip()] = v.strip()
+    except Exception:  # lscpu absent: the core count above stands
+        pass
+    return info
+
+
+def _median_ms(fn, reps: int = 5, warm: int = 2) -> float:
+    """The reference's protocol (bench/bench_persistence.py:25-26): median of
+    `reps` timed runs after `warm` untimed ones."""
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return round(1000.0 * statistics.median(ts), 3)
+
+
+def bench_cpu_pipeline(pts_np) -> dict:
+    """The host pipeline (libflashph_cpu, C++/OpenMP; bit-identical diagrams)
+    on one core and on every core of this process, on the n=10K d=10 H0-H2
+    workload and on the n=1000 d=10 auto-threshold H0+H1 workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cases
+    from flash_ph import _cpu, _pipeline
+    cores = len(os.sched_getaffinity(0))
+    thr2 = _pipeline.threshold_sq_f32(T_EDGE)
+    p1 = cases.randn(1000, 10, 0)
+    t1 = float(_cpu_auto_threshold(p1))
+    thr1 = _pipeline.threshold_sq_f32(t1)
+    out = {"host": host_info(), "protocol": "median of 5 after 2 warm-ups"}
+    for name, x, thr, md in (("n10000_d10_t1.6_h0h2", pts_np, thr2, 2),
+                             ("n1000_d10_auto_h0h1", p1, thr1, 1)):
+        out[name] = {"ms_1core": _median_ms(lambda: _cpu.rips(x, thr, md, exact_h1=md == 2,
+                                                              threads=1)),
+                     "ms_all_cores": _median_ms(lambda: _cpu.rips(x, thr, md,
+                                                                  exact_h1=md == 2,
+                                                                  threads=cores)),
+                     "cores": cores}
+    out["n1000_d10_auto_h0h1"]["threshold"] = t1
+    return out
+
+
+def _cpu_auto_threshold(x):
+    import torch
+    from flash_ph import auto_threshold
+    return auto_threshold(torch.from_numpy(x), k=20, percentile=95.0)
+
+
+def bench_dense_n1000() -> dict:
+    """n=1,000, d=10, N(0,1) default_rng(0), t = auto_threshold(k=20, p=95)
+    (~3.37; E ~ 79K, T ~ 2.76M), H0+H1 in the reference's semantics: the
+    GPU call (threshold included) and its parity with the host pipeline."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cases
+    from flash_ph import _cpu, _pipeline, auto_threshold, rips_persistence
+    x_np = cases.randn(1000, 10, 0)
+    x = torch.from_numpy(x_np).cuda()
+
+    def call():
+        t = auto_threshold(x, k=20, percentile=95.0)
+        return t, rips_persistence(x, t, max_dim=1)
+
+    t, dg = call()
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(5):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        call()
+        torch.cuda.synchronize()
+        ms.append(1000.0 * (time.perf_counter() - t0))
+    host = _cpu.rips(x_np, _pipeline.threshold_sq_f32(t), 1)
+    same = all(np.array_equal(a.cpu().numpy().view(np.uint32), b.view(np.uint32))
+               for a, b in zip(dg, host))
+    return {"workload": "n=1000, d=10, N(0,1) seed 0, auto_threshold(k=20, p=95), H0+H1",
+            "threshold": t, "ms": round(statistics.median(ms), 3),
+            "rows": [int(v.shape[0]) for v in dg], "bit_exact_vs_host_pipeline": bool(same)}
+
+
 def bench_batch(steps: int) -> dict:
     import torch
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -154,6 +248,7 @@ def bench_precision_n200k() -> dict:
     n, d, t = 200_000, 50, 6.0
     x = torch.from_numpy(np.random.default_rng(0).standard_normal((n, d)).astype(np.float32)).cuda()
     out = {"workload": f"n={n}, d={d}, N(0,1) float32 seed 0, t={t}, H0+H1"}
+    kept = {}
     for mode in ("mfma", "direct"):
         _pipeline.SCREEN = mode
         _pipeline.build_edges(x, t)                           # warm-up
@@ -161,6 +256,7 @@ def bench_precision_n200k() -> dict:
         reps = 3 if mode == "mfma" else 1
         for _ in range(reps):
             cx = _pipeline.build_edges(x, t)
+        kept[mode] = cx
         ms = _pipeline.TIMER.summary()["edge_screen"][1]
         _pipeline.TIMER.enable(False)
         out[f"screen_ms_{mode}"] = round(ms, 3)
@@ -182,6 +278,16 @@ def bench_precision_n200k() -> dict:
             ulp = (torch.nextafter(d32, torch.full_like(d32, float("inf"))) - d32).double()
             out["fp64_err_max_ulp"] = round(((d32.double() - d64).abs() / ulp).max().item(), 3)
             del i, j, x64, d64
+    a, b = kept["mfma"], kept["direct"]
+    out["edges_identical_mfma_vs_direct"] = bool(
+        a.E == b.E and all(torch.equal(getattr(a, k)[:a.E].view(torch.int32),
+                                       getattr(b, k)[:b.E].view(torch.int32))
+                           for k in ("edge_i", "edge_j", "edge_dsq")))
+    del kept, a, b, cx
+    _pipeline.SCREEN = "mfma"
+    out["fp64_validation"] = _pipeline.fp64_screen_report(x, t)
+    _pipeline.SCREEN = "direct"
+    dg_direct = rips_persistence(x, t, max_dim=1)
     _pipeline.SCREEN = "auto"
     rips_persistence(x, t, max_dim=1)
     torch.cuda.synchronize()
@@ -190,6 +296,8 @@ def bench_precision_n200k() -> dict:
     torch.cuda.synchronize()
     out["h0h1_ms"] = round(1000.0 * (time.perf_counter() - t0), 2)
     out["rows"] = [int(v.shape[0]) for v in dg]
+    out["diagrams_identical_auto_vs_direct_screen"] = bool(all(
+        torch.equal(p.view(torch.int32), q.view(torch.int32)) for p, q in zip(dg, dg_direct)))
     return out
 
 
@@ -412,7 +520,7 @@ def main():
             "entries_merged_per_s": round(merged / (reduce_ms * 1e-3), 1),
         },
         "roofline_screen": {
-            "bound": "mfma",
+            "bound": "fp32_valu",
             "kernel": "k_edge_screen (FP32 vector FMA chain)",
             "achieved": round(achieved, 3),
             "peak": PEAK_FP32_TFLOPS,
@@ -427,11 +535,13 @@ def main():
     if not args.no_batch:
         out["batch"] = bench_batch(args.batch_steps)
     if not args.no_extra:
+        out["dense_n1000_d10_auto"] = bench_dense_n1000()
         out["precision_n200k_d50"] = bench_precision_n200k()
         out["autograd_clifford_64x2000"] = bench_autograd_clifford()
         out["graph_h0"] = bench_graph_h0(pts)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pts_np)
+        out["cpu_pipeline"] = bench_cpu_pipeline(pts_np)
     print(json.dumps(out), flush=True)
 
 
