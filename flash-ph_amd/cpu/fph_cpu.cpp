@@ -643,9 +643,132 @@ int run(const float* pts, int64_t n, int d, float thr, int max_dim, int exact_h1
     return FPHC_OK;
 }
 
+// ---------------------------------------------------------------- apparent-pair scans
+//
+// The reference's CPU paths of its unwired scans (Numba,
+// kernels/apparent_pair_kernel.py:36-96, apparent_pair_h2_kernel.py:61-153):
+// complete merges, no step bound.
+
+int64_t colex3(int64_t a, int64_t b, int64_t c) {  // a < b < c
+    return c * (c - 1) * (c - 2) / 6 + b * (b - 1) / 2 + a;
+}
+
+int64_t colex4(int64_t a, int64_t b, int64_t c, int64_t d) {  // a < b < c < d
+    return d * (d - 1) / 2 * ((d - 2) * (d - 3) / 2) / 6 + colex3(a, b, c);
+}
+
+// position of w in the sorted list [s, e), or -1
+int64_t find_in(const int32_t* idx, int64_t s, int64_t e, int32_t w) {
+    const int32_t* p = std::lower_bound(idx + s, idx + e, w);
+    return p != idx + e && *p == w ? p - idx : -1;
+}
+
+// Triangle (a, b, c), a < b < c, largest edge rank rt: the smallest common
+// neighbour w whose edges to a, b, c all rank below rt gives the youngest
+// tetrahedron cofacet when one ranks rt at all (colex4 grows with w); the
+// triangle is apparent when w also lies below the vertex off its largest
+// edge.  Returns the tetrahedron's colex index, or -1.
+int64_t scan_h2_one(int32_t a, int32_t b, int32_t c, int32_t rt, const int32_t* ptr,
+                    const int32_t* idx, const int32_t* rank) {
+    const int32_t v[3] = {a, b, c};
+    int q0 = 0;
+    for (int q = 1; q < 3; q++)
+        if (ptr[v[q] + 1] - ptr[v[q]] < ptr[v[q0] + 1] - ptr[v[q0]]) q0 = q;
+    const int q1 = q0 == 0 ? 1 : 0, q2 = q0 == 2 ? 1 : 2;
+    int32_t w = -1;
+    for (int64_t h = ptr[v[q0]]; h < ptr[v[q0] + 1]; h++) {
+        if (rank[h] >= rt) continue;
+        const int32_t x = idx[h];
+        const int64_t p1 = find_in(idx, ptr[v[q1]], ptr[v[q1] + 1], x);
+        if (p1 < 0 || rank[p1] >= rt) continue;
+        const int64_t p2 = find_in(idx, ptr[v[q2]], ptr[v[q2] + 1], x);
+        if (p2 < 0 || rank[p2] >= rt) continue;
+        w = x;
+        break;
+    }
+    if (w < 0 || w > c) return -1;
+    int32_t s = c;  // the vertex off the largest edge
+    if (w > a) {
+        const int64_t pbc = find_in(idx, ptr[b], ptr[b + 1], c);
+        const int64_t pac = find_in(idx, ptr[a], ptr[a + 1], c);
+        if (pbc >= 0 && rank[pbc] == rt) s = a;
+        else if (pac >= 0 && rank[pac] == rt) s = b;
+    }
+    if (w >= s) return -1;
+    int64_t t[4] = {a, b, c, w};
+    for (int i = 3; i > 0 && t[i] < t[i - 1]; i--) std::swap(t[i], t[i - 1]);
+    return colex4(t[0], t[1], t[2], t[3]);
+}
+
 }  // namespace
 
 extern "C" {
+
+int fphc_apparent_scan(const int32_t* edge_i, const int32_t* edge_j, const int32_t* ranks,
+                       int64_t K, const int32_t* adj_ptr, const int32_t* adj_idx,
+                       const int32_t* adj_rank, int32_t* is_apparent, int64_t* pivot_colex,
+                       int threads) {
+    if (K < 0 || (K > 0 && (!edge_i || !edge_j || !ranks || !adj_ptr || !adj_idx || !adj_rank ||
+                            !is_apparent || !pivot_colex)))
+        return FPHC_ERR_ARG;
+    const int prev = omp_get_max_threads();
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t k = 0; k < K; k++) {
+        const int32_t re = ranks[k];
+        int32_t u = edge_i[re], v = edge_j[re];
+        if (u > v) std::swap(u, v);
+        int64_t iu = adj_ptr[u], iv = adj_ptr[v];
+        const int64_t ue = adj_ptr[u + 1], ve = adj_ptr[v + 1];
+        int32_t w = -1;
+        while (iu < ue && iv < ve) {
+            const int32_t nu = adj_idx[iu], nv = adj_idx[iv];
+            if (nu == nv) {
+                if (adj_rank[iu] < re && adj_rank[iv] < re) {
+                    w = nu;
+                    break;
+                }
+                iu++;
+                iv++;
+            } else if (nu < nv) {
+                iu++;
+            } else {
+                iv++;
+            }
+        }
+        is_apparent[k] = w >= 0;
+        if (w < 0) {
+            pivot_colex[k] = 0;
+        } else {
+            int64_t t[3] = {u, v, w};
+            for (int i = 2; i > 0 && t[i] < t[i - 1]; i--) std::swap(t[i], t[i - 1]);
+            pivot_colex[k] = colex3(t[0], t[1], t[2]);
+        }
+    }
+    if (threads > 0) omp_set_num_threads(prev);
+    return FPHC_OK;
+}
+
+int fphc_apparent_scan_h2(const int32_t* tri_v0, const int32_t* tri_v1, const int32_t* tri_v2,
+                          const int32_t* tri_max_rank, int64_t T, const int32_t* adj_ptr,
+                          const int32_t* adj_idx, const int32_t* adj_rank, int32_t* is_apparent,
+                          int64_t* pivot_colex, int threads) {
+    if (T < 0 || (T > 0 && (!tri_v0 || !tri_v1 || !tri_v2 || !tri_max_rank || !adj_ptr ||
+                            !adj_idx || !adj_rank || !is_apparent || !pivot_colex)))
+        return FPHC_ERR_ARG;
+    const int prev = omp_get_max_threads();
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t t = 0; t < T; t++) {
+        const int64_t cx = scan_h2_one(tri_v0[t], tri_v1[t], tri_v2[t], tri_max_rank[t], adj_ptr,
+                                       adj_idx, adj_rank);
+        is_apparent[t] = cx >= 0;
+        pivot_colex[t] = cx >= 0 ? cx : 0;
+    }
+    if (threads > 0) omp_set_num_threads(prev);
+    return FPHC_OK;
+}
+
 
 const char* fphc_version(void) { return "flash_ph cpu 0.1.0 (OpenMP)"; }
 

@@ -59,6 +59,10 @@ def load():
     lib.fphc_knn_kth.argtypes = [P, I64, I32, I32, P, I32]
     lib.fphc_far_dsq.restype = I32
     lib.fphc_far_dsq.argtypes = [P, I64, I32, P, I32]
+    lib.fphc_apparent_scan.restype = I32
+    lib.fphc_apparent_scan.argtypes = [P, P, P, I64, P, P, P, P, P, I32]
+    lib.fphc_apparent_scan_h2.restype = I32
+    lib.fphc_apparent_scan_h2.argtypes = [P, P, P, P, I64, P, P, P, P, P, I32]
     _lib = lib
     return lib
 
@@ -144,3 +148,37 @@ def filtration(points: np.ndarray, thr_sq: float, exact_colex: bool = False,
         return out
     finally:
         lib.fphc_filtration_free(h)
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def apparent_scan(edge_i, edge_j, nonmst_ranks, adj_ptr, adj_idx, adj_rank,
+                  threads: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    """H1 apparent-pair scan, complete (fphc_apparent_scan)."""
+    lib = load()
+    a = [_i32(x) for x in (edge_i, edge_j, nonmst_ranks, adj_ptr, adj_idx, adj_rank)]
+    K = a[2].shape[0]
+    flag = np.zeros(K, dtype=np.int32)
+    colex = np.zeros(K, dtype=np.int64)
+    _check(lib.fphc_apparent_scan(a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data, K,
+                                  a[3].ctypes.data, a[4].ctypes.data, a[5].ctypes.data,
+                                  flag.ctypes.data, colex.ctypes.data, int(threads)),
+           "fphc_apparent_scan")
+    return flag, colex
+
+
+def apparent_scan_h2(tri_v0, tri_v1, tri_v2, tri_max_rank, adj_ptr, adj_idx, adj_rank,
+                     threads: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    """H2 apparent-pair scan, complete (fphc_apparent_scan_h2)."""
+    lib = load()
+    a = [_i32(x) for x in (tri_v0, tri_v1, tri_v2, tri_max_rank, adj_ptr, adj_idx, adj_rank)]
+    T = a[0].shape[0]
+    flag = np.zeros(T, dtype=np.int32)
+    colex = np.zeros(T, dtype=np.int64)
+    _check(lib.fphc_apparent_scan_h2(a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data,
+                                     a[3].ctypes.data, T, a[4].ctypes.data, a[5].ctypes.data,
+                                     a[6].ctypes.data, flag.ctypes.data, colex.ctypes.data,
+                                     int(threads)), "fphc_apparent_scan_h2")
+    return flag, colex

@@ -53,6 +53,7 @@ SIGNATURES = {
                                _PSZ, _P]),
     "fph_apparent": (_I32, [_I64, _I64, _P, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P]),
     "fph_apparent_scan": (_I32, [_P, _P, _P, _I64, _P, _P, _P, _I64, _P, _P, _P]),
+    "fph_apparent_scan_h2": (_I32, [_P, _P, _P, _P, _I64, _P, _P, _P, _I64, _P, _P, _P]),
     "fph_reduce_scratch_bytes": (_SZ, [_I64]),
     "fph_reduce": (_I32, [_I64, _P, _P, _I64, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _P,
                           _SZ, _P, _I32, _P]),

@@ -23,36 +23,48 @@ def apparent_pair_scan(
     adj_idx: Tensor,
     adj_rank: Tensor,
     max_degree: int,
+    *,
+    complete: bool = False,
 ) -> tuple[Tensor, Tensor]:
     """Classify non-MST edges as apparent (reference
     kernels/apparent_pair_kernel.py:200-236).
 
     Returns (is_apparent (K,) int32, pivot_colex (K,) int64) on the inputs'
-    device.  ``max_degree`` bounds the merge walk as the reference's GPU
-    loop does (md = max_degree rounded up to a power of two in [64, 2048],
-    2 md steps); pass 0 for a complete scan (the reference's CPU path).
-    CPU tensors are computed on the GPU and returned on the CPU.
+    device.  CUDA tensors: ``max_degree`` bounds the merge walk as the
+    reference's GPU loop does (md = max_degree rounded up to a power of two
+    in [64, 2048], 2 md steps); ``complete=True`` lifts the bound.  CPU
+    tensors: the complete scan on the host (the reference's Numba path
+    ignores max_degree too).
     """
     K = nonmst_ranks.shape[0]
     device = edge_i.device
     if K == 0:
         return (torch.zeros(0, dtype=torch.int32, device=device),
                 torch.zeros(0, dtype=torch.int64, device=device))
+    if device.type != "cuda":
+        from flash_ph import _cpu
+
+        def host(t):
+            return t.detach().to(torch.int32).contiguous().numpy()
+
+        flag, colex = _cpu.apparent_scan(host(edge_i), host(edge_j), host(nonmst_ranks),
+                                         host(adj_ptr), host(adj_idx), host(adj_rank))
+        return torch.from_numpy(flag), torch.from_numpy(colex)
     _lib.require_gpu()
-    dev = device if device.type == "cuda" else torch.device("cuda")
 
     def prep(t):
-        return t.to(device=dev, dtype=torch.int32).contiguous()
+        return t.to(device=device, dtype=torch.int32).contiguous()
 
     ei, ej, nm = prep(edge_i), prep(edge_j), prep(nonmst_ranks)
     ap, ai, ar = prep(adj_ptr), prep(adj_idx), prep(adj_rank)
-    flag = torch.empty(K, dtype=torch.int32, device=dev)
-    colex = torch.empty(K, dtype=torch.int64, device=dev)
+    flag = torch.empty(K, dtype=torch.int32, device=device)
+    colex = torch.empty(K, dtype=torch.int64, device=device)
     lib = _lib.load()
+    # the C entry point takes max_degree <= 0 as "no bound"; the reference's
+    # loop treats every max_degree <= 64 as md = 64
+    md = 0 if complete else max(int(max_degree), 1)
     _lib.check(lib.fph_apparent_scan(ei.data_ptr(), ej.data_ptr(), nm.data_ptr(), K,
-                                     ap.data_ptr(), ai.data_ptr(), ar.data_ptr(),
-                                     int(max_degree), flag.data_ptr(), colex.data_ptr(),
-                                     _lib.stream_of(dev)), "fph_apparent_scan")
-    if dev != device:
-        return flag.to(device), colex.to(device)
+                                     ap.data_ptr(), ai.data_ptr(), ar.data_ptr(), md,
+                                     flag.data_ptr(), colex.data_ptr(), _lib.stream_of(device)),
+               "fph_apparent_scan")
     return flag, colex

@@ -282,6 +282,26 @@ int fph_apparent_scan(const int32_t *edge_i, const int32_t *edge_j,
                       const int32_t *adj_idx, const int32_t *adj_rank, int64_t max_degree,
                       int32_t *is_apparent, int64_t *pivot_colex, void *stream);
 
+/*
+ * Ripser-style apparent-pair scan for H2: for each of T triangles
+ * (tri_v0 < tri_v1 < tri_v2, largest edge rank tri_max_rank), the youngest
+ * tetrahedron cofacet by (largest edge rank, colex4) over the common
+ * neighbours of its three vertices; is_apparent = 1 when that rank equals
+ * the triangle's and the triangle is the tetrahedron's oldest face (the
+ * added vertex is below the vertex off the largest edge), with pivot_colex
+ * = the tetrahedron's colex index C(v3,4)+C(v2,3)+C(v1,2)+v0 (else 0).
+ * max_degree > 0 bounds the walk as the reference's GPU loop does (md =
+ * max_degree rounded up to a power of two in [64, 2048], 3 md merge steps);
+ * <= 0: no bound (the reference's Numba path).  FPH_ERR_LIMIT past 2^33
+ * triangles.
+ * Replaces h2_apparent_pair_scan, kernels/apparent_pair_h2_kernel.py:
+ * 61-153, 161-317, 328-441 (not wired into the reference's pipeline).
+ */
+int fph_apparent_scan_h2(const int32_t *tri_v0, const int32_t *tri_v1, const int32_t *tri_v2,
+                         const int32_t *tri_max_rank, int64_t T, const int32_t *adj_ptr,
+                         const int32_t *adj_idx, const int32_t *adj_rank, int64_t max_degree,
+                         int32_t *is_apparent, int64_t *pivot_colex, void *stream);
+
 /* ---- stage (e): residual reduction ------------------------------------ */
 
 /* Bytes of device scratch fph_reduce needs for K columns. */

@@ -85,6 +85,22 @@ void fphc_filtration_free(fphc_filtration_t *h);
 int fphc_knn_kth(const float *pts, int64_t n, int d, int k, float *kth_dsq, int threads);
 int fphc_far_dsq(const float *pts, int64_t n, int d, float *far_dsq, int threads);
 
+/*
+ * The apparent-pair scans on host arrays, complete (no step bound), as the
+ * reference's CPU paths compute them.  Same arguments and outputs as
+ * fph_apparent_scan / fph_apparent_scan_h2 in flashph.h.
+ * Replace _numba_apparent_scan (kernels/apparent_pair_kernel.py:36-96) and
+ * _numba_h2_apparent_scan (kernels/apparent_pair_h2_kernel.py:61-153).
+ */
+int fphc_apparent_scan(const int32_t *edge_i, const int32_t *edge_j, const int32_t *nonmst_ranks,
+                       int64_t K, const int32_t *adj_ptr, const int32_t *adj_idx,
+                       const int32_t *adj_rank, int32_t *is_apparent, int64_t *pivot_colex,
+                       int threads);
+int fphc_apparent_scan_h2(const int32_t *tri_v0, const int32_t *tri_v1, const int32_t *tri_v2,
+                          const int32_t *tri_max_rank, int64_t T, const int32_t *adj_ptr,
+                          const int32_t *adj_idx, const int32_t *adj_rank, int32_t *is_apparent,
+                          int64_t *pivot_colex, int threads);
+
 #ifdef __cplusplus
 }
 #endif

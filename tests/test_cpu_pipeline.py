@@ -71,3 +71,41 @@ def test_cpu_wrapped_key_order():
     g = _cpu.filtration(pts, oracle.threshold_sq_f32(1.5), exact_colex=True)
     o2 = np.lexsort((i, j, dsq.view(np.uint32)))
     assert np.array_equal(g["edge_i"], i[o2]) and np.array_equal(g["edge_j"], j[o2])
+
+
+SCAN_CASES = [("randn_d3_n300", cases.randn(300, 3, 1), 0.7),
+              ("clifford_300", cases.clifford(300, 2, 0.02), 0.6),
+              ("sphere_d4_n250", cases.sphere(250, 4, 3, 0.05), 0.7),
+              ("dup_points", np.repeat(cases.randn(60, 2, 3), 2, axis=0), 0.5)]
+
+
+@pytest.mark.parametrize("name,pts,t", SCAN_CASES, ids=[c[0] for c in SCAN_CASES])
+def test_cpu_apparent_scans_match_definitions(name, pts, t):
+    """Host scans (the reference's Numba paths) against the predicates
+    written from their definitions; for H2 also the reference's d < s case
+    rule, which must agree with "largest face of the youngest cofacet"."""
+    import scan_defs
+    from flash_ph.kernels.apparent_pair_h2_kernel import h2_apparent_pair_scan
+    from flash_ph.kernels.apparent_pair_kernel import apparent_pair_scan
+    f = _cpu.filtration(pts, oracle.threshold_sq_f32(t))
+    ei, ej = f["edge_i"], f["edge_j"]
+    ptr, idx, rank = f["adj_ptr"], f["adj_idx"], f["adj_rank"]
+    ranks = np.arange(ei.size, dtype=np.int32)
+    tt = [torch.from_numpy(np.ascontiguousarray(x)) for x in (ei, ej, ranks, ptr, idx, rank)]
+    flag, colex = apparent_pair_scan(*tt, 0)
+    wf, wc = scan_defs.h1_definition(ei, ej, ranks, ptr, idx, rank)
+    assert np.array_equal(flag.numpy(), wf) and np.array_equal(colex.numpy(), wc)
+    assert wf.sum() > 0
+    tri = scan_defs.triangles(ei, ej, ptr, idx, rank)
+    cols = [torch.from_numpy(np.ascontiguousarray(tri[:, q]).astype(np.int32)) for q in range(4)]
+    f2, c2 = h2_apparent_pair_scan(*cols, *tt[3:], 0)
+    wf2, wc2 = scan_defs.h2_definition(tri, ptr, idx, rank)
+    assert np.array_equal(f2.numpy(), wf2) and np.array_equal(c2.numpy(), wc2)
+    assert 0 < wf2.sum() < len(tri)
+    # the reference's case rule on the unbounded walk gives the same set
+    big = 1 << 40
+    for k in range(len(tri)):
+        a, b, c, rt = (int(x) for x in tri[k])
+        d = scan_defs.h2_reference_walk(a, b, c, rt, ptr, idx, rank, big)
+        app = d >= 0 and scan_defs.h2_oldest_face(a, b, c, rt, d, ptr, idx, rank)
+        assert app == bool(wf2[k]), (name, k)

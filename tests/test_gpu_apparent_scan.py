@@ -16,30 +16,13 @@ import pytest
 import torch
 
 import cases
+import scan_defs
 
 pytestmark = pytest.mark.gpu
 
 
-def _colex3(a, b, c):
-    a, b, c = sorted((int(a), int(b), int(c)))
-    return c * (c - 1) * (c - 2) // 6 + b * (b - 1) // 2 + a
-
-
-def _numpy_scan(ei, ej, ranks, ptr, idx, rank):
-    flag = np.zeros(len(ranks), dtype=np.int32)
-    colex = np.zeros(len(ranks), dtype=np.int64)
-    for k, re in enumerate(ranks):
-        u, v = sorted((int(ei[re]), int(ej[re])))
-        nu = dict(zip(idx[ptr[u]:ptr[u + 1]].tolist(), rank[ptr[u]:ptr[u + 1]].tolist()))
-        best = None
-        for w, rv in zip(idx[ptr[v]:ptr[v + 1]].tolist(), rank[ptr[v]:ptr[v + 1]].tolist()):
-            ru = nu.get(w)
-            if ru is not None and ru < re and rv < re and (best is None or w < best):
-                best = w
-        if best is not None:
-            flag[k] = 1
-            colex[k] = _colex3(u, v, best)
-    return flag, colex
+_colex3 = scan_defs.colex3
+_numpy_scan = scan_defs.h1_definition
 
 
 CASES = [("randn_d3_n300", cases.randn(300, 3, 1), 0.7),
@@ -61,7 +44,7 @@ def test_scan_predicate_subset_and_pivots(name, pts, t):
     flag, colex = apparent_pair_scan(cx.edge_i, cx.edge_j, nonmst, cx.adj_ptr, cx.adj_idx,
                                      cx.adj_rank, deg)
     flag0, colex0 = apparent_pair_scan(cx.edge_i, cx.edge_j, nonmst, cx.adj_ptr, cx.adj_idx,
-                                       cx.adj_rank, 0)
+                                       cx.adj_rank, 0, complete=True)
     torch.cuda.synchronize()
     f, c = flag.cpu().numpy(), colex.cpu().numpy()
     assert deg <= 2048
@@ -86,25 +69,7 @@ def test_scan_predicate_subset_and_pivots(name, pts, t):
     print(f"[gpu] {name}: scan {int(f.sum())} of {int((kind == 2).sum())} apparent", flush=True)
 
 
-def _triton_walk(u, v, re, ptr, idx, rank, md):
-    """The reference's per-edge merge loop (kernels/apparent_pair_kernel.py:
-    133-156) with its 2 md step bound, one edge at a time."""
-    iu, iv = int(ptr[u]), int(ptr[v])
-    ue, ve = int(ptr[u + 1]), int(ptr[v + 1])
-    for _ in range(2 * md):
-        if iu >= ue or iv >= ve:
-            break
-        nu, nv = idx[iu], idx[iv]
-        if nu == nv:
-            if rank[iu] < re and rank[iv] < re:
-                return int(nu)
-            iu += 1
-            iv += 1
-        elif nu < nv:
-            iu += 1
-        else:
-            iv += 1
-    return -1
+_triton_walk = scan_defs.h1_reference_walk
 
 
 def test_scan_step_cap_on_a_hub():
@@ -131,7 +96,7 @@ def test_scan_step_cap_on_a_hub():
     capped = apparent_pair_scan(f.edge_i, f.edge_j, ranks, f.vert_adj_ptr, f.vert_adj_idx,
                                 f.vert_adj_rank, deg)
     full = apparent_pair_scan(f.edge_i, f.edge_j, ranks, f.vert_adj_ptr, f.vert_adj_idx,
-                              f.vert_adj_rank, 0)
+                              f.vert_adj_rank, 0, complete=True)
     fc, cc = (x.cpu().numpy() for x in capped)
     ff, cf = (x.cpu().numpy() for x in full)
     missed = np.nonzero((ff == 1) & (fc == 0))[0]
@@ -147,3 +112,126 @@ def test_scan_step_cap_on_a_hub():
         assert (w >= 0) == bool(fc[k]), k
         if w >= 0:
             assert cc[k] == _colex3(u, v, w)
+
+
+def _pipeline_triangles(cx):
+    """(T, 4) rows (a, b, c, largest edge rank) of the pipeline's triangles,
+    in its triangle order."""
+    T = cx.T
+    te = cx.tri_e[:T].cpu().numpy().astype(np.int64)
+    tw = cx.tri_w[:T].cpu().numpy().astype(np.int64)
+    ei, ej = cx.edge_i.cpu().numpy().astype(np.int64), cx.edge_j.cpu().numpy().astype(np.int64)
+    v = np.sort(np.stack([ei[te], ej[te], tw], 1), axis=1)
+    return np.concatenate([v, te[:, None]], 1)
+
+
+@pytest.mark.parametrize("name,pts,t", CASES, ids=[c[0] for c in CASES])
+def test_scan_h2_definition_and_pipeline(name, pts, t):
+    """H2 scan (reference kernels/apparent_pair_h2_kernel.py:161-317):
+    flags and tetrahedron colex equal the definition on sampled triangles,
+    the bounded and complete walks agree (degrees <= 2048), CPU tensors give
+    the same result on the host, and on the triangles the H1 pivots leave
+    the scan's set lies inside the pipeline's H2 apparent set with the same
+    pivot tetrahedra."""
+    from flash_ph import _pipeline
+    from flash_ph.kernels.apparent_pair_h2_kernel import h2_apparent_pair_scan
+    print(f"[gpu] h2 apparent scan {name}", flush=True)
+    dg, cx = _pipeline.compute(torch.from_numpy(pts).cuda(), t, 2, semantics="exact")
+    E, T, Q = cx.E, cx.T, cx.Q
+    tri = _pipeline_triangles(cx)
+    cols = [torch.from_numpy(np.ascontiguousarray(tri[:, q]).astype(np.int32)).cuda()
+            for q in range(4)]
+    deg = int((cx.adj_ptr[1:] - cx.adj_ptr[:-1]).max().item())
+    assert deg <= 2048
+    flag, colex = h2_apparent_pair_scan(*cols, cx.adj_ptr, cx.adj_idx, cx.adj_rank, deg)
+    flag0, colex0 = h2_apparent_pair_scan(*cols, cx.adj_ptr, cx.adj_idx, cx.adj_rank, 0,
+                                          complete=True)
+    f, c = flag.cpu().numpy(), colex.cpu().numpy()
+    assert np.array_equal(f, flag0.cpu().numpy()) and np.array_equal(c, colex0.cpu().numpy())
+    hf, hc = h2_apparent_pair_scan(*(x.cpu() for x in cols), cx.adj_ptr.cpu(),
+                                   cx.adj_idx.cpu(), cx.adj_rank.cpu(), deg)
+    assert np.array_equal(hf.numpy(), f) and np.array_equal(hc.numpy(), c)
+    ptr, idx, rank = cx.adj_ptr.cpu().numpy(), cx.adj_idx.cpu().numpy(), cx.adj_rank.cpu().numpy()
+    sel = np.arange(T) if T <= 2500 else np.random.default_rng(0).choice(T, 2500, replace=False)
+    wf, wc = scan_defs.h2_definition(tri[sel], ptr, idx, rank)
+    assert np.array_equal(f[sel], wf) and np.array_equal(c[sel], wc)
+    assert f.sum() > 0
+    # inside the pipeline's H2 apparent set, same pivot tetrahedron
+    cleared = cx.cleared[:T].cpu().numpy().astype(bool)
+    kind = cx.kind2[:T].cpu().numpy()
+    piv = cx.pivot2[:T].cpu().numpy().astype(np.int64)
+    qe = cx.tet_e[:Q].cpu().numpy()
+    key = cx.tet_key[:Q].cpu().numpy().view(np.uint64)
+    ei, ej = cx.edge_i.cpu().numpy(), cx.edge_j.cpu().numpy()
+    hits = np.nonzero((f == 1) & ~cleared)[0]
+    assert hits.size > 0
+    assert (kind[hits] == 2).all(), "scan pair not apparent in the pipeline"
+    for k in hits[:3000]:
+        q = int(piv[k]) - E - T
+        tv = (ei[qe[q]], ej[qe[q]], int(key[q] >> np.uint64(32)),
+              int(key[q] & np.uint64(0xFFFFFFFF)))
+        assert scan_defs.colex4(*tv) == c[k]
+    print(f"[gpu] {name}: h2 scan {int(f.sum())} of {T}; pipeline apparent "
+          f"{int((kind == 2).sum())} of {int((~cleared).sum())} uncleared", flush=True)
+
+
+def _hub_clusters():
+    """Vertex 0 at the origin; 8000 points in 160 tight clusters of 50
+    around random directions in 50 dimensions, radius 1 - 0.1 k / 8000 for
+    point k (so later points are nearer the hub), cluster labels shuffled.
+    t = 1.05: the hub meets every point (degree 8000 > 2048), clusters are
+    cliques, distinct clusters do not meet."""
+    rng = np.random.default_rng(7)
+    nc, m, d = 160, 50, 50
+    dirs = rng.standard_normal((nc, d))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    lab = rng.permutation(np.repeat(np.arange(nc), m))
+    g = dirs[lab] + 0.09 * rng.standard_normal((nc * m, d)) / np.sqrt(d)
+    g /= np.linalg.norm(g, axis=1, keepdims=True)
+    r = 1.0 - 0.1 * np.arange(1, nc * m + 1) / (nc * m)
+    return np.concatenate([np.zeros((1, d)), g * r[:, None]]).astype(np.float32)
+
+
+def test_scan_h2_step_cap_on_a_hub():
+    """Triangles (0, p, q) on the hub: the three-way walk passes the hub's
+    list one vertex per step, so a witness d beyond ~6,144 is past the
+    reference loop's 3 md bound (md = 2048) and the triangle is missed (its
+    false negatives).  The bounded scan must equal a literal walk of that
+    loop on sampled triangles, and only ever remove pairs."""
+    from flash_ph.kernels.apparent_pair_h2_kernel import h2_apparent_pair_scan
+    from flash_ph.rips import rips_filtration
+    pts = _hub_clusters()
+    print("[gpu] h2 apparent scan hub n=8001", flush=True)
+    f = rips_filtration(torch.from_numpy(pts).cuda(), 1.05)
+    ptr = f.vert_adj_ptr.cpu().numpy()
+    idx, rank = f.vert_adj_idx.cpu().numpy(), f.vert_adj_rank.cpu().numpy()
+    deg = int(np.diff(ptr).max())
+    assert deg == 8000 and np.diff(ptr)[0] == 8000
+    ei, ej = f.edge_i.cpu().numpy().astype(np.int64), f.edge_j.cpu().numpy().astype(np.int64)
+    hub_rank = np.zeros(pts.shape[0], dtype=np.int64)
+    hub_rank[idx[:ptr[1]]] = rank[:ptr[1]]
+    rim = np.nonzero((ei != 0) & (ej != 0))[0]
+    a = np.minimum(ei[rim], ej[rim])
+    b = np.maximum(ei[rim], ej[rim])
+    rt = np.maximum(rim, np.maximum(hub_rank[a], hub_rank[b]))
+    tri = np.stack([np.zeros_like(a), a, b, rt], 1)
+    cols = [torch.from_numpy(np.ascontiguousarray(tri[:, q]).astype(np.int32)).cuda()
+            for q in range(4)]
+    capped = h2_apparent_pair_scan(*cols, f.vert_adj_ptr, f.vert_adj_idx, f.vert_adj_rank, deg)
+    full = h2_apparent_pair_scan(*cols, f.vert_adj_ptr, f.vert_adj_idx, f.vert_adj_rank, 0,
+                                 complete=True)
+    fc, cc = (x.cpu().numpy() for x in capped)
+    ff, cf = (x.cpu().numpy() for x in full)
+    missed = np.nonzero((ff == 1) & (fc == 0))[0]
+    assert missed.size > 0                      # the bound matters here
+    assert not ((fc == 1) & (ff == 0)).any()     # it only removes
+    assert np.array_equal(cc[fc == 1], cf[fc == 1])
+    rng = np.random.default_rng(1)
+    sample = np.concatenate([missed[:60], rng.choice(len(tri), 140, replace=False)])
+    for k in sample:
+        a_, b_, c_, r_ = (int(x) for x in tri[k])
+        d = scan_defs.h2_reference_walk(a_, b_, c_, r_, ptr, idx, rank, 2048)
+        app = d >= 0 and scan_defs.h2_oldest_face(a_, b_, c_, r_, d, ptr, idx, rank)
+        assert app == bool(fc[k]), k
+        if app:
+            assert cc[k] == scan_defs.colex4(a_, b_, c_, d)
