@@ -426,6 +426,9 @@ void h2_columns(const Filtration& f, const Triangles& t, const Tetrahedra& q,
 // [r - row_base] = the largest non-skipped face of row r (-1: none); r2f over
 // all global ranks.  exact: the complete reduction (every reduced column
 // stored, no truncation); else the reference's store capacity and MAX_COL.
+// ref_cof: the cofacet entries the reference sizes its store from -- every
+// column's, skipped ones included (3 per triangle for H1,
+// reduce_h1_gpu.py:286-289, 523-534).
 struct Reduced {
     std::vector<int32_t> app_b, app_d;   // apparent pairs (persistence > 0)
     std::vector<int32_t> res_b, res_d;   // residual pairs, processing order
@@ -455,8 +458,8 @@ int64_t xor_merge(const int32_t* a, int64_t na, const int32_t* b, int64_t nb, in
 void reduce(int64_t K, int64_t base, const Columns& c, const std::vector<uint8_t>& skip,
             const std::vector<int32_t>& row_cmf, int64_t row_base, int64_t nrows,
             const std::vector<float>& r2f, bool exact, int64_t max_col, int64_t max_pool,
-            Reduced& out) {
-    const int64_t total_cof = c.off[K];
+            int64_t ref_cof, Reduced& out) {
+    const int64_t total_cof = ref_cof;
     int64_t cap = std::max<int64_t>(std::max<int64_t>(2 * total_cof, 16 * K), 100000);
     cap = std::min(cap, max_pool);
     const int64_t lim = exact ? std::numeric_limits<int64_t>::max() : max_col;
@@ -598,7 +601,7 @@ int run(const float* pts, int64_t n, int d, float thr, int max_dim, int exact_h1
     }
     Reduced r1;
     const bool exact1 = exact_h1 || max_dim >= 2;
-    reduce(E, 0, c1, mst, cmf1, E, T, r2f, exact1, max_col, max_pool, r1);
+    reduce(E, 0, c1, mst, cmf1, E, T, r2f, exact1, max_col, max_pool, 3 * T, r1);
     res.stats[FPHC_STAT_XOR1] = r1.xors;
     res.stats[FPHC_STAT_DROPS] = r1.drops;
     res.stats[FPHC_STAT_TRUNC] = r1.truncations;
@@ -637,7 +640,7 @@ int run(const float* pts, int64_t n, int d, float thr, int max_dim, int exact_h1
         }
     }
     Reduced r2;
-    reduce(T, E, c2, cleared, cmf2, E + T, Q, r2f2, true, max_col, max_pool, r2);
+    reduce(T, E, c2, cleared, cmf2, E + T, Q, r2f2, true, max_col, max_pool, c2.off[T], r2);
     res.stats[FPHC_STAT_XOR2] = r2.xors;
     assemble(r2, r2f2, res.dg[2]);
     return FPHC_OK;

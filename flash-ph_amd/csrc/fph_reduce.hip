@@ -1981,24 +1981,16 @@ __device__ __forceinline__ void serial_bitmap(RedArgs& a, SerShared& sh, const u
     }
 }
 
-__global__ __launch_bounds__(kSerThreads) void k_reduce_serial(
-    RedArgs a, const unsigned* __restrict__ order, const int* __restrict__ res_count,
-    const int* __restrict__ seg_cols, const long long* __restrict__ seg_pool, int variant,
-    int maxcol_ref) {
-    extern __shared__ unsigned smem[];
-    __shared__ SerShared sh;
-    unsigned* half0 = smem;
-    unsigned* stage = smem + 2 * kSerHalf;
+// The reference's pool prefill (_numba_utils.py:68-104, reduce_h1_gpu.py:
+// 282-289): apparent columns of [c0, c1) in ascending column order enter the
+// store while it has room (cap reference entries); a column that does not
+// fit is dropped and later ones may still fit.  Sets own[] for the stored
+// ones (their rows in place in the pool, written by k_publish_apparent) and
+// returns the entries used.  Whole workgroup of kSerThreads; barriers.
+__device__ long long serial_prefill(const RedArgs& a, SerShared& sh, int c0, int c1,
+                                    long long cap) {
     const int t = threadIdx.x;
-    const int seg = blockIdx.x;
-    const int c0 = seg_cols ? seg_cols[seg] : 0;
-    const int c1 = seg_cols ? seg_cols[seg + 1] : (int)a.K;
-    const long long* sp = seg_pool + kSegPoolWords * seg;
-    const long long base = sp[0];
-    const long long cap = sp[1];
-    long long used = 0;    // reference entries in the store (uniform)
-    long long stored = 0;  // words written at base (uniform)
-    // 1. apparent columns, ascending: store while pool_ptr + cn <= pool_cap
+    long long used = 0;
     for (int cb = c0; cb < c1; cb += kSerThreads) {
         const int c = cb + t;
         const bool app = c < c1 && a.kind[c] == 2;
@@ -2039,6 +2031,27 @@ __global__ __launch_bounds__(kSerThreads) void k_reduce_serial(
             p = f + 1;  // column f (if any) is dropped
         }
     }
+    return used;
+}
+
+__global__ __launch_bounds__(kSerThreads) void k_reduce_serial(
+    RedArgs a, const unsigned* __restrict__ order, const int* __restrict__ res_count,
+    const int* __restrict__ seg_cols, const long long* __restrict__ seg_pool, int variant,
+    int maxcol_ref) {
+    extern __shared__ unsigned smem[];
+    __shared__ SerShared sh;
+    unsigned* half0 = smem;
+    unsigned* stage = smem + 2 * kSerHalf;
+    const int t = threadIdx.x;
+    const int seg = blockIdx.x;
+    const int c0 = seg_cols ? seg_cols[seg] : 0;
+    const int c1 = seg_cols ? seg_cols[seg + 1] : (int)a.K;
+    const long long* sp = seg_pool + kSegPoolWords * seg;
+    const long long base = sp[0];
+    const long long cap = sp[1];
+    long long stored = 0;  // words written at base (uniform)
+    // 1. apparent columns, ascending: store while pool_ptr + cn <= pool_cap
+    long long used = serial_prefill(a, sh, c0, c1, cap);  // reference entries in the store
     __threadfence_block();
     __syncthreads();
     // 2. residual columns of the segment, descending rank
@@ -2265,6 +2278,468 @@ __global__ __launch_bounds__(kOrdThreads) void k_order_desc(const unsigned char*
     for (int w = 0; w < wid; w++) pos += s_wave[w];
     for (long long c = hi - 1; c >= lo; c--)
         if (kind[c] == 3) order[pos++] = (unsigned)c;
+}
+
+// ---------------------------------------------------------------------
+// The reference's semantics with many columns in flight.
+//
+// In the reference's loop (cohomology_general.py:255-366) a column adds, at
+// each pivot, the column stored for that pivot and stops at the first pivot
+// with none; it is then stored iff it fits (cohomology_general.py:340-356).
+// A stored column never changes and a pivot is stored at most once (a column
+// that finds one stored keeps reducing), so every addition a column makes is
+// the one the serial loop makes, whenever it is made.  Only the decision to
+// stop depends on the order: a column may stop at pivot p only once every
+// earlier column (in the reference's order) has settled, and whether it is
+// stored depends on the store's fill after them.  So each wavefront takes
+// the next column of a segment, reduces it as far as the stored columns
+// allow, and commits in order: it waits for the previous position's record
+// (the store's fill after it), re-reads the owner of its pivot, and then
+// continues (a column committed meanwhile stored that pivot) or stops and
+// stores or drops.  A stored pivot is marked pending in own[] before the
+// record is written and the rows follow the record, so the chain of records
+// stays short; a column that meets a pending owner waits for its rows.
+//
+// Coherence: all wavefronts of a segment run on one XCD (an XCD claims a
+// whole segment, and a wavefront only takes columns of its XCD's segment),
+// so every shared word (own[], records, stored rows) meets in that XCD's L2:
+// loads are device-scope (they bypass the CU's L1), and a store that another
+// one publishes is drained (s_waitcnt vmcnt(0)) before the publishing store
+// is issued.
+
+constexpr int kPipeLdsMax = 20480;                   // LDS words per wavefront at most
+constexpr unsigned long long kPending = 1ull << 63;  // own[]: stored, rows being written
+constexpr unsigned long long kNoRecord = ~0ull;
+constexpr int kPipeSlots = 16;                       // open segments per XCD
+constexpr int kPipeNext = 8 * kPipeSlots;            // ctl: next segment to open
+constexpr int kPipeAbort = kPipeNext + 1;            // ctl: stop (an error)
+constexpr int kPipeCtl = kPipeNext + 8;              // ctl: [8 x kPipeSlots slots, next, abort]
+
+struct PipeArgs {
+    RedArgs a;
+    const unsigned* order;          // residual columns, descending
+    const long long* seg_pool;      // kSegPoolWords per segment
+    const int* seg_lo;              // nseg: first position of each segment in order
+    const int* seg_hi;              // nseg: one past its last position
+    int* seg_next;                  // nseg: next position handed out, relative to seg_lo
+    int* seg_done;                  // nseg: positions settled
+    const int* seg_order;           // nseg: segments, most residual columns first
+    const unsigned long long* seg_state;  // nseg: store fill after the prefill
+    unsigned long long* rec;        // per position: store fill after it (used << 32 | words)
+    int* ctl;                       // kPipeCtl control words
+    unsigned* gbits;                // global bitmaps, wmax words per workgroup (or null)
+    long long wmax;
+    long long spin_limit;
+    int nseg;
+    int arena;                      // LDS words per wavefront
+    int maxcol;
+    int exact;                      // the complete reduction: every column is stored
+    int force_global;               // bitmaps in global memory even where LDS holds them
+    int inflight;                   // columns of a segment in flight at most, while
+                                    // unopened segments remain
+};
+
+__device__ __forceinline__ void vm_drain() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+__device__ __forceinline__ unsigned long long ld_coh64(const unsigned long long* p) {
+    return __hip_atomic_load(const_cast<unsigned long long*>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_coh64(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int ld_coh_i(const int* p) {
+    return __hip_atomic_load(const_cast<int*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// First error wins; every wavefront leaves at its next check of the abort word.
+__device__ bool pipe_fail(const PipeArgs& P, int code) {
+    if (fph_lane() == 0) {
+        atomicCAS(&P.a.status[0], 0, code);
+        atomicExch(&P.ctl[kPipeAbort], 1);
+    }
+    return false;
+}
+
+__device__ __forceinline__ int wave_excl_scan(int v, int* total) {
+    const int lane = fph_lane();
+    int incl = v;
+    for (int o = 1; o < FPH_WAVE; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    *total = __shfl(incl, FPH_WAVE - 1);
+    return incl - v;
+}
+
+// The store's fill after the position before r (after the prefill for the
+// segment's first position); kNoRecord while that position is unsettled.
+__device__ __forceinline__ unsigned long long pipe_prev(const PipeArgs& P, int s, long long r) {
+    return r == P.seg_lo[s] ? P.seg_state[s] : ld_coh64(&P.rec[r - 1]);
+}
+
+// Flips rows src[b0 + u * 64 + lane] (u < kPipeUnroll, < n) of a sorted
+// column: all loads, then all bit flips, then the summary updates, so that
+// the memory round trips of the chunks overlap.  delta += change in the
+// working column's length, lmax = max(lmax, highest row flipped).  False
+// when a row lies outside the segment.
+constexpr int kPipeUnroll = 4;
+
+template <bool G>
+__device__ __forceinline__ bool pipe_flip_chunk(const unsigned* src, int n, int b0,
+                                                long long row_lo, int rows, unsigned* bits,
+                                                unsigned* summ, int& delta, int& lmax) {
+    const int lane = fph_lane();
+    int x[kPipeUnroll];
+    bool ok[kPipeUnroll];
+    unsigned old[kPipeUnroll];
+    bool bad = false;
+#pragma unroll
+    for (int u = 0; u < kPipeUnroll; u++) {
+        const int e = b0 + u * FPH_WAVE + lane;
+        ok[u] = false;
+        x[u] = 0;
+        if (e < n) {
+            const long long xx = (long long)ld_coh(src + e) - row_lo;
+            ok[u] = xx >= 0 && xx < rows;
+            bad |= !ok[u];
+            x[u] = ok[u] ? (int)xx : 0;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kPipeUnroll; u++) {
+        old[u] = 0u;
+        if (ok[u]) {
+            const unsigned m = 1u << (x[u] & 31);
+            if constexpr (G)
+                old[u] = __hip_atomic_fetch_xor(&bits[x[u] >> 5], m, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+            else
+                old[u] = atomicXor(&bits[x[u] >> 5], m);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kPipeUnroll; u++) {
+        const unsigned m = 1u << (x[u] & 31);
+        if (ok[u] && (old[u] == 0u || old[u] == m)) {
+            const int w = x[u] >> 5;
+            atomicXor(&summ[w >> 5], 1u << (w & 31));
+        }
+        const unsigned long long mok = __ballot(ok[u]);
+        delta += 2 * __popcll(__ballot(ok[u] && !(old[u] & m))) - __popcll(mok);
+        if (mok) lmax = max(lmax, __builtin_amdgcn_readlane(x[u], 63 - __clzll((long long)mok)));
+    }
+    return !__ballot(bad);
+}
+
+// Column order[r] of segment s, by one wavefront; bits/summ as in
+// serial_bitmap (all zero on entry and on return).  False on error or abort.
+template <bool G>
+__device__ bool pipe_column(const PipeArgs& P, int s, long long r, unsigned* bits,
+                            unsigned* summ, int& nxor, long long& merged, int& longest) {
+    const RedArgs& a = P.a;
+    const int lane = fph_lane();
+    const long long* sp = P.seg_pool + kSegPoolWords * s;
+    const long long base = sp[0], cap = sp[1], row_lo = sp[2];
+    const int rows = (int)sp[3];
+    const int S = ((rows + 31) / 32 + 31) / 32;
+    const int c = (int)P.order[r];
+    const int lo_c = a.off[c];
+    const int n0 = min(a.off[c + 1] - lo_c, P.maxcol);
+    unsigned long long* own = a.own + (row_lo - a.row_base);  // by local row
+    int len = 0, p = 0, phigh = 0;
+    if (n0 > 0) {
+        int d0 = 0, lm0 = 0;
+        bool bad = false;
+        for (int b0 = 0; b0 < n0; b0 += kPipeUnroll * FPH_WAVE)
+            if (!pipe_flip_chunk<G>(a.data + lo_c, n0, b0, row_lo, rows, bits, summ, d0, lm0))
+                bad = true;
+        if (bad) return pipe_fail(P, 3);
+        len = n0;
+        p = (int)((long long)a.data[lo_c] - row_lo);
+        phigh = (int)((long long)a.data[lo_c + n0 - 1] - row_lo);
+    }
+    unsigned long long prev = kNoRecord;
+    Deadline dl;
+    while (len > 0) {
+        unsigned long long v = ld_coh64(&own[p]);
+        if (v == kNoOwner) {
+            // p may be final: wait for the previous position's record,
+            // watching for an owner stored meanwhile
+            for (int it = 0; prev == kNoRecord; it++) {
+                prev = pipe_prev(P, s, r);
+                if (prev != kNoRecord) break;
+                if ((it & 3) == 3) {
+                    v = ld_coh64(&own[p]);
+                    if (v != kNoOwner) break;
+                }
+                if (ld_coh_i(&P.ctl[kPipeAbort])) return false;
+                if (dl.expired(P.spin_limit)) return pipe_fail(P, 5);
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (v == kNoOwner) v = ld_coh64(&own[p]);  // final once the record is seen
+            if (v == kNoOwner) break;
+        }
+        while (v & kPending) {  // stored; its rows are being written
+            if (ld_coh_i(&P.ctl[kPipeAbort])) return false;
+            if (dl.expired(P.spin_limit)) return pipe_fail(P, 5);
+            __builtin_amdgcn_s_sleep(1);
+            v = ld_coh64(&own[p]);
+        }
+        const unsigned* ob = a.pool + (v & 0xFFFFFFFFull);  // [len, rows...]
+        const int nb = (int)ld_coh(ob);
+        int delta = 0, lmax = 0;
+        bool badw = false;
+        for (int b0 = 0; b0 < nb; b0 += kPipeUnroll * FPH_WAVE) {
+            if (!pipe_flip_chunk<G>(ob + 1, nb, b0, row_lo, rows, bits, summ, delta, lmax))
+                badw = true;
+        }
+        if (badw) return pipe_fail(P, 3);
+        merged += 2LL * len + nb + delta;
+        len += delta;
+        phigh = max(phigh, lmax);
+        nxor++;
+        if (len > P.maxcol) return pipe_fail(P, 4);  // the reference truncates: list form
+        if (len == 0) break;
+        // the next pivot: the lowest set bit above p, through the summary
+        const int ws = p >> 5;
+        const int s_lo = ws >> 5, s_hi = min(phigh >> 10, S - 1);
+        int np = -1;
+        for (int i0 = s_lo; i0 <= s_hi; i0 += FPH_WAVE) {
+            const int i = i0 + lane;
+            unsigned val = i <= s_hi ? summ[i] : 0u;
+            if (i == s_lo) val &= ~0u << (ws & 31);
+            const unsigned long long hit = __ballot(val != 0u);
+            if (hit) {
+                const int l = __ffsll((long long)hit) - 1;
+                int cand = -1;
+                if (lane == l) {
+                    const int w = i * 32 + __builtin_ctz(val);
+                    const unsigned bw = bm_load<G>(&bits[w]);
+                    cand = bw ? w * 32 + __builtin_ctz(bw) : -1;
+                }
+                np = __builtin_amdgcn_readlane(cand, l);
+                break;
+            }
+        }
+        if (!(np > p)) return pipe_fail(P, 3);
+        p = np;
+    }
+    // settle: the store's fill after every earlier column
+    for (; prev == kNoRecord;) {
+        prev = pipe_prev(P, s, r);
+        if (prev != kNoRecord) break;
+        if (ld_coh_i(&P.ctl[kPipeAbort])) return false;
+        if (dl.expired(P.spin_limit)) return pipe_fail(P, 5);
+        __builtin_amdgcn_s_sleep(1);
+    }
+    unsigned long long out = prev;
+    long long at = -1;
+    if (len > 0) {
+        longest = max(longest, len);
+        const long long used = (long long)(prev >> 32);
+        const long long stored = (long long)(prev & 0xFFFFFFFFull);
+        if (P.exact) {  // stored columns bump-allocated from the pool
+            const unsigned long long need =
+                ((unsigned long long)len + 1 + kLine - 1) / kLine * kLine;
+            unsigned long long at_u = 0;
+            if (lane == 0) at_u = atomicAdd(a.pool_ptr, need);
+            at_u = __shfl(at_u, 0);
+            if (at_u + need > a.pool_cap || at_u + need > 0xFFFFFFFFull) return pipe_fail(P, 1);
+            at = (long long)at_u;
+        } else if (used + len <= cap) {
+            at = base + stored;
+            out = ((unsigned long long)(used + len) << 32) | (unsigned long long)(stored + len + 1);
+        }
+        if (at >= 0) {
+            if (lane == 0) st_coh64(&own[p], kPending);
+            vm_drain();
+        }
+    }
+    if (lane == 0) {
+        a.pivot[c] = len > 0 ? (int)(row_lo + p) : -1;
+        st_coh64(&P.rec[r], out);
+        atomicAdd(&P.seg_done[s], 1);
+    }
+    if (len == 0) return true;
+    // rows out (when stored) in ascending order, and the bitmap cleared
+    int written = 0;
+    const int sw_hi = min(phigh >> 10, S - 1);
+    for (int sw0 = p >> 10; sw0 <= sw_hi; sw0 += FPH_WAVE) {
+        const int sw = sw0 + lane;
+        const unsigned val = sw <= sw_hi ? summ[sw] : 0u;
+        int cnt = 0;
+        for (unsigned m = val; m; m &= m - 1)
+            cnt += __popc(bm_load<G>(&bits[sw * 32 + __builtin_ctz(m)]));
+        int tot;
+        long long o = wave_excl_scan(cnt, &tot) + written;
+        for (unsigned m = val; m; m &= m - 1) {
+            const int w = sw * 32 + __builtin_ctz(m);
+            unsigned bw = bm_load<G>(&bits[w]);
+            if (at >= 0)
+                for (; bw; bw &= bw - 1)
+                    st_coh(a.pool + at + 1 + o++,
+                           (unsigned)(row_lo + w * 32 + __builtin_ctz(bw)));
+            bm_zero<G>(&bits[w]);
+        }
+        if (val) summ[sw] = 0u;
+        written += tot;
+    }
+    if (written != len) return pipe_fail(P, 3);
+    if (at >= 0) {
+        if (lane == 0) st_coh(a.pool + at, (unsigned)len);
+        vm_drain();
+        if (lane == 0)
+            st_coh64(&own[p], ((unsigned long long)(c + 1) << 32) | (unsigned long long)at);
+    }
+    return true;
+}
+
+// One wavefront per workgroup.  Each XCD keeps up to kPipeSlots segments
+// open (an XCD opens whole segments, most residual columns first; all the
+// columns of a segment are taken by wavefronts of its XCD).  A wavefront
+// takes the next column of the first open segment of its XCD with columns
+// left and fewer than `inflight` of them unsettled; when none qualifies it
+// opens the next segment in a free slot (or one whose columns are all
+// handed out).  Once every segment is open the cap is lifted.
+__global__ __launch_bounds__(FPH_WAVE) void k_reduce_pipe(PipeArgs P) {
+    extern __shared__ unsigned smem[];
+    const int lane = fph_lane();
+    const int xcc = (int)(__builtin_amdgcn_s_getreg((20) | (0 << 6) | (3 << 11)) & 7);
+    for (int i = lane; i < P.arena; i += FPH_WAVE) smem[i] = 0u;
+    wave_sync();
+    unsigned* gslot = P.gbits ? P.gbits + (long long)blockIdx.x * P.wmax : nullptr;
+    int nxor = 0, longest = 0;
+    long long merged = 0;
+    int* slots = P.ctl + xcc * kPipeSlots;
+    while (!ld_coh_i(&P.ctl[kPipeAbort])) {
+        const bool all_open = ld_coh_i(&P.ctl[kPipeNext]) >= P.nseg;
+        int seg = -1, pos = -1, freeable = -1, freeable_val = -1;
+        bool left = false;  // an open segment of this XCD has columns left
+        for (int k = 0; k < kPipeSlots && seg < 0; k++) {
+            const int cur = ld_coh_i(slots + k);
+            if (cur == -1) {
+                if (freeable < 0) freeable = k, freeable_val = cur;
+                continue;
+            }
+            if (cur < 0) {  // being opened
+                left = true;
+                continue;
+            }
+            const int lo = P.seg_lo[cur], R = P.seg_hi[cur] - lo;
+            const int nx = ld_coh_i(&P.seg_next[cur]);
+            if (nx >= R) {
+                if (freeable < 0) freeable = k, freeable_val = cur;
+                continue;
+            }
+            left = true;
+            if (!all_open && nx - ld_coh_i(&P.seg_done[cur]) >= P.inflight) continue;
+            int i = 0;
+            if (lane == 0) i = atomicAdd(&P.seg_next[cur], 1);
+            i = __shfl(i, 0);
+            if (i < R) {
+                seg = cur;
+                pos = lo + i;
+            }
+        }
+        if (seg >= 0) {
+            const int rows = (int)P.seg_pool[kSegPoolWords * seg + 3];
+            const int W = (rows + 31) / 32, S = (W + 31) / 32;
+            const bool ok =
+                W + S <= P.arena && !P.force_global
+                    ? pipe_column<false>(P, seg, pos, smem + S, smem, nxor, merged, longest)
+                    : pipe_column<true>(P, seg, pos, gslot, smem, nxor, merged, longest);
+            if (!ok) break;
+            continue;
+        }
+        if (all_open) {
+            if (!left) break;  // nothing left for this XCD
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+        // open the next segment in a free slot (only if the slot still holds
+        // what the scan found there: -1 or a segment with every column taken)
+        if (freeable >= 0 && lane == 0) {
+            if (atomicCAS(slots + freeable, freeable_val, -2) == freeable_val) {
+                const int t = atomicAdd(&P.ctl[kPipeNext], 1);
+                __hip_atomic_store(slots + freeable, t < P.nseg ? P.seg_order[t] : -1,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (freeable < 0) __builtin_amdgcn_s_sleep(2);
+        wave_sync();
+    }
+    if (lane == 0) {
+        if (nxor) atomicAdd(&P.a.status[2], nxor);
+        if (merged) count_merged(P.a, merged);
+        if (longest) atomicMax(&P.a.status[3], longest);
+    }
+}
+
+// Positions of each segment in order (contiguous: order is descending and a
+// segment is a column range), its cost rank, and the control words.
+__global__ void k_pipe_prep(const unsigned* __restrict__ order, const int* __restrict__ res_count,
+                            const int* __restrict__ seg_cols, int nseg, long long K,
+                            int* __restrict__ seg_lo, int* __restrict__ seg_hi,
+                            int* __restrict__ seg_next, int* __restrict__ seg_done,
+                            int* __restrict__ ctl) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < kPipeCtl) ctl[s] = s < kPipeNext ? -1 : 0;
+    if (s >= nseg) return;
+    const int R = *res_count;
+    const int c0 = seg_cols ? seg_cols[s] : 0;
+    const int c1 = seg_cols ? seg_cols[s + 1] : (int)K;
+    int lo = 0, hi = R;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int)order[mid] >= c1) lo = mid + 1;
+        else hi = mid;
+    }
+    int lo2 = lo, hi2 = R;
+    while (lo2 < hi2) {
+        const int mid = (lo2 + hi2) >> 1;
+        if ((int)order[mid] >= c0) lo2 = mid + 1;
+        else hi2 = mid;
+    }
+    seg_lo[s] = lo;
+    seg_hi[s] = lo2;
+    seg_next[s] = 0;
+    seg_done[s] = 0;
+}
+
+// seg_order: segments by descending residual count (ties by index); up to
+// kPipeRankMax segments, beyond that in index order.
+constexpr int kPipeRankMax = 8192;
+
+__global__ void k_pipe_rank(const int* __restrict__ seg_lo, const int* __restrict__ seg_hi,
+                            int nseg, int* __restrict__ seg_order) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    if (nseg > kPipeRankMax) {
+        seg_order[s] = s;
+        return;
+    }
+    const int rs = seg_hi[s] - seg_lo[s];
+    int rank = 0;
+    for (int t = 0; t < nseg; t++) {
+        const int rt = seg_hi[t] - seg_lo[t];
+        rank += rt > rs || (rt == rs && t < s);
+    }
+    seg_order[rank] = s;
+}
+
+// The prefill of each segment (one workgroup each) and its record.
+__global__ __launch_bounds__(kSerThreads) void k_pipe_prefill(RedArgs a, const int* seg_cols,
+                                                             const long long* seg_pool, int exact,
+                                                             unsigned long long* seg_state) {
+    __shared__ SerShared sh;
+    const int seg = blockIdx.x;
+    const int c0 = seg_cols ? seg_cols[seg] : 0;
+    const int c1 = seg_cols ? seg_cols[seg + 1] : (int)a.K;
+    const long long used = serial_prefill(a, sh, c0, c1, seg_pool[kSegPoolWords * seg + 1]);
+    if (threadIdx.x == 0) seg_state[seg] = exact ? 0ull : (unsigned long long)used << 32;
 }
 
 inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
@@ -2532,6 +3007,166 @@ extern "C" int fph_reduce_replay(int64_t K, const int32_t* col_off, const uint32
     const long long limit_ticks = secs * (long long)std::max(khz, 1) * 1000LL;
     // 8 XCDs receive workgroups in turn: 8 x 16 puts 16 on the XCD taking part
     k_replay<<<8 * 16, kPhaseBWaves * FPH_WAVE, lds_b, s>>>(a, order, res_count, qs, limit_ticks);
+    FPH_LAUNCH_CHECK();
+    return FPH_OK;
+}
+
+// ---- fph_reduce_pipe ---------------------------------------------------
+
+namespace {
+
+struct PipeLayout {
+    int grid = 0, arena = 0;
+    long long wmax = 0;
+    size_t rec = 0, seg_lo = 0, seg_hi = 0, seg_next = 0, seg_done = 0, seg_order = 0,
+           seg_state = 0, ctl = 0, gbits = 0, total = 0;
+};
+
+// Launch shape and workspace of fph_reduce_pipe.  A segment whose bitmap
+// and summary fit kPipeLdsMax words keeps them in LDS; otherwise the summary
+// is in LDS and the bitmap in a global slot per workgroup.  Returns false
+// when a segment's summary alone does not fit.
+bool pipe_layout(int64_t K, int64_t nseg, int64_t max_rows, int cus, bool force_global,
+                 PipeLayout& L) {
+    const long long W = (max_rows + 31) / 32, S = (W + 31) / 32;
+    if (S > kPipeLdsMax) return false;
+    // the arena holds the largest bitmap that fits, so that segments up to
+    // that size use LDS; larger ones keep only their summary there
+    const bool global = W + S > kPipeLdsMax || force_global;
+    L.arena = (int)std::max<long long>(std::min<long long>(W + S, kPipeLdsMax), 64);
+    // workgroups per CU by LDS (160 KB), at most 4; global slots at most 256 MiB
+    int wpc = (int)std::min<long long>(4, std::max<long long>(1, 163840 / (4LL * L.arena + 1024)));
+    if (global)
+        while (wpc > 1 && (long long)cus * wpc * W * 4 > (256LL << 20)) wpc--;
+    L.grid = cus * wpc;
+    L.wmax = global ? W : 0;
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        const size_t here = at;
+        at += align_up(bytes);
+        return here;
+    };
+    L.rec = take(8 * (size_t)std::max<int64_t>(K, 1));
+    L.seg_lo = take(4 * (size_t)nseg);
+    L.seg_hi = take(4 * (size_t)nseg);
+    L.seg_next = take(4 * (size_t)nseg);
+    L.seg_done = take(4 * (size_t)nseg);
+    L.seg_order = take(4 * (size_t)nseg);
+    L.seg_state = take(8 * (size_t)nseg);
+    L.ctl = take(4 * kPipeCtl);
+    L.gbits = take(global ? 4 * (size_t)L.grid * (size_t)W : 4);
+    L.total = at;
+    return true;
+}
+
+}  // namespace
+
+extern "C" int fph_reduce_pipe(int64_t K, const int32_t* col_off, const uint32_t* col_data,
+                               int64_t row_base, int64_t nrows, const int32_t* seg_cols,
+                               int64_t nseg, const int64_t* seg_pool, int64_t max_rows,
+                               int64_t pool_start, const uint8_t* kind, uint64_t* own,
+                               int32_t* pivot, const int32_t* res_list, const int32_t* res_count,
+                               uint32_t* pool, int64_t pool_cap, void* scratch,
+                               size_t scratch_bytes, void* ws, size_t* ws_bytes, int32_t* status,
+                               int variant, int64_t max_col, int wait_seconds, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (K < 0 || nrows < 0 || pool_cap < 0 || nseg < 1 || nseg > 0x7FFFFFFFLL || !ws_bytes ||
+        max_rows < 0 || max_rows > 0x7FFFFFFFLL)
+        return FPH_ERR_ARG;
+    if (max_col == 0) max_col = kSerialMaxCol;
+    if (max_col < 1 || max_col > 0x7FFFFFFFLL) return FPH_ERR_ARG;
+    int dev = 0, cus = 0, khz = 0;
+    FPH_CHECK_HIP(hipGetDevice(&dev));
+    FPH_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    PipeLayout L;
+    if (!pipe_layout(K, nseg, max_rows, std::max(cus, 8), (variant & 4) != 0, L))
+        return FPH_ERR_LIMIT;
+    if (!ws) {
+        *ws_bytes = L.total;
+        return FPH_OK;
+    }
+    if (*ws_bytes < L.total || !scratch || !status || !seg_pool || !pool) return FPH_ERR_ARG;
+    if (nseg > 1 && !seg_cols) return FPH_ERR_ARG;
+    if (scratch_bytes < fph_reduce_scratch_bytes(K)) return FPH_ERR_ARG;
+    if (pool_start < 0 || pool_start > pool_cap || pool_cap > 0xFFFFFFFFLL) return FPH_ERR_ARG;
+    if (variant & 1) return FPH_ERR_ARG;  // the sorted-list form is fph_reduce_serial's
+    const size_t k4 = align_up(4 * (size_t)std::max<int64_t>(K, 1));
+    char* w = (char*)scratch;
+    char* wb = (char*)ws;
+    RedArgs a;
+    a.off = col_off;
+    a.data = col_data;
+    a.row_base = row_base;
+    a.nrows = nrows;
+    a.K = K;
+    a.kind = kind;
+    a.own = (unsigned long long*)own;
+    a.pivot = pivot;
+    a.desc = (unsigned*)w;
+    unsigned* keys = (unsigned*)(w + 3 * k4);
+    unsigned* order = (unsigned*)(w + 4 * k4);
+    void* sort_tmp = w + 5 * k4;
+    size_t sort_sz = sort_bytes(K);
+    a.pool_ptr = (unsigned long long*)(w + 5 * k4 + sort_sz);
+    a.pool = pool;
+    a.pool_cap = (unsigned long long)pool_cap;
+    a.status = status;
+    a.use_lines = 0;
+    a.lines = nullptr;
+    PipeArgs P;
+    P.a = a;
+    P.order = order;
+    P.seg_pool = (const long long*)seg_pool;
+    P.seg_lo = (int*)(wb + L.seg_lo);
+    P.seg_hi = (int*)(wb + L.seg_hi);
+    P.seg_next = (int*)(wb + L.seg_next);
+    P.seg_done = (int*)(wb + L.seg_done);
+    P.seg_order = (int*)(wb + L.seg_order);
+    P.seg_state = (unsigned long long*)(wb + L.seg_state);
+    P.rec = (unsigned long long*)(wb + L.rec);
+    P.ctl = (int*)(wb + L.ctl);
+    P.gbits = L.wmax > 0 ? (unsigned*)(wb + L.gbits) : nullptr;
+    P.wmax = L.wmax;
+    P.nseg = (int)nseg;
+    P.arena = L.arena;
+    P.maxcol = (variant & 2) ? 0x7FFFFFFF : (int)max_col;
+    P.exact = (variant & 2) ? 1 : 0;
+    P.force_global = (variant & 4) ? 1 : 0;
+    P.inflight = std::max(8, L.grid / 16);  // half of an XCD's wavefronts
+    FPH_CHECK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    const long long secs = wait_seconds > 0 ? (long long)wait_seconds : 120LL;
+    P.spin_limit = secs * (long long)std::max(khz, 1) * 1000LL;
+    k_reduce_init<<<1, 64, 0, s>>>(status, a.pool_ptr, col_off, K, a);
+    FPH_LAUNCH_CHECK();
+    k_set_u64<<<1, 64, 0, s>>>(a.pool_ptr, (unsigned long long)pool_start);
+    FPH_LAUNCH_CHECK();
+    if (K > 0) {
+        const unsigned grid_w =
+            (unsigned)std::min<int64_t>(1024, (K + kPhaseAWaves - 1) / kPhaseAWaves);
+        k_publish_apparent<<<grid_w, kPhaseAWaves * FPH_WAVE, 0, s>>>(a, K, 0);
+        FPH_LAUNCH_CHECK();
+        const int orc = launch_order(kind, res_list, res_count, K, keys, order, sort_tmp, sort_sz, s);
+        if (orc) return orc;
+        k_fill_u64<<<fph_blocks(K, 256), 256, 0, s>>>(P.rec, K, kNoRecord);
+        if (P.gbits)
+            k_fill_u32<<<fph_blocks((long long)L.grid * L.wmax, 256), 256, 0, s>>>(
+                P.gbits, (long long)L.grid * L.wmax, 0u);
+        k_pipe_prep<<<fph_blocks(std::max<int64_t>(nseg, kPipeCtl), 256), 256, 0, s>>>(
+            order, res_count, seg_cols, (int)nseg, K, (int*)P.seg_lo, (int*)P.seg_hi, P.seg_next,
+            P.seg_done, P.ctl);
+        FPH_LAUNCH_CHECK();
+        k_pipe_rank<<<fph_blocks(nseg, 256), 256, 0, s>>>(P.seg_lo, P.seg_hi, (int)nseg,
+                                                          (int*)P.seg_order);
+        k_pipe_prefill<<<(unsigned)nseg, kSerThreads, 0, s>>>(a, seg_cols, P.seg_pool, P.exact,
+                                                              (unsigned long long*)P.seg_state);
+        FPH_LAUNCH_CHECK();
+        const size_t lds = 4 * (size_t)L.arena;
+        FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_reduce_pipe,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        k_reduce_pipe<<<(unsigned)L.grid, FPH_WAVE, lds, s>>>(P);
+        FPH_LAUNCH_CHECK();
+    }
+    k_reduce_fini<<<1, 64, 0, s>>>(status, a.pool_ptr);
     FPH_LAUNCH_CHECK();
     return FPH_OK;
 }

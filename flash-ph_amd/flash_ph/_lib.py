@@ -61,6 +61,8 @@ SIGNATURES = {
                                  _SZ, _P, _I32, _P]),
     "fph_reduce_serial": (_I32, [_I64, _P, _P, _I64, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P,
                                  _P, _P, _I64, _P, _SZ, _P, _I32, _I64, _P]),
+    "fph_reduce_pipe": (_I32, [_I64, _P, _P, _I64, _I64, _P, _I64, _P, _I64, _I64, _P, _P, _P,
+                               _P, _P, _P, _I64, _P, _SZ, _P, _PSZ, _P, _I32, _I64, _I32, _P]),
     "fph_diagram": (_I32, [_I64, _P, _P, _P, _P, _P, _I64, _P, _P, _I64, _I32, _P, _P, _P, _I64,
                            _P, _P, _P, _P, _PSZ, _P]),
     "fph_clear_pivots": (_I32, [_I64, _P, _P, _I64, _I64, _P, _P]),

@@ -66,6 +66,22 @@ EXACT_SERIAL_MIN_ROWS = 100_000
 # Working column of the serial reducer: 0 = bitmap (list form only where a
 # column would pass the reference's MAX_COL), 1 = sorted list.
 SERIAL_VARIANT = 0
+# The reference-semantics reduction with many columns in flight
+# (fph_reduce_pipe); False: one workgroup per segment (fph_reduce_serial).
+# The sorted-list form (a column past MAX_COL) is always the serial one.
+PIPE = True
+# Tests: the pipelined reducer's working bitmaps in global memory even where
+# LDS holds them.
+PIPE_FORCE_GLOBAL = False
+# Rows whose bitmap and summary fit the pipelined reducer's LDS arena
+# (20480 words); larger segments reduce faster one workgroup each (their
+# bitmaps in global memory take 256 threads' flips per round trip).
+_PIPE_LDS_WORDS = 20480
+
+
+def _pipe_fits(rows: int) -> bool:
+    w = (rows + 31) // 32
+    return w + (w + 31) // 32 <= _PIPE_LDS_WORDS
 # When a list, every reduction attempt appends (pool words, status words).
 REDUCE_LOG = None
 
@@ -548,6 +564,7 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
             base += _round32(w)
         bm_hi = base
         pool_start = base
+        max_rows = max(rows for _, _, rows in ref_sizes)
         seg_pool_t = torch.tensor(seg_pool, dtype=torch.int64).to(device)
         # + two growing off-chip buffers per segment (list form only), and
         # for the complete reduction every stored column
@@ -585,13 +602,23 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
         else:
             if pool_words > 0xFFFFFFFF:
                 raise _lib.FlashPHError("reduction store exceeds 2^32 words")
-            if bm_hi > bm_lo:
-                pool[bm_lo:bm_hi].zero_()
-            check(lib.fph_reduce_serial(K, ptr(off), ptr(data), row_base, nrows, ptr(seg_cols),
-                                        nseg, ptr(seg_pool_t), pool_start, ptr(kind), ptr(own),
-                                        ptr(pivot), ptr(res_list), rescnt_p, ptr(pool),
-                                        pool_words, ptr(scratch), scratch_bytes, status_p,
-                                        state["variant"], REF_MAX_COL, s), "fph_reduce_serial")
+            if PIPE and not state["variant"] & 1 and (PIPE_FORCE_GLOBAL or _pipe_fits(max_rows)):
+                call_ws("fph_reduce_pipe",
+                        (K, ptr(off), ptr(data), row_base, nrows, ptr(seg_cols), nseg,
+                         ptr(seg_pool_t), max_rows, pool_start, ptr(kind), ptr(own), ptr(pivot),
+                         ptr(res_list), rescnt_p, ptr(pool), pool_words, ptr(scratch),
+                         scratch_bytes),
+                        (status_p, state["variant"] | (4 if PIPE_FORCE_GLOBAL else 0),
+                         REF_MAX_COL, _WAIT_SECONDS, s), device)
+            else:
+                if bm_hi > bm_lo:
+                    pool[bm_lo:bm_hi].zero_()
+                check(lib.fph_reduce_serial(K, ptr(off), ptr(data), row_base, nrows,
+                                            ptr(seg_cols), nseg, ptr(seg_pool_t), pool_start,
+                                            ptr(kind), ptr(own), ptr(pivot), ptr(res_list),
+                                            rescnt_p, ptr(pool), pool_words, ptr(scratch),
+                                            scratch_bytes, status_p, state["variant"],
+                                            REF_MAX_COL, s), "fph_reduce_serial")
         TIMER.stop(tok)
         call_ws("fph_diagram",
                 (K, ptr(kind), ptr(off), ptr(data), ptr(pivot), ptr(col_edge), row_base,

@@ -396,6 +396,32 @@ int fph_reduce_serial(int64_t K, const int32_t *col_off, const uint32_t *col_dat
                       void *scratch, size_t scratch_bytes, int32_t *status, int variant,
                       int64_t max_col, void *stream);
 
+/*
+ * fph_reduce_serial's results (same arguments, same pairs, store contents,
+ * additions and status codes) with many columns in flight: one wavefront
+ * per column takes the additions the stored columns allow and the columns
+ * settle in the reference's order (a column stops at a pivot, and is stored
+ * or dropped, only after every earlier column has settled).  Segments run
+ * concurrently, most residual columns first, each on one XCD.  seg_pool[4]
+ * is ignored: working bitmaps are per wavefront, in LDS when ceil(max_rows
+ * / 32) + ceil(max_rows / 1024) <= 20480 words, else in `ws` (summaries in
+ * LDS; FPH_ERR_LIMIT past 20480 summary words, about 2.1e7 rows).
+ * max_rows: the largest seg_pool[3].  Workspace by size query (ws == NULL).
+ * variant bit 0 (the sorted-list form) is not taken: status[0] = 4 asks the
+ * caller for fph_reduce_serial with bit 0; bit 2 puts every working bitmap
+ * in `ws` (for tests of that form on small clouds).  wait_seconds bounds a wait for
+ * an earlier column (status 5; <= 0: 120 s).
+ * Replaces _general_cohomology_reduce_v2 (cohomology_general.py:202-371)
+ * like fph_reduce_serial.
+ */
+int fph_reduce_pipe(int64_t K, const int32_t *col_off, const uint32_t *col_data,
+                    int64_t row_base, int64_t nrows, const int32_t *seg_cols, int64_t nseg,
+                    const int64_t *seg_pool, int64_t max_rows, int64_t pool_start,
+                    const uint8_t *kind, uint64_t *own, int32_t *pivot, const int32_t *res_list,
+                    const int32_t *res_count, uint32_t *pool, int64_t pool_cap, void *scratch,
+                    size_t scratch_bytes, void *ws, size_t *ws_bytes, int32_t *status,
+                    int variant, int64_t max_col, int wait_seconds, void *stream);
+
 /* ---- diagrams --------------------------------------------------------- */
 
 /*

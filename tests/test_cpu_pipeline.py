@@ -109,3 +109,18 @@ def test_cpu_apparent_scans_match_definitions(name, pts, t):
         d = scan_defs.h2_reference_walk(a, b, c, rt, ptr, idx, rank, big)
         app = d >= 0 and scan_defs.h2_oldest_face(a, b, c, rt, d, ptr, idx, rank)
         assert app == bool(wf2[k]), (name, k)
+
+
+@pytest.mark.parametrize("idx", [140, 225])
+def test_cpu_store_overflow_default_capacity(idx):
+    """Clouds of the uneven-batch workload whose reference store overflows at
+    its own capacity (sized from 3 cofacet entries per triangle, forest
+    columns included): the host pipeline drops the oracle's columns, makes
+    its additions and gives its diagram."""
+    x = cases.batch_uneven(256, seed=0)[idx]
+    t = oracle.auto_threshold(x)
+    ref = oracle.run(x, t, 1)
+    assert ref["stats"]["pool_drops"] > 0
+    dg, st = _cpu.rips(x, oracle.threshold_sq_f32(t), 1, want_stats=True)
+    assert st["drops"] == ref["stats"]["pool_drops"] and st["xor1"] == ref["stats"]["xor1"]
+    assert _eq(dg[0], ref["h0"]) and _eq(dg[1], ref["h1"])
