@@ -2325,7 +2325,11 @@ struct PipeArgs {
     int* seg_done;                  // nseg: positions settled
     const int* seg_order;           // nseg: segments, most residual columns first
     const unsigned long long* seg_state;  // nseg: store fill after the prefill
-    unsigned long long* rec;        // per position: store fill after it (used << 32 | words)
+    unsigned long long* rec;        // per position: store fill after it (used << 32 | words),
+                                    // written where a settling run ends
+    unsigned long long* fin;        // per position: candidate final pivot and length
+    unsigned long long* verdict;    // per position: stored at / not stored / continue
+    int* claim;                     // per position: 2 once its own wavefront settles from it
     int* ctl;                       // kPipeCtl control words
     unsigned* gbits;                // global bitmaps, wmax words per workgroup (or null)
     long long wmax;
@@ -2434,6 +2438,107 @@ __device__ __forceinline__ bool pipe_flip_chunk(const unsigned* src, int n, int 
     return !__ballot(bad);
 }
 
+// A column's candidate final pivot, published for the settling run:
+// (p << 32) | len, len = 0 for a column reduced to zero; kNoRecord before.
+// Verdicts: the pool offset of a stored column, or one of these.
+constexpr unsigned long long kNoVerdict = ~0ull;
+constexpr unsigned long long kContinue = ~0ull - 1;  // p was stored meanwhile: add it
+constexpr unsigned long long kUnstored = ~0ull - 2;  // settled, not stored (or zero)
+
+// Settles positions from q (claimed by the caller, whose candidate is
+// published) onwards while they are ready: one round trip reads the next 64
+// candidates, one the owners of their pivots; then, in order, a candidate
+// whose pivot has an owner (stored before the run or by an earlier candidate
+// of it) is told to continue and ends the run, otherwise it is stored if it
+// fits.  Pending owners are drained before the verdicts go out.  The run
+// ends at the first candidate not yet published or at a continued one (the
+// caller's own claim is released first); the record of its last settled
+// position lets that position's wavefront take over.  Positions inside the
+// run need no claim: their predecessors get no record, so their wavefronts
+// only wait for verdicts.  state = used << 32 | words stored.
+__device__ bool pipe_hold(const PipeArgs& P, int s, long long q, unsigned long long state) {
+    const RedArgs& a = P.a;
+    const int lane = fph_lane();
+    const long long* sp = P.seg_pool + kSegPoolWords * s;
+    const long long base = sp[0], cap = sp[1], row_lo = sp[2];
+    unsigned long long* own = a.own + (row_lo - a.row_base);
+    const long long hi = P.seg_hi[s];
+    long long used = (long long)(state >> 32), stored = (long long)(state & 0xFFFFFFFFull);
+    bool first = true;
+    long long last = -1;  // last position this run settled
+    while (true) {
+        const long long pos = q + lane;
+        const int m = (int)min<long long>(FPH_WAVE, hi - q);
+        unsigned long long f = lane < m ? ld_coh64(&P.fin[pos]) : kNoRecord;
+        const unsigned long long rdy = __ballot(f != kNoRecord);
+        int m2 = (int)(__ffsll((long long)~rdy) - 1);  // ready prefix
+        if (~rdy == 0ull) m2 = FPH_WAVE;
+        m2 = min(m2, m);
+        const int pl = (int)(f >> 32), ll = (int)(f & 0xFFFFFFFFull);
+        unsigned long long o = 0;
+        if (lane < m2 && ll > 0) o = ld_coh64(&own[pl]);
+        int stop = m2;  // first position not settled
+        bool kept = false;
+        unsigned long long vd = kUnstored;
+        long long batch_words = 0;
+        for (int l = 0; l < m2; l++) {
+            const int p_l = __shfl(pl, l), len_l = __shfl(ll, l);
+            const unsigned long long o_l = __shfl(o, l);
+            if (len_l == 0) continue;  // reduced to zero: kUnstored
+            const bool dup = __ballot(lane < l && kept && pl == p_l) != 0ull;
+            if (o_l != kNoOwner || dup) {
+                stop = l;
+                break;
+            }
+            bool keep;
+            long long at;
+            if (P.exact) {
+                keep = true;
+                at = batch_words;  // offset in this batch's block, placed below
+                batch_words += ((long long)len_l + 1 + kLine - 1) / kLine * kLine;
+            } else {
+                keep = used + len_l <= cap;
+                at = base + stored;
+                if (keep) {
+                    used += len_l;
+                    stored += len_l + 1;
+                }
+            }
+            if (lane == l) {
+                kept = keep;
+                vd = keep ? (unsigned long long)at : kUnstored;
+            }
+        }
+        if (P.exact && batch_words > 0) {
+            unsigned long long at0 = 0;
+            if (lane == 0) at0 = atomicAdd(a.pool_ptr, (unsigned long long)batch_words);
+            at0 = __shfl(at0, 0);
+            if (at0 + batch_words > a.pool_cap || at0 + batch_words > 0xFFFFFFFFull)
+                return pipe_fail(P, 1);
+            if (kept) vd += at0;
+        }
+        if (lane < stop && kept) st_coh64(&own[pl], kPending);
+        const bool cont = stop < m2;  // the candidate at `stop` continues
+        if (cont && lane == stop) __hip_atomic_store(&P.claim[pos], 0, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+        vm_drain();
+        if (lane < stop) st_coh64(&P.verdict[pos], vd);
+        if (cont && lane == stop) st_coh64(&P.verdict[pos], kContinue);
+        if (lane == 0 && stop > 0) atomicAdd(&P.seg_done[s], stop);
+        if (stop > 0) last = q + stop - 1;
+        first = false;
+        if (stop < m || q + m >= hi) {
+            // the run ends: the record of its last settled position
+            vm_drain();
+            if (lane == 0 && last >= 0)
+                st_coh64(&P.rec[last],
+                         ((unsigned long long)used << 32) | (unsigned long long)stored);
+            return true;
+        }
+        q += m;
+    }
+}
+
 // Column order[r] of segment s, by one wavefront; bits/summ as in
 // serial_bitmap (all zero on entry and on return).  False on error or abort.
 template <bool G>
@@ -2442,7 +2547,7 @@ __device__ bool pipe_column(const PipeArgs& P, int s, long long r, unsigned* bit
     const RedArgs& a = P.a;
     const int lane = fph_lane();
     const long long* sp = P.seg_pool + kSegPoolWords * s;
-    const long long base = sp[0], cap = sp[1], row_lo = sp[2];
+    const long long row_lo = sp[2];
     const int rows = (int)sp[3];
     const int S = ((rows + 31) / 32 + 31) / 32;
     const int c = (int)P.order[r];
@@ -2461,109 +2566,92 @@ __device__ bool pipe_column(const PipeArgs& P, int s, long long r, unsigned* bit
         p = (int)((long long)a.data[lo_c] - row_lo);
         phigh = (int)((long long)a.data[lo_c + n0 - 1] - row_lo);
     }
-    unsigned long long prev = kNoRecord;
     Deadline dl;
-    while (len > 0) {
-        unsigned long long v = ld_coh64(&own[p]);
-        if (v == kNoOwner) {
-            // p may be final: wait for the previous position's record,
-            // watching for an owner stored meanwhile
-            for (int it = 0; prev == kNoRecord; it++) {
-                prev = pipe_prev(P, s, r);
-                if (prev != kNoRecord) break;
-                if ((it & 3) == 3) {
-                    v = ld_coh64(&own[p]);
-                    if (v != kNoOwner) break;
-                }
+    unsigned long long vd = kNoVerdict;
+    while (true) {
+        // add stored columns while the pivot has one
+        while (len > 0) {
+            unsigned long long v = ld_coh64(&own[p]);
+            if (v == kNoOwner) break;
+            while (v & kPending) {  // stored; its rows are being written
                 if (ld_coh_i(&P.ctl[kPipeAbort])) return false;
                 if (dl.expired(P.spin_limit)) return pipe_fail(P, 5);
                 __builtin_amdgcn_s_sleep(1);
+                v = ld_coh64(&own[p]);
             }
-            if (v == kNoOwner) v = ld_coh64(&own[p]);  // final once the record is seen
-            if (v == kNoOwner) break;
+            const unsigned* ob = a.pool + (v & 0xFFFFFFFFull);  // [len, rows...]
+            const int nb = (int)ld_coh(ob);
+            int delta = 0, lmax = 0;
+            bool badw = false;
+            for (int b0 = 0; b0 < nb; b0 += kPipeUnroll * FPH_WAVE) {
+                if (!pipe_flip_chunk<G>(ob + 1, nb, b0, row_lo, rows, bits, summ, delta, lmax))
+                    badw = true;
+            }
+            if (badw) return pipe_fail(P, 3);
+            merged += 2LL * len + nb + delta;
+            len += delta;
+            phigh = max(phigh, lmax);
+            nxor++;
+            if (len > P.maxcol) return pipe_fail(P, 4);  // the reference truncates: list form
+            if (len == 0) break;
+            // the next pivot: the lowest set bit above p, through the summary
+            const int ws = p >> 5;
+            const int s_lo = ws >> 5, s_hi = min(phigh >> 10, S - 1);
+            int np = -1;
+            for (int i0 = s_lo; i0 <= s_hi; i0 += FPH_WAVE) {
+                const int i = i0 + lane;
+                unsigned val = i <= s_hi ? summ[i] : 0u;
+                if (i == s_lo) val &= ~0u << (ws & 31);
+                const unsigned long long hit = __ballot(val != 0u);
+                if (hit) {
+                    const int l = __ffsll((long long)hit) - 1;
+                    int cand = -1;
+                    if (lane == l) {
+                        const int w = i * 32 + __builtin_ctz(val);
+                        const unsigned bw = bm_load<G>(&bits[w]);
+                        cand = bw ? w * 32 + __builtin_ctz(bw) : -1;
+                    }
+                    np = __builtin_amdgcn_readlane(cand, l);
+                    break;
+                }
+            }
+            if (!(np > p)) return pipe_fail(P, 3);
+            p = np;
         }
-        while (v & kPending) {  // stored; its rows are being written
+        // a candidate: publish it, then wait for its verdict, or settle it
+        // (and what follows) once every earlier position has settled
+        if (lane == 0)
+            st_coh64(&P.fin[r], ((unsigned long long)(unsigned)p << 32) | (unsigned)len);
+        vm_drain();
+        while (true) {
+            vd = ld_coh64(&P.verdict[r]);
+            if (vd != kNoVerdict) break;
+            const unsigned long long prev = pipe_prev(P, s, r);
+            if (prev != kNoRecord) {
+                int won = 0;
+                if (lane == 0) won = atomicCAS(&P.claim[r], 0, 2) == 0;
+                won = __shfl(won, 0);
+                if (won) {
+                    if (!pipe_hold(P, s, r, prev)) return false;
+                    continue;  // its own verdict is written
+                }
+            }
             if (ld_coh_i(&P.ctl[kPipeAbort])) return false;
             if (dl.expired(P.spin_limit)) return pipe_fail(P, 5);
             __builtin_amdgcn_s_sleep(1);
-            v = ld_coh64(&own[p]);
         }
-        const unsigned* ob = a.pool + (v & 0xFFFFFFFFull);  // [len, rows...]
-        const int nb = (int)ld_coh(ob);
-        int delta = 0, lmax = 0;
-        bool badw = false;
-        for (int b0 = 0; b0 < nb; b0 += kPipeUnroll * FPH_WAVE) {
-            if (!pipe_flip_chunk<G>(ob + 1, nb, b0, row_lo, rows, bits, summ, delta, lmax))
-                badw = true;
+        if (vd != kContinue) break;
+        // the pivot was stored by an earlier column: withdraw and go on
+        if (lane == 0) {
+            st_coh64(&P.verdict[r], kNoVerdict);
+            st_coh64(&P.fin[r], kNoRecord);
         }
-        if (badw) return pipe_fail(P, 3);
-        merged += 2LL * len + nb + delta;
-        len += delta;
-        phigh = max(phigh, lmax);
-        nxor++;
-        if (len > P.maxcol) return pipe_fail(P, 4);  // the reference truncates: list form
-        if (len == 0) break;
-        // the next pivot: the lowest set bit above p, through the summary
-        const int ws = p >> 5;
-        const int s_lo = ws >> 5, s_hi = min(phigh >> 10, S - 1);
-        int np = -1;
-        for (int i0 = s_lo; i0 <= s_hi; i0 += FPH_WAVE) {
-            const int i = i0 + lane;
-            unsigned val = i <= s_hi ? summ[i] : 0u;
-            if (i == s_lo) val &= ~0u << (ws & 31);
-            const unsigned long long hit = __ballot(val != 0u);
-            if (hit) {
-                const int l = __ffsll((long long)hit) - 1;
-                int cand = -1;
-                if (lane == l) {
-                    const int w = i * 32 + __builtin_ctz(val);
-                    const unsigned bw = bm_load<G>(&bits[w]);
-                    cand = bw ? w * 32 + __builtin_ctz(bw) : -1;
-                }
-                np = __builtin_amdgcn_readlane(cand, l);
-                break;
-            }
-        }
-        if (!(np > p)) return pipe_fail(P, 3);
-        p = np;
+        vm_drain();
     }
-    // settle: the store's fill after every earlier column
-    for (; prev == kNoRecord;) {
-        prev = pipe_prev(P, s, r);
-        if (prev != kNoRecord) break;
-        if (ld_coh_i(&P.ctl[kPipeAbort])) return false;
-        if (dl.expired(P.spin_limit)) return pipe_fail(P, 5);
-        __builtin_amdgcn_s_sleep(1);
-    }
-    unsigned long long out = prev;
-    long long at = -1;
-    if (len > 0) {
-        longest = max(longest, len);
-        const long long used = (long long)(prev >> 32);
-        const long long stored = (long long)(prev & 0xFFFFFFFFull);
-        if (P.exact) {  // stored columns bump-allocated from the pool
-            const unsigned long long need =
-                ((unsigned long long)len + 1 + kLine - 1) / kLine * kLine;
-            unsigned long long at_u = 0;
-            if (lane == 0) at_u = atomicAdd(a.pool_ptr, need);
-            at_u = __shfl(at_u, 0);
-            if (at_u + need > a.pool_cap || at_u + need > 0xFFFFFFFFull) return pipe_fail(P, 1);
-            at = (long long)at_u;
-        } else if (used + len <= cap) {
-            at = base + stored;
-            out = ((unsigned long long)(used + len) << 32) | (unsigned long long)(stored + len + 1);
-        }
-        if (at >= 0) {
-            if (lane == 0) st_coh64(&own[p], kPending);
-            vm_drain();
-        }
-    }
-    if (lane == 0) {
-        a.pivot[c] = len > 0 ? (int)(row_lo + p) : -1;
-        st_coh64(&P.rec[r], out);
-        atomicAdd(&P.seg_done[s], 1);
-    }
+    const long long at = vd == kUnstored ? -1 : (long long)vd;
+    if (lane == 0) a.pivot[c] = len > 0 ? (int)(row_lo + p) : -1;
     if (len == 0) return true;
+    longest = max(longest, len);
     // rows out (when stored) in ascending order, and the bitmap cleared
     int written = 0;
     const int sw_hi = min(phigh >> 10, S - 1);
@@ -3018,8 +3106,8 @@ namespace {
 struct PipeLayout {
     int grid = 0, arena = 0;
     long long wmax = 0;
-    size_t rec = 0, seg_lo = 0, seg_hi = 0, seg_next = 0, seg_done = 0, seg_order = 0,
-           seg_state = 0, ctl = 0, gbits = 0, total = 0;
+    size_t rec = 0, fin = 0, verdict = 0, claim = 0, seg_lo = 0, seg_hi = 0, seg_next = 0,
+           seg_done = 0, seg_order = 0, seg_state = 0, ctl = 0, gbits = 0, total = 0;
 };
 
 // Launch shape and workspace of fph_reduce_pipe.  A segment whose bitmap
@@ -3047,6 +3135,9 @@ bool pipe_layout(int64_t K, int64_t nseg, int64_t max_rows, int cus, bool force_
         return here;
     };
     L.rec = take(8 * (size_t)std::max<int64_t>(K, 1));
+    L.fin = take(8 * (size_t)std::max<int64_t>(K, 1));
+    L.verdict = take(8 * (size_t)std::max<int64_t>(K, 1));
+    L.claim = take(4 * (size_t)std::max<int64_t>(K, 1));
     L.seg_lo = take(4 * (size_t)nseg);
     L.seg_hi = take(4 * (size_t)nseg);
     L.seg_next = take(4 * (size_t)nseg);
@@ -3124,6 +3215,9 @@ extern "C" int fph_reduce_pipe(int64_t K, const int32_t* col_off, const uint32_t
     P.seg_order = (int*)(wb + L.seg_order);
     P.seg_state = (unsigned long long*)(wb + L.seg_state);
     P.rec = (unsigned long long*)(wb + L.rec);
+    P.fin = (unsigned long long*)(wb + L.fin);
+    P.verdict = (unsigned long long*)(wb + L.verdict);
+    P.claim = (int*)(wb + L.claim);
     P.ctl = (int*)(wb + L.ctl);
     P.gbits = L.wmax > 0 ? (unsigned*)(wb + L.gbits) : nullptr;
     P.wmax = L.wmax;
@@ -3148,6 +3242,9 @@ extern "C" int fph_reduce_pipe(int64_t K, const int32_t* col_off, const uint32_t
         const int orc = launch_order(kind, res_list, res_count, K, keys, order, sort_tmp, sort_sz, s);
         if (orc) return orc;
         k_fill_u64<<<fph_blocks(K, 256), 256, 0, s>>>(P.rec, K, kNoRecord);
+        k_fill_u64<<<fph_blocks(K, 256), 256, 0, s>>>(P.fin, K, kNoRecord);
+        k_fill_u64<<<fph_blocks(K, 256), 256, 0, s>>>(P.verdict, K, kNoVerdict);
+        k_fill_u32<<<fph_blocks(K, 256), 256, 0, s>>>((unsigned*)P.claim, K, 0u);
         if (P.gbits)
             k_fill_u32<<<fph_blocks((long long)L.grid * L.wmax, 256), 256, 0, s>>>(
                 P.gbits, (long long)L.grid * L.wmax, 0u);

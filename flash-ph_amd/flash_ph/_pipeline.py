@@ -74,14 +74,17 @@ PIPE = True
 # LDS holds them.
 PIPE_FORCE_GLOBAL = False
 # Rows whose bitmap and summary fit the pipelined reducer's LDS arena
-# (20480 words); larger segments reduce faster one workgroup each (their
-# bitmaps in global memory take 256 threads' flips per round trip).
+# (20480 words).  Larger segments keep their bitmaps in global memory; a few
+# of them still gain from many columns in flight (n=1000, d=10 at its auto
+# threshold: 2.76M rows, 8.9 vs 32.6 ms), many at once reduce faster one
+# workgroup each (64 Clifford clouds of 2.8M rows: 132 vs 362 ms).
 _PIPE_LDS_WORDS = 20480
+_PIPE_GLOBAL_MAX_SEGS = 8
 
 
-def _pipe_fits(rows: int) -> bool:
-    w = (rows + 31) // 32
-    return w + (w + 31) // 32 <= _PIPE_LDS_WORDS
+def _use_pipe(max_rows: int, nseg: int) -> bool:
+    w = (max_rows + 31) // 32
+    return w + (w + 31) // 32 <= _PIPE_LDS_WORDS or nseg <= _PIPE_GLOBAL_MAX_SEGS
 # When a list, every reduction attempt appends (pool words, status words).
 REDUCE_LOG = None
 
@@ -602,7 +605,8 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
         else:
             if pool_words > 0xFFFFFFFF:
                 raise _lib.FlashPHError("reduction store exceeds 2^32 words")
-            if PIPE and not state["variant"] & 1 and (PIPE_FORCE_GLOBAL or _pipe_fits(max_rows)):
+            if (PIPE and not state["variant"] & 1
+                    and (PIPE_FORCE_GLOBAL or _use_pipe(max_rows, nseg))):
                 call_ws("fph_reduce_pipe",
                         (K, ptr(off), ptr(data), row_base, nrows, ptr(seg_cols), nseg,
                          ptr(seg_pool_t), max_rows, pool_start, ptr(kind), ptr(own), ptr(pivot),
