@@ -2539,6 +2539,33 @@ __device__ bool pipe_hold(const PipeArgs& P, int s, long long q, unsigned long l
     }
 }
 
+// The first 63 rows of a column of nb rows whose words [len, rows...] lane l
+// loaded into w (lane l > 0 holds row l - 1); as pipe_flip_chunk.
+template <bool G>
+__device__ __forceinline__ bool pipe_flip_first(unsigned w, int nb, long long row_lo, int rows,
+                                                unsigned* bits, unsigned* summ, int& delta,
+                                                int& lmax) {
+    const int lane = fph_lane();
+    const bool need = lane >= 1 && lane <= nb;
+    const long long xx = (long long)w - row_lo;
+    const bool ok = need && xx >= 0 && xx < rows;
+    const int x = ok ? (int)xx : 0;
+    const unsigned m = 1u << (x & 31);
+    unsigned old = 0u;
+    if (ok) {
+        if constexpr (G)
+            old = __hip_atomic_fetch_xor(&bits[x >> 5], m, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+            old = atomicXor(&bits[x >> 5], m);
+        if (old == 0u || old == m) atomicXor(&summ[(x >> 5) >> 5], 1u << ((x >> 5) & 31));
+    }
+    const unsigned long long mok = __ballot(ok);
+    delta += 2 * __popcll(__ballot(ok && !(old & m))) - __popcll(mok);
+    if (mok) lmax = max(lmax, __builtin_amdgcn_readlane(x, 63 - __clzll((long long)mok)));
+    return !__ballot(need && !ok);
+}
+
 // Column order[r] of segment s, by one wavefront; bits/summ as in
 // serial_bitmap (all zero on entry and on return).  False on error or abort.
 template <bool G>
@@ -2579,11 +2606,15 @@ __device__ bool pipe_column(const PipeArgs& P, int s, long long r, unsigned* bit
                 __builtin_amdgcn_s_sleep(1);
                 v = ld_coh64(&own[p]);
             }
-            const unsigned* ob = a.pool + (v & 0xFFFFFFFFull);  // [len, rows...]
-            const int nb = (int)ld_coh(ob);
+            // [len, rows...]: the length and the first 63 rows in one round trip
+            // (a read past the column's end is a pool word, ignored; the pool
+            // has 64 words of slack past its capacity)
+            const unsigned* ob = a.pool + (v & 0xFFFFFFFFull);
+            const unsigned w0 = ld_coh(ob + lane);
+            const int nb = __builtin_amdgcn_readlane((int)w0, 0);
             int delta = 0, lmax = 0;
-            bool badw = false;
-            for (int b0 = 0; b0 < nb; b0 += kPipeUnroll * FPH_WAVE) {
+            bool badw = !pipe_flip_first<G>(w0, nb, row_lo, rows, bits, summ, delta, lmax);
+            for (int b0 = FPH_WAVE - 1; b0 < nb; b0 += kPipeUnroll * FPH_WAVE) {
                 if (!pipe_flip_chunk<G>(ob + 1, nb, b0, row_lo, rows, bits, summ, delta, lmax))
                     badw = true;
             }
