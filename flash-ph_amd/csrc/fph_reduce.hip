@@ -1037,6 +1037,43 @@ __device__ void bm_next_pivot(const RedArgs& a, BmWork& w) {
     }
 }
 
+// Adds a published column of n <= 63 rows whose words [len, rows...] are
+// all in v (lane l holds word l, every word written) and moves the pivot to
+// the lowest set bit, in one LDS round trip: the flips and the first window
+// of the pivot scan are issued back to back, and a wavefront's LDS
+// operations complete in issue order, so the window reads see the flips.
+// The added column's pivot is the current one, so every row it flips is at
+// or above it and the lowest set bit lies at or above the current pivot's
+// word.  Returns 1, 0 when the working column became zero, or -1 when no
+// set bit lies in [pivot word, hi_w] although len > 0 (inconsistent).
+__device__ int bm_add_short(const RedArgs& a, BmWork& w, unsigned v, int n) {
+    const int lane = fph_lane();
+    const bool need = lane >= 1 && lane <= n;
+    const unsigned r = v - (unsigned)a.row_base;
+    const int wd = need ? (int)(r >> 5) : 0;
+    const unsigned bit = 1u << (r & 31);
+    // rows ascend: lane n holds the highest word
+    w.hi_w = max(w.hi_w, __builtin_amdgcn_readlane(wd, n));
+    int base = (int)((w.piv - (unsigned)a.row_base) >> 5);
+    unsigned old = 0u;
+    if (need) old = atomicXor(&w.bm[wd], bit);
+    asm volatile("" ::: "memory");  // keep the window reads after the flips
+    unsigned x = base + lane <= w.hi_w ? w.bm[base + lane] : 0u;
+    w.len += 2 * __popcll(__ballot(need && (old & bit) == 0u)) - n;
+    if (w.len == 0) return 0;
+    unsigned long long b = __ballot(x != 0u);
+    while (!b) {  // len > 0: a set bit lies at or below hi_w
+        base += FPH_WAVE;
+        if (base > w.hi_w) return -1;
+        x = base + lane <= w.hi_w ? w.bm[base + lane] : 0u;
+        b = __ballot(x != 0u);
+    }
+    const int l = __ffsll((long long)b) - 1;
+    const unsigned xw = (unsigned)__builtin_amdgcn_readlane((int)x, l);
+    w.piv = (unsigned)a.row_base + 32u * (unsigned)(base + l) + (unsigned)(__ffs(xw) - 1);
+    return 1;
+}
+
 // Publishes the set rows as [len, rows...] (write-once words) and, with
 // owner lines, the line of the pivot row tagged with column c; -1 on error.
 __device__ long long bm_publish(const RedArgs& a, const BmWork& w, Alloc& al, int c) {
@@ -1089,6 +1126,7 @@ __device__ void bm_clear(BmWork& w) {
 // csr: c starts from its cofacet column in the CSR (not yet published).
 __device__ int reduce_claim_dev_bm(const RedArgs& a, int c, unsigned src, bool csr, unsigned* bm,
                                    int* nxor, Alloc& al, long long wait) {
+    const int lane = fph_lane();
     BmWork w;
     w.bm = bm;
     w.len = 0;
@@ -1120,24 +1158,23 @@ __device__ int reduce_claim_dev_bm(const RedArgs& a, int c, unsigned src, bool c
         const unsigned p = w.piv;
         // the owner entry and, in the same round trip, the owner line
         unsigned long long lw = 0;
-        if (a.lines && fph_lane() < kLineWords)
-            lw = __hip_atomic_load(&a.lines[((long long)p - a.row_base) * kLineWords + fph_lane()],
+        if (a.lines && lane < kLineWords)
+            lw = __hip_atomic_load(&a.lines[((long long)p - a.row_base) * kLineWords + lane],
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         unsigned long long v = load_own_fast(a, p);
         if ((int)(v >> 32) - 1 <= c) {
             const long long at = bm_publish(a, w, al, c);
             if (at < 0) return -2;
-            if (fph_lane() == 0) st_coh(&a.desc[c], (unsigned)at);
+            if (lane == 0) st_coh(&a.desc[c], (unsigned)at);
             const unsigned long long mine =
                 ((unsigned long long)(c + 1) << 32) | (unsigned long long)at;
             unsigned long long old = 0;
-            if (fph_lane() == 0) old = atomicMax(&a.own[(long long)p - a.row_base], mine);
+            if (lane == 0) old = atomicMax(&a.own[(long long)p - a.row_base], mine);
             old = __shfl(old, 0);
             const int oc = (int)(old >> 32) - 1;
             if (oc < c) {
                 al.maxlen = max(al.maxlen, w.len);
-                if (oc >= 0 && fph_lane() == 0)
-                    st_coh(&a.desc[oc], (unsigned)(old & 0xFFFFFFFFull));
+                if (oc >= 0 && lane == 0) st_coh(&a.desc[oc], (unsigned)(old & 0xFFFFFFFFull));
                 bm_clear(w);
                 return oc;
             }
@@ -1151,16 +1188,18 @@ __device__ int reduce_claim_dev_bm(const RedArgs& a, int c, unsigned src, bool c
         // the column itself is read.
         unsigned vb = 0;
         int nb = -1;
+        bool complete = false;  // every word of [len, rows...] in vb
         if (a.lines && (unsigned long long)v != kNoOwner) {
             const unsigned tag = (unsigned)(v >> 32);
             const unsigned lo32 = (unsigned)lw, hi32 = (unsigned)(lw >> 32);
             const unsigned t0 = (unsigned)__builtin_amdgcn_readlane((int)hi32, 0);
             const unsigned l0 = (unsigned)__builtin_amdgcn_readlane((int)lo32, 0);
             if (t0 == tag && l0 <= (unsigned)(kLineWords - 1)) {
-                const bool need = fph_lane() >= 1 && fph_lane() <= (int)l0;
+                const bool need = lane >= 1 && lane <= (int)l0;
                 if (!__ballot(need && hi32 != tag)) {
                     nb = (int)l0;
                     vb = lo32;
+                    complete = true;
                 }
             }
         }
@@ -1172,14 +1211,28 @@ __device__ int reduce_claim_dev_bm(const RedArgs& a, int c, unsigned src, bool c
             }
         }
         const int na = w.len;
-        if (bm_flip_published(a, w, ob, vb, nb, wait) == 0xFFFFFFFFu) {
-            fail_wait(a);
-            return -2;
+        if (nb <= FPH_WAVE - 1) {
+            if (!complete) {  // rows read from the pool: wait for unwritten words
+                Deadline dl;
+                if (!settle_word(ob + lane, lane >= 1 && lane <= nb, vb, dl, wait)) {
+                    fail_wait(a);
+                    return -2;
+                }
+            }
+            if (bm_add_short(a, w, vb, nb) < 0) {
+                if (lane == 0) atomicExch(&a.status[0], 3);
+                return -2;
+            }
+        } else {
+            if (bm_flip_published(a, w, ob, vb, nb, wait) == 0xFFFFFFFFu) {
+                fail_wait(a);
+                return -2;
+            }
+            if (w.len > 0) bm_next_pivot(a, w);
         }
-        if (w.len > 0) bm_next_pivot(a, w);
         // the owner's pivot is p, so p is cleared and the pivot must rise
         if (w.len > 0 && !(w.piv > p)) {
-            if (fph_lane() == 0) atomicExch(&a.status[0], 3);
+            if (lane == 0) atomicExch(&a.status[0], 3);
             return -2;
         }
         al.merged += (unsigned long long)(na + nb + w.len);
