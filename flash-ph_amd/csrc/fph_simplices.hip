@@ -470,20 +470,67 @@ int scan_counts(const int* counts, int* off, long long K, long long* total_dev, 
     return FPH_OK;
 }
 
-size_t segsort_bytes(long long C, long long K) {
-    size_t tmp = 0;
-    rocprim::segmented_radix_sort_keys(nullptr, tmp, (const unsigned*)nullptr, (unsigned*)nullptr,
-                                       (unsigned)std::max<long long>(C, 1),
-                                       (unsigned)std::max<long long>(K, 1), (const int*)nullptr,
-                                       (const int*)nullptr, 0, 32, (hipStream_t)0);
-    return align_up(tmp);
+// Sorts each column [off[c], off[c+1]) of src ascending into dst (the
+// values of one column are distinct), one wavefront per column, in one
+// launch: a value's position is the number of smaller values in its column.
+// Up to 256 values the column sits in registers (lane l holds values l,
+// l + 64, ...) and each value is broadcast by readlane; a longer column is
+// ranked 256 values at a time against the whole column read from memory
+// (O(L^2 / 256) per column of L values; such columns are rare).  The
+// library's segmented sort takes five launches and a host round trip for
+// the same work.
+constexpr int kColReg = 4;  // values per lane held in registers
+
+__global__ __launch_bounds__(256) void k_colsort(const int* __restrict__ off, long long K,
+                                                 const unsigned* __restrict__ src,
+                                                 unsigned* __restrict__ dst) {
+    const int lane = fph_lane();
+    const long long nw = (long long)gridDim.x * (blockDim.x / FPH_WAVE);
+    for (long long c = (long long)blockIdx.x * (blockDim.x / FPH_WAVE) + threadIdx.x / FPH_WAVE;
+         c < K; c += nw) {
+        const int lo = off[c], L = off[c + 1] - lo;
+        if (L <= 0) continue;
+        for (int c0 = 0; c0 < L; c0 += kColReg * FPH_WAVE) {
+            unsigned v[kColReg];
+            int rk[kColReg];
+#pragma unroll
+            for (int q = 0; q < kColReg; q++) {
+                const int i = c0 + q * FPH_WAVE + lane;
+                v[q] = i < L ? src[lo + i] : 0xFFFFFFFFu;
+                rk[q] = 0;
+            }
+            if (L <= kColReg * FPH_WAVE) {
+                for (int j = 0; j < L; j++) {
+                    const int qj = j / FPH_WAVE;
+                    const unsigned x = (unsigned)__builtin_amdgcn_readlane(
+                        (int)(qj == 0 ? v[0] : qj == 1 ? v[1] : qj == 2 ? v[2] : v[3]),
+                        j % FPH_WAVE);
+#pragma unroll
+                    for (int q = 0; q < kColReg; q++) rk[q] += x < v[q];
+                }
+            } else {
+#pragma unroll 8
+                for (int j = 0; j < L; j++) {
+                    const unsigned x = src[lo + j];
+#pragma unroll
+                    for (int q = 0; q < kColReg; q++) rk[q] += x < v[q];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kColReg; q++) {
+                const int i = c0 + q * FPH_WAVE + lane;
+                if (i < L) dst[lo + rk[q]] = v[q];
+            }
+        }
+    }
 }
 
-int bit_width(unsigned long long x) {
-    int b = 0;
-    while (x) { b++; x >>= 1; }
-    return b;
+void sort_columns(const unsigned* raw, unsigned* out, const int* off, long long K,
+                  hipStream_t s) {
+    const long long blocks = std::min<long long>((K + 3) / 4, 4096);
+    k_colsort<<<(unsigned)std::max<long long>(blocks, 1), 256, 0, s>>>(off, K, raw, out);
 }
+
 
 }  // namespace
 
@@ -551,12 +598,12 @@ extern "C" int fph_cofacets_h1(const int32_t* edge_i, const int32_t* edge_j, int
     hipStream_t s = (hipStream_t)stream;
     if (E < 0 || C < 0 || !ws_bytes) return FPH_ERR_ARG;
     if (E + T >= 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
-    const size_t tmp = segsort_bytes(C, E);
-    const size_t need = tmp + align_up(4 * (size_t)std::max<int64_t>(C, 1));
+    // the unsorted columns, then sorted into cof_data
+    const size_t need = align_up(4 * (size_t)std::max<int64_t>(C, 1));
     if (!ws) { *ws_bytes = need; return FPH_OK; }
     if (*ws_bytes < need) return FPH_ERR_ARG;
     if (C == 0) return FPH_OK;
-    unsigned* raw = (unsigned*)((char*)ws + tmp);
+    unsigned* raw = (unsigned*)ws;
     LinkOut o{};
     o.tri_off = tri_off;
     o.tri_w = const_cast<int*>(tri_w);  // read only in this mode
@@ -566,11 +613,8 @@ extern "C" int fph_cofacets_h1(const int32_t* edge_i, const int32_t* edge_j, int
     k_edge_link<kLinkCof><<<fph_blocks(E, nw), nw * FPH_WAVE, nw * kLinkStage * 4, s>>>(
         edge_i, edge_j, E, adj_ptr, adj_idx, adj_rank, mst_mask, o);
     FPH_LAUNCH_CHECK();
-    size_t t = tmp;
-    const int end_bit = std::max(1, bit_width((unsigned long long)(E + T)));
-    FPH_CHECK_HIP(rocprim::segmented_radix_sort_keys(ws, t, raw, (unsigned*)cof_data, (unsigned)C,
-                                                     (unsigned)E, cof_off, cof_off + 1, 0, end_bit,
-                                                     s));
+    sort_columns(raw, (unsigned*)cof_data, cof_off, E, s);
+    FPH_LAUNCH_CHECK();
     return FPH_OK;
 }
 
@@ -681,21 +725,18 @@ extern "C" int fph_cofacets_h2(int64_t T, int64_t E, int64_t Q, const int32_t* e
     hipStream_t s = (hipStream_t)stream;
     if (T < 0 || C < 0 || !ws_bytes) return FPH_ERR_ARG;
     if (E + T + Q >= 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
-    const size_t tmp = segsort_bytes(C, T);
-    const size_t need = tmp + align_up(4 * (size_t)std::max<int64_t>(C, 1));
+    // the unsorted columns, then sorted into cof_data
+    const size_t need = align_up(4 * (size_t)std::max<int64_t>(C, 1));
     if (!ws) { *ws_bytes = need; return FPH_OK; }
     if (*ws_bytes < need) return FPH_ERR_ARG;
     if (C == 0) return FPH_OK;
-    unsigned* raw = (unsigned*)((char*)ws + tmp);
+    unsigned* raw = (unsigned*)ws;
     const int nw = link_waves(T);
     k_tri_link<1><<<fph_blocks(T, nw), nw * FPH_WAVE, nw * kLinkStage * 4, s>>>(
         T, E, edge_i, edge_j, adj_ptr, adj_idx, adj_rank, tri_w, tri_e, cleared, tet_off,
         (const unsigned long long*)tet_key, nullptr, cof_off, raw);
     FPH_LAUNCH_CHECK();
-    size_t t = tmp;
-    const int end_bit = std::max(1, bit_width((unsigned long long)(E + T + Q)));
-    FPH_CHECK_HIP(rocprim::segmented_radix_sort_keys(ws, t, raw, (unsigned*)cof_data, (unsigned)C,
-                                                     (unsigned)T, cof_off, cof_off + 1, 0, end_bit,
-                                                     s));
+    sort_columns(raw, (unsigned*)cof_data, cof_off, T, s);
+    FPH_LAUNCH_CHECK();
     return FPH_OK;
 }
