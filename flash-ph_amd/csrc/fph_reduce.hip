@@ -1400,15 +1400,21 @@ __global__ void k_queue_state_init(int* __restrict__ qs, const int* __restrict__
 }
 
 // Set-up of the device-wide queue with bitmap working columns, one launch:
-// owner lines zero, the ring empty, desc[] unpublished, pivot[] reset (no
-// phase A ran and the queue never reads pivot[]; k_final_own rebuilds it),
-// the unallocated pool reading kUnwritten, and the queue state.
+// the status and pool pointer (k_reduce_init's work), owner lines zero, the
+// ring empty, desc[] unpublished, pivot[] reset (no phase A ran and the
+// queue never reads pivot[]; k_final_own rebuilds it), the unallocated pool
+// reading kUnwritten, and the queue state.
 __global__ void k_queue_prep(RedArgs a, unsigned long long* __restrict__ ring,
                              int* __restrict__ qs, const int* __restrict__ res_count,
                              long long fill_end) {
     unsigned long long* lines = lines_of(a);
     const long long nl = lines ? a.nrows * kLineWords : 0;
-    const long long start = (long long)*a.pool_ptr;
+    // the pool's first free word, as k_reduce_init sets it: apparent region
+    // (off[K] + K words), one line of padding, then the owner lines
+    unsigned long long pp =
+        ((unsigned long long)a.off[a.K] + a.K + kLine - 1) / kLine * kLine + kLine;
+    if (lines) pp += 2ull * kLineWords * (unsigned long long)a.nrows;
+    const long long start = (long long)pp;
     const long long nf = fill_end > start ? fill_end - start : 0;
     const long long n = max(max(nl, a.K), nf);
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -1422,11 +1428,13 @@ __global__ void k_queue_prep(RedArgs a, unsigned long long* __restrict__ ring,
         if (i < nf) a.pool[start + i] = kUnwritten;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int i = 0; i < 8; i++) a.status[i] = 0;  // k_reduce_init's reset
+        *a.pool_ptr = pp;
         qs[0] = 0;
         qs[1] = 0;
         qs[2] = 0;
         qs[3] = *res_count;
-        qs[4] = a.status[0] != 0;
+        qs[4] = 0;
         qs[5] = -1;
     }
 }
@@ -1440,6 +1448,10 @@ __global__ void k_final_reset(RedArgs a) {
 
 __global__ void k_final_own(RedArgs a) {
     long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r == 0) {  // k_reduce_fini's work: the pool words used, in lines
+        const unsigned long long lines = a.pool_ptr[0] / kLine;
+        a.status[5] = lines > 0x7FFFFFFFull ? 0x7FFFFFFF : (int)lines;
+    }
     if (r >= a.nrows) return;
     const unsigned long long v = a.own[r];
     if (v == kNoOwner) return;
@@ -3050,8 +3062,10 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
     const int from_csr = qwgs > 1 && bitmap;
     a.use_lines = from_csr;
     a.lines = nullptr;
-    k_reduce_init<<<1, 64, 0, s>>>(status, a.pool_ptr, col_off, K, a);
-    FPH_LAUNCH_CHECK();
+    if (!(from_csr && K > 0)) {  // else k_queue_prep does it
+        k_reduce_init<<<1, 64, 0, s>>>(status, a.pool_ptr, col_off, K, a);
+        FPH_LAUNCH_CHECK();
+    }
     if (K == 0) {
         k_reduce_fini<<<1, 64, 0, s>>>(status, a.pool_ptr);
         FPH_LAUNCH_CHECK();
@@ -3110,13 +3124,14 @@ extern "C" int fph_reduce(int64_t K, const int32_t* col_off, const uint32_t* col
         }
         FPH_LAUNCH_CHECK();
         if (!from_csr) k_final_reset<<<fph_blocks(K, 256), 256, 0, s>>>(a);
+        // also k_reduce_fini's work
         k_final_own<<<fph_blocks(std::max<int64_t>(nrows, 1), 256), 256, 0, s>>>(a);
         FPH_LAUNCH_CHECK();
-    } else {
-        k_reduce_queue<<<(unsigned)nseg, kPhaseBWaves * FPH_WAVE, lds_b, s>>>(
-            a, order, res_count, seg_cols, ring, limit_ticks);
-        FPH_LAUNCH_CHECK();
+        return FPH_OK;
     }
+    k_reduce_queue<<<(unsigned)nseg, kPhaseBWaves * FPH_WAVE, lds_b, s>>>(
+        a, order, res_count, seg_cols, ring, limit_ticks);
+    FPH_LAUNCH_CHECK();
     k_reduce_fini<<<1, 64, 0, s>>>(status, a.pool_ptr);
     FPH_LAUNCH_CHECK();
     return FPH_OK;

@@ -171,6 +171,13 @@ __global__ void k_fill_i32(int* __restrict__ p, long long n, int v) {
     if (i < n) p[i] = v;
 }
 
+// own[0..n) = 0 and *count = 0 in one launch.
+__global__ void k_zero_own_count(int* __restrict__ own, long long n, int* __restrict__ count) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) own[i] = 0;
+    if (i == 0) *count = 0;
+}
+
 __global__ void k_fill_u8(unsigned char* __restrict__ p, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -625,11 +632,9 @@ extern "C" int fph_apparent(int64_t K, int64_t col_base, const int32_t* col_off,
                             int32_t* res_count, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     if (K < 0 || nrows < 0) return FPH_ERR_ARG;
-    if (nrows > 0) {
-        k_fill_i32<<<fph_blocks(2 * nrows, 256), 256, 0, s>>>((int*)own, 2 * nrows, 0);
-        FPH_LAUNCH_CHECK();
-    }
-    k_fill_i32<<<1, 64, 0, s>>>(res_count, 1, 0);
+    // own (2 nrows words) and the residual count cleared together
+    k_zero_own_count<<<fph_blocks(std::max<int64_t>(2 * nrows, 1), 256), 256, 0, s>>>(
+        (int*)own, 2 * nrows, res_count);
     FPH_LAUNCH_CHECK();
     if (K == 0) return FPH_OK;
     const int tb = fph_tune_value(FPH_TUNE_APPARENT_BLOCK, K, 0);

@@ -534,9 +534,10 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
     pivot = _empty(torch.int32, device, max(K, 1))
     res_list = _empty(torch.int32, device, max(K, 1))
     # info32: [0..8) reduction status, [8] residual count, [12..12+nseg) rows,
-    # [16..24) replay status
-    info32 = torch.zeros(max(12 + nseg, 24 if replay_cap is not None else 0),
-                         dtype=torch.int32, device=device)
+    # [16..24) replay status; every entry read back is written by the
+    # entry points (their status resets, fph_apparent's count, fph_diagram's
+    # row counts, the replay's reset), so it needs no clearing here
+    info32 = _empty(torch.int32, device, max(12 + nseg, 24 if replay_cap is not None else 0))
     status_p = info32.data_ptr()
     rescnt_p = info32.data_ptr() + 32
     counts_p = info32.data_ptr() + 48
