@@ -15,6 +15,8 @@
 // so any simplex's rank is found with one binary search.
 #include "fph_common.h"
 
+#include <cstdlib>
+
 #include <rocprim/rocprim.hpp>
 
 namespace {
@@ -532,10 +534,51 @@ __global__ __launch_bounds__(256) void k_colsort(const int* __restrict__ off, lo
     }
 }
 
-void sort_columns(const unsigned* raw, unsigned* out, const int* off, long long K,
-                  hipStream_t s) {
-    const long long blocks = std::min<long long>((K + 3) / 4, 4096);
-    k_colsort<<<(unsigned)std::max<long long>(blocks, 1), 256, 0, s>>>(off, K, raw, out);
+// The comparison sort costs sum(L^2) over the columns, at least C^2 / K for C
+// values in K columns; past kColSortWork of that the library's segmented
+// radix sort (linear in C, five launches and a host round trip) is used.
+// The environment variable FLASH_PH_COLSORT_WORK replaces the bound (tests
+// force either sort with it).
+constexpr double kColSortWork = 1073741824.0;
+
+bool colsort_fits(long long C, long long K) {
+    const char* env = std::getenv("FLASH_PH_COLSORT_WORK");
+    const double work = env ? std::atof(env) : kColSortWork;
+    return K > 0 && (double)C * (double)C <= work * (double)K;
+}
+
+size_t segsort_bytes(long long C, long long K) {
+    if (colsort_fits(C, K)) return 0;
+    size_t tmp = 0;
+    (void)rocprim::segmented_radix_sort_keys(nullptr, tmp, (const unsigned*)nullptr,
+                                             (unsigned*)nullptr,
+                                             (unsigned)std::max<long long>(C, 1),
+                                             (unsigned)std::max<long long>(K, 1),
+                                             (const int*)nullptr, (const int*)nullptr, 0, 32,
+                                             (hipStream_t)0);
+    return align_up(tmp);
+}
+
+int bit_width(unsigned long long x) {
+    int b = 0;
+    while (x) { b++; x >>= 1; }
+    return b;
+}
+
+// Sorts the columns of raw into out: one launch of k_colsort, or the
+// segmented radix sort over end_bit bits (tmp: segsort_bytes(C, K) bytes).
+int sort_columns(const unsigned* raw, unsigned* out, const int* off, long long K, long long C,
+                 int end_bit, void* tmp, size_t tmp_bytes, hipStream_t s) {
+    if (colsort_fits(C, K)) {
+        const long long blocks = std::min<long long>((K + 3) / 4, 4096);
+        k_colsort<<<(unsigned)std::max<long long>(blocks, 1), 256, 0, s>>>(off, K, raw, out);
+        FPH_LAUNCH_CHECK();
+        return FPH_OK;
+    }
+    size_t t = tmp_bytes;
+    FPH_CHECK_HIP(rocprim::segmented_radix_sort_keys(tmp, t, raw, out, (unsigned)C, (unsigned)K,
+                                                     off, off + 1, 0, end_bit, s));
+    return FPH_OK;
 }
 
 
@@ -605,12 +648,13 @@ extern "C" int fph_cofacets_h1(const int32_t* edge_i, const int32_t* edge_j, int
     hipStream_t s = (hipStream_t)stream;
     if (E < 0 || C < 0 || !ws_bytes) return FPH_ERR_ARG;
     if (E + T >= 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
-    // the unsorted columns, then sorted into cof_data
-    const size_t need = align_up(4 * (size_t)std::max<int64_t>(C, 1));
+    // [sort temp | the unsorted columns], then sorted into cof_data
+    const size_t tmp = segsort_bytes(C, E);
+    const size_t need = tmp + align_up(4 * (size_t)std::max<int64_t>(C, 1));
     if (!ws) { *ws_bytes = need; return FPH_OK; }
     if (*ws_bytes < need) return FPH_ERR_ARG;
     if (C == 0) return FPH_OK;
-    unsigned* raw = (unsigned*)ws;
+    unsigned* raw = (unsigned*)((char*)ws + tmp);
     LinkOut o{};
     o.tri_off = tri_off;
     o.tri_w = const_cast<int*>(tri_w);  // read only in this mode
@@ -620,9 +664,8 @@ extern "C" int fph_cofacets_h1(const int32_t* edge_i, const int32_t* edge_j, int
     k_edge_link<kLinkCof><<<fph_blocks(E, nw), nw * FPH_WAVE, nw * kLinkStage * 4, s>>>(
         edge_i, edge_j, E, adj_ptr, adj_idx, adj_rank, mst_mask, o);
     FPH_LAUNCH_CHECK();
-    sort_columns(raw, (unsigned*)cof_data, cof_off, E, s);
-    FPH_LAUNCH_CHECK();
-    return FPH_OK;
+    const int end_bit = std::max(1, bit_width((unsigned long long)(E + T)));
+    return sort_columns(raw, (unsigned*)cof_data, cof_off, E, C, end_bit, ws, tmp, s);
 }
 
 extern "C" int fph_apparent(int64_t K, int64_t col_base, const int32_t* col_off,
@@ -730,18 +773,18 @@ extern "C" int fph_cofacets_h2(int64_t T, int64_t E, int64_t Q, const int32_t* e
     hipStream_t s = (hipStream_t)stream;
     if (T < 0 || C < 0 || !ws_bytes) return FPH_ERR_ARG;
     if (E + T + Q >= 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
-    // the unsorted columns, then sorted into cof_data
-    const size_t need = align_up(4 * (size_t)std::max<int64_t>(C, 1));
+    // [sort temp | the unsorted columns], then sorted into cof_data
+    const size_t tmp = segsort_bytes(C, T);
+    const size_t need = tmp + align_up(4 * (size_t)std::max<int64_t>(C, 1));
     if (!ws) { *ws_bytes = need; return FPH_OK; }
     if (*ws_bytes < need) return FPH_ERR_ARG;
     if (C == 0) return FPH_OK;
-    unsigned* raw = (unsigned*)ws;
+    unsigned* raw = (unsigned*)((char*)ws + tmp);
     const int nw = link_waves(T);
     k_tri_link<1><<<fph_blocks(T, nw), nw * FPH_WAVE, nw * kLinkStage * 4, s>>>(
         T, E, edge_i, edge_j, adj_ptr, adj_idx, adj_rank, tri_w, tri_e, cleared, tet_off,
         (const unsigned long long*)tet_key, nullptr, cof_off, raw);
     FPH_LAUNCH_CHECK();
-    sort_columns(raw, (unsigned*)cof_data, cof_off, T, s);
-    FPH_LAUNCH_CHECK();
-    return FPH_OK;
+    const int end_bit = std::max(1, bit_width((unsigned long long)(E + T + Q)));
+    return sort_columns(raw, (unsigned*)cof_data, cof_off, T, C, end_bit, ws, tmp, s);
 }
