@@ -413,7 +413,22 @@ def main():
     ap.add_argument("--profile-mfma", action="store_true",
                     help="only run the n=200K d=50 t=6.0 edge stage three times with the "
                          "split-bf16 screen (for a kernel trace)")
+    ap.add_argument("--profile-plain", action="store_true",
+                    help="only run the n=10K d=10 t=1.6 H0-H2 call: 5 untimed, then 20 "
+                         "(default launch parameters; for a kernel trace)")
     args = ap.parse_args()
+    if args.profile_plain:
+        import torch
+        from flash_ph import rips_persistence
+        pts = torch.from_numpy(make_points()).cuda()
+        for _ in range(5):
+            rips_persistence(pts, T_EDGE, max_dim=2)
+        torch.cuda.synchronize()
+        for _ in range(20):
+            dg = rips_persistence(pts, T_EDGE, max_dim=2)
+        torch.cuda.synchronize()
+        print(f"n=10K H0-H2 x25, rows {[d.shape[0] for d in dg]}", flush=True)
+        return
     if args.profile_mfma:
         import numpy as np
         import torch
