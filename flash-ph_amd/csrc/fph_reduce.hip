@@ -108,6 +108,10 @@ __device__ __forceinline__ void count_merged(const RedArgs& a, long long n) {
     atomicAdd(reinterpret_cast<unsigned long long*>(a.status + 6), (unsigned long long)n);
 }
 
+// Waits until this wave's vector memory operations (loads, stores and
+// atomics, with or without a returned value) have completed.
+__device__ __forceinline__ void vm_drain() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -2046,6 +2050,415 @@ __device__ __forceinline__ void serial_bitmap(RedArgs& a, SerShared& sh, const u
     }
 }
 
+// ---- working column over an LDS window of a global bitmap -------------
+//
+// For segments whose bitmap does not fit LDS.  The rows of ww bitmap words
+// starting at the word of the column's first pivot (the window) are kept in
+// an LDS bitmap with its summary, as the LDS form keeps all rows; a row past
+// the window is flipped in the segment's global bitmap by an atomic whose
+// result is not waited for, and its word is marked in an LDS bitmap of
+// touched global words.  The pivot only increases and every added column
+// starts at the pivot, so no row below the window is ever flipped.  An
+// addition thus waits on two global reads (the owner word, then the owner
+// column) instead of four (also the flips' old values and the pivot's word).
+// When the window holds no set row, the workgroup waits for its global
+// flips, finds the first nonzero marked word and moves the window there;
+// none left means the column is zero.  The column length is kept as an
+// upper bound (rows at the start + rows added); when that passes MAX_COL the
+// set rows are counted.  Same additions in the same order as the list form,
+// so the same results.
+//
+// Marks exist only for words past the window (a window's range is copied
+// into LDS and unmarked when it moves there); all are cleared, with their
+// global words zeroed, when the column ends.
+constexpr int kWinMinWords = 1024;  // smallest window worth this form
+
+struct Win {
+    unsigned* ts;    // marks: bit g <=> global word g may be nonzero
+    unsigned* wsum;  // window summary: bit j <=> wb[j] != 0
+    unsigned* wb;    // window bitmap: word j holds rows 32 (wlo + j) ..
+    unsigned* gbits; // the segment's global bitmap (gw words)
+    int gw;          // global words
+    int tw;          // mark words
+    int ww;          // window words (a multiple of 1024)
+    int wlo;         // first global word of the window
+};
+
+__device__ __forceinline__ unsigned ld_agent(const unsigned* p) {
+    return __hip_atomic_load(const_cast<unsigned*>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Flips local row x (32 wlo <= x < rows).
+__device__ __forceinline__ void win_flip(const Win& W, int x) {
+    const int g = x >> 5;
+    const unsigned m = 1u << (x & 31);
+    const int j = g - W.wlo;
+    if (j < W.ww) {
+        const unsigned old = atomicXor(&W.wb[j], m);
+        if (old == 0u || old == m) atomicXor(&W.wsum[j >> 5], 1u << (j & 31));
+    } else {
+        (void)__hip_atomic_fetch_xor(&W.gbits[g], m, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+        atomicOr(&W.ts[g >> 5], 1u << (g & 31));
+    }
+}
+
+// Up to 8 marked words of mark word tw (bits m, taken from the lowest and
+// removed): their indices to g[], values to v[], all loads in flight at
+// once.  Returns how many.
+__device__ __forceinline__ int win_load8(const Win& W, int tw, unsigned& m, int* g, unsigned* v) {
+    int k = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        if (m) {
+            g[q] = tw * 32 + __builtin_ctz(m);
+            m &= m - 1;
+            v[q] = ld_agent(&W.gbits[g[q]]);
+            k = q + 1;
+        }
+    }
+    return k;
+}
+
+// Workgroup OR of `bad` and maximum of `hi`.  Barriers; uniform results.
+__device__ void win_reduce(SerShared& sh, int bad, int hi, int* nbad, int* hmax) {
+    const int wid = threadIdx.x / FPH_WAVE;
+    const int b = __ballot(bad != 0) != 0ull;
+    for (int o = FPH_WAVE / 2; o; o >>= 1) hi = max(hi, __shfl_xor(hi, o));
+    if (fph_lane() == 0) {
+        sh.red[0][wid][1] = b;
+        sh.red[0][wid][2] = hi;
+    }
+    __syncthreads();
+    int nb = 0, h = 0;
+    for (int w = 0; w < kSerWaves; w++) {
+        nb |= sh.red[0][w][1];
+        h = max(h, sh.red[0][w][2]);
+    }
+    __syncthreads();
+    *nbad = nb;
+    *hmax = h;
+}
+
+// Lowest set row (local) of the window at or above window word j0, -1 if
+// none, -2 if the summary and the bitmap disagree; found by the calling
+// wavefront alone (every wavefront finds the same while no flips run).
+__device__ __forceinline__ int win_search(const Win& W, int j0) {
+    const int lane = fph_lane();
+    const int S = W.ww >> 5;
+    for (int i0 = j0 >> 5; i0 < S; i0 += FPH_WAVE) {
+        const int i = i0 + lane;
+        unsigned val = i < S ? W.wsum[i] : 0u;
+        if (i == (j0 >> 5)) val &= ~0u << (j0 & 31);
+        const unsigned long long hit = __ballot(val != 0u);
+        if (hit) {
+            const int l = __ffsll((long long)hit) - 1;
+            const int j = __builtin_amdgcn_readlane(i * 32 + (val ? __builtin_ctz(val) : 0), l);
+            const unsigned bw = W.wb[j];
+            return bw ? (W.wlo + j) * 32 + __builtin_ctz(bw) : -2;
+        }
+    }
+    return -1;
+}
+
+// The first nonzero marked global word at or past g0 and at most gmax, or
+// -1; marks of the zero words passed on the way are cleared.  After the
+// workgroup's global flips have completed.  Barriers; uniform result.
+__device__ int win_next_word(Win& W, SerShared& sh, int g0, int gmax) {
+    const int t = threadIdx.x, wid = t / FPH_WAVE;
+    const int te = min(W.tw - 1, gmax >> 5);
+    for (int tb = g0 >> 5; tb <= te; tb += kSerThreads) {
+        const int tw = tb + t;
+        unsigned bits = tw <= te ? W.ts[tw] : 0u;
+        if (tw == (g0 >> 5)) bits &= ~0u << (g0 & 31);
+        int first = 0x7FFFFFFF;
+        unsigned zero = 0u;
+        for (unsigned m = bits; m; m &= m - 1) {
+            const int g = tw * 32 + __builtin_ctz(m);
+            if (ld_agent(&W.gbits[g])) {
+                first = g;
+                break;
+            }
+            zero |= m & (~m + 1u);
+        }
+        if (zero) W.ts[tw] &= ~zero;
+        for (int o = FPH_WAVE / 2; o; o >>= 1) first = min(first, __shfl_xor(first, o));
+        if (fph_lane() == 0) sh.wave_sum[wid] = first;
+        __syncthreads();
+        int all = sh.wave_sum[0];
+        for (int w = 1; w < kSerWaves; w++) all = min(all, sh.wave_sum[w]);
+        __syncthreads();
+        if (all != 0x7FFFFFFF) return all;
+    }
+    return -1;
+}
+
+// Moves the (empty) window to start at global word gn: the marked words of
+// [gn, gn + ww) are copied into it, zeroed and unmarked.  Barriers.
+__device__ void win_move(Win& W, int gn) {
+    const int t = threadIdx.x;
+    W.wlo = gn;
+    const int ge = min(gn + W.ww, W.gw);
+    for (int g = gn + t; g < ge; g += kSerThreads) {
+        if ((W.ts[g >> 5] >> (g & 31)) & 1u) {
+            const unsigned v = ld_agent(&W.gbits[g]);
+            if (v) {
+                const int j = g - gn;
+                W.wb[j] = v;
+                atomicOr(&W.wsum[j >> 5], 1u << (j & 31));
+                __hip_atomic_store(&W.gbits[g], 0u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+    }
+    __syncthreads();  // every mark read before any is cleared
+    for (int tw = (gn >> 5) + t; tw <= (ge - 1) >> 5; tw += kSerThreads) {
+        const int lo = max(gn - tw * 32, 0), hi = min(ge - tw * 32, 32);  // bits [lo, hi)
+        const unsigned m = (hi == 32 ? ~0u : (1u << hi) - 1u) & (~0u << lo);
+        W.ts[tw] &= ~m;
+    }
+    vm_drain();
+    __syncthreads();
+}
+
+// Set rows of window summary words [s0, s1] and marked global words of mark
+// words [m0, m1].  After the global flips have completed.  Barriers.
+__device__ int win_count(const Win& W, SerShared& sh, int s0, int s1, int m0, int m1) {
+    const int t = threadIdx.x;
+    int cnt = 0;
+    for (int sw = s0 + t; sw <= s1; sw += kSerThreads)
+        for (unsigned m = W.wsum[sw]; m; m &= m - 1) cnt += __popc(W.wb[sw * 32 + __builtin_ctz(m)]);
+    for (int tw = m0 + t; tw <= m1; tw += kSerThreads) {
+        unsigned m = W.ts[tw];
+        while (m) {
+            int g[8];
+            unsigned v[8];
+            const int k = win_load8(W, tw, m, g, v);
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (q < k) cnt += __popc(v[q]);
+        }
+    }
+    int tot;
+    (void)block_excl_scan(cnt, sh, &tot);
+    return tot;
+}
+
+// Writes the column's rows in ascending order to dst (unless null) and
+// clears it: window summary words [s0, s1] and marked global words of mark
+// words [m0, m1] (zeroed, unmarked).  Returns the rows found.  After the
+// global flips have completed.  Barriers.
+__device__ int win_drain(Win& W, SerShared& sh, int s0, int s1, int m0, int m1, unsigned* dst,
+                         long long row_lo) {
+    const int t = threadIdx.x;
+    int written = 0;
+    for (int sb = s0; sb <= s1; sb += kSerThreads) {
+        const int sw = sb + t;
+        const unsigned val = sw <= s1 ? W.wsum[sw] : 0u;
+        int cnt = 0;
+        for (unsigned m = val; m; m &= m - 1) cnt += __popc(W.wb[sw * 32 + __builtin_ctz(m)]);
+        int tot;
+        int at = written + block_excl_scan(cnt, sh, &tot);
+        for (unsigned m = val; m; m &= m - 1) {
+            const int j = sw * 32 + __builtin_ctz(m);
+            unsigned bw = W.wb[j];
+            if (dst)
+                for (; bw; bw &= bw - 1)
+                    dst[at++] = (unsigned)(row_lo + (long long)(W.wlo + j) * 32 + __builtin_ctz(bw));
+            W.wb[j] = 0u;
+        }
+        if (val) W.wsum[sw] = 0u;
+        written += tot;
+    }
+    for (int tb = m0; tb <= m1; tb += kSerThreads) {
+        const int tw = tb + t;
+        const unsigned bits = tw <= m1 ? W.ts[tw] : 0u;
+        int cnt = 0;
+        for (unsigned m = bits; m;) {
+            int g[8];
+            unsigned v[8];
+            const int k = win_load8(W, tw, m, g, v);
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (q < k) cnt += __popc(v[q]);
+        }
+        int tot;
+        int at = written + block_excl_scan(cnt, sh, &tot);
+        for (unsigned m = bits; m;) {
+            int g[8];
+            unsigned v[8];
+            const int k = win_load8(W, tw, m, g, v);
+            for (int q = 0; q < k; q++) {
+                if (dst)
+                    for (unsigned bw = v[q]; bw; bw &= bw - 1)
+                        dst[at++] = (unsigned)(row_lo + (long long)g[q] * 32 + __builtin_ctz(bw));
+                __hip_atomic_store(&W.gbits[g[q]], 0u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+        if (bits) W.ts[tw] = 0u;
+        written += tot;
+    }
+    vm_drain();
+    __syncthreads();
+    return written;
+}
+
+__device__ void serial_window(RedArgs& a, SerShared& sh, const unsigned* __restrict__ order,
+                              int lo, int lo2, long long row_lo, int rows, unsigned* gbits,
+                              unsigned* smem, int ww, long long base, long long cap,
+                              long long& used, long long& stored, int& nxor, int& longest,
+                              int maxcol) {
+    const int t = threadIdx.x;
+    Win W;
+    W.gw = (rows + 31) / 32;
+    W.tw = (W.gw + 31) / 32;
+    W.ww = ww;
+    W.ts = smem;
+    W.wsum = smem + W.tw;
+    W.wb = smem + W.tw + (W.ww >> 5);
+    W.gbits = gbits;
+    W.wlo = 0;
+    long long merged = 0;
+    for (int r = lo; r < lo2; r++) {
+        const int c = (int)order[r];
+        const int lo_c = a.off[c];
+        const int n0 = min(a.off[c + 1] - lo_c, maxcol);
+        const long long p0 = (long long)a.data[lo_c] - row_lo;
+        if (p0 < 0 || p0 >= rows) {  // uniform
+            if (t == 0) atomicExch(&a.status[0], 3);
+            return;
+        }
+        W.wlo = (int)(p0 >> 5);
+        int bad = 0, hi = (int)p0;  // per thread: a row outside, highest row flipped
+        for (int e = t; e < n0; e += kSerThreads) {
+            const long long x = (long long)a.data[lo_c + e] - row_lo;
+            if (x < 32LL * W.wlo || x >= rows) bad = 1;
+            else {
+                win_flip(W, (int)x);
+                hi = max(hi, (int)x);
+            }
+        }
+        merged += n0;
+        long long ub = n0;  // >= the column's length
+        int p = (int)p0;
+        bool zero = false;
+        // Per addition: the owner read, a barrier (every wavefront's search
+        // of the previous pivot is done; it waits while the read is in
+        // flight), the flips, a barrier (the flips are visible), and the
+        // search for the new pivot by every wavefront on its own.
+        while (true) {
+            const unsigned long long v = a.own[row_lo + p - a.row_base];
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            if (v == kNoOwner) break;
+            const unsigned long long vo = v & 0xFFFFFFFFull;
+            const unsigned* ob = a.pool + vo;
+            // the length and each thread's first row in one round trip (a
+            // row past the column's end is a pool word, then ignored)
+            const int nb = (int)ob[0];
+            const unsigned r0 = vo + 1 + t < a.pool_cap ? ob[1 + t] : 0u;
+            for (int b = 0; b < nb; b += kSerThreads) {
+                const int e = b + t;
+                if (e < nb) {
+                    const long long x = (long long)(b == 0 ? r0 : ob[1 + e]) - row_lo;
+                    if (x < 32LL * W.wlo || x >= rows) bad = 1;
+                    else {
+                        win_flip(W, (int)x);
+                        hi = max(hi, (int)x);
+                    }
+                }
+            }
+            merged += nb;
+            nxor++;
+            ub += nb;
+            if (ub > maxcol) {  // uniform: count the rows
+                vm_drain();
+                __syncthreads();
+                int nbad, hmax;
+                win_reduce(sh, bad, hi, &nbad, &hmax);
+                const int n = win_count(W, sh, 0, (W.ww >> 5) - 1, (W.wlo + W.ww) >> 5,
+                                        min(W.tw - 1, hmax >> 10));
+                if (n > maxcol) {  // the reference would truncate: list form
+                    if (t == 0) atomicExch(&a.status[0], 4);
+                    return;
+                }
+                ub = n;
+            }
+            __syncthreads();  // the flips are visible
+            int np = win_search(W, (p >> 5) - W.wlo);
+            if (np == -1) {  // the window is empty: move it to the next set row
+                vm_drain();
+                __syncthreads();
+                int nbad, hmax;
+                win_reduce(sh, bad, hi, &nbad, &hmax);
+                const int gn = win_next_word(W, sh, W.wlo + W.ww, hmax >> 5);
+                if (gn < 0) {
+                    zero = true;
+                    break;
+                }
+                win_move(W, gn);
+                const unsigned w0 = W.wb[0];  // gn's word, nonzero
+                np = w0 ? gn * 32 + __builtin_ctz(w0) : -1;
+            }
+            if (!(np > p)) {  // pivot must increase
+                if (t == 0) atomicExch(&a.status[0], 3);
+                return;
+            }
+            p = np;
+        }
+        vm_drain();
+        int nbad, hmax;
+        win_reduce(sh, bad, hi, &nbad, &hmax);  // also the barrier after the flips
+        if (nbad) {  // a row outside the segment
+            if (t == 0) atomicExch(&a.status[0], 3);
+            return;
+        }
+        if (zero) {  // window empty, every marked word zero and unmarked
+            if (t == 0) a.pivot[c] = -1;
+            continue;
+        }
+        if (t == 0) a.pivot[c] = (int)(row_lo + p);
+        const int s0 = ((p >> 5) - W.wlo) >> 5;
+        const int s1 = min((W.ww >> 5) - 1, ((hmax >> 5) - W.wlo) >> 5);
+        const int m0 = (W.wlo + W.ww) >> 5, m1 = min(W.tw - 1, hmax >> 10);
+        int len, written;
+        bool keep;
+        long long at;
+        if (base >= 0 && used + ub <= cap) {
+            // the segment's store takes the column whatever its length: read
+            // it out once, straight into place
+            keep = true;
+            at = base + stored;
+            len = written = win_drain(W, sh, s0, s1, m0, m1, a.pool + at + 1, row_lo);
+        } else {
+            len = win_count(W, sh, s0, s1, m0, m1);
+            keep = used + len <= cap;
+            at = keep ? serial_store_at(a, sh, base, stored, len) : 0;
+            if (at < 0) return;
+            written = win_drain(W, sh, s0, s1, m0, m1, keep ? a.pool + at + 1 : nullptr, row_lo);
+        }
+        longest = max(longest, len);
+        if (keep) {
+            if (t == 0) a.pool[at] = (unsigned)len;
+            __threadfence_block();
+            __syncthreads();
+            if (t == 0)
+                a.own[row_lo + p - a.row_base] =
+                    ((unsigned long long)(c + 1) << 32) | (unsigned long long)at;
+            used += len;
+            stored += len + 1;
+        }
+        if (written != len && t == 0) atomicExch(&a.status[0], 3);
+        __threadfence_block();
+        __syncthreads();
+        if (written != len) return;
+    }
+    if (t == 0 && merged) count_merged(a, merged);
+}
+
 // The reference's pool prefill (_numba_utils.py:68-104, reduce_h1_gpu.py:
 // 282-289): apparent columns of [c0, c1) in ascending column order enter the
 // store while it has room (cap reference entries); a column that does not
@@ -2138,17 +2551,24 @@ __global__ __launch_bounds__(kSerThreads) void k_reduce_serial(
     // word offset of a zeroed global bitmap in the pool, or -1 for LDS
     const int rows = (int)sp[3];
     // variant bit 0: sorted-list working column; bit 1: the complete
-    // reduction (no store capacity, no MAX_COL truncation)
+    // reduction (no store capacity, no MAX_COL truncation); bit 3: a global
+    // bitmap without the LDS window; bits 8..15: at most that many 1024-word
+    // units of window (0: as many as LDS holds)
     const int maxcol = (variant & 2) ? 0x7FFFFFFF : maxcol_ref;
     if ((variant & 1) == 0 && rows > 0) {
         const int W = (rows + 31) / 32, S = (W + 31) / 32;
-        for (int i = t; i < S; i += kSerThreads) smem[i] = 0u;
-        if (sp[4] < 0)
-            for (int i = t; i < W; i += kSerThreads) smem[S + i] = 0u;
+        int ww = ((kBmLdsWords - S) * 32 / 33) & ~1023;  // window words beside the marks
+        if ((variant >> 8) & 0xFF) ww = min(ww, ((variant >> 8) & 0xFF) * 1024);
+        const bool window = sp[4] >= 0 && (variant & 8) == 0 && ww >= kWinMinWords;
+        const int zw = window ? kBmLdsWords : S + (sp[4] < 0 ? W : 0);
+        for (int i = t; i < zw; i += kSerThreads) smem[i] = 0u;
         __syncthreads();
         if (sp[4] < 0)
             serial_bitmap<false>(a, sh, order, lo, lo2, sp[2], rows, smem + S, smem, base, cap,
                                  used, stored, nxor, longest, maxcol);
+        else if (window)
+            serial_window(a, sh, order, lo, lo2, sp[2], rows, a.pool + sp[4], smem, ww, base,
+                          cap, used, stored, nxor, longest, maxcol);
         else
             serial_bitmap<true>(a, sh, order, lo, lo2, sp[2], rows, a.pool + sp[4], smem, base,
                                 cap, used, stored, nxor, longest, maxcol);
@@ -2407,8 +2827,6 @@ struct PipeArgs {
     int inflight;                   // columns of a segment in flight at most, while
                                     // unopened segments remain
 };
-
-__device__ __forceinline__ void vm_drain() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 __device__ __forceinline__ unsigned long long ld_coh64(const unsigned long long* p) {
     return __hip_atomic_load(const_cast<unsigned long long*>(p), __ATOMIC_RELAXED,

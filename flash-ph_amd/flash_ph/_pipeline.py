@@ -66,6 +66,17 @@ EXACT_SERIAL_MIN_ROWS = 100_000
 # Working column of the serial reducer: 0 = bitmap (list form only where a
 # column would pass the reference's MAX_COL), 1 = sorted list.
 SERIAL_VARIANT = 0
+# A global bitmap of the serial reducer keeps a window of rows in LDS (False:
+# every flip and pivot search in global memory).  Tests: SERIAL_WINDOW_UNITS
+# caps the window at that many 1024-word units (0: as many as LDS holds) so
+# that small clouds move it.
+SERIAL_WINDOW = True
+SERIAL_WINDOW_UNITS = 0
+
+
+def _serial_variant(serial_exact: bool) -> int:
+    return (SERIAL_VARIANT | (2 if serial_exact else 0) | (0 if SERIAL_WINDOW else 8)
+            | (min(max(int(SERIAL_WINDOW_UNITS), 0), 255) << 8))
 # The reference-semantics reduction with many columns in flight
 # (fph_reduce_pipe); False: one workgroup per segment (fph_reduce_serial).
 # The sorted-list form (a column past MAX_COL) is always the serial one.
@@ -575,7 +586,7 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
         pool_words = pool_start + max(_POOL_MIN_WORDS, nseg * 262144)
         if serial_exact:
             pool_words += 64 * K + 4 * C
-        variant = SERIAL_VARIANT | (2 if serial_exact else 0)
+        variant = _serial_variant(serial_exact)
     out = _empty(torch.float32, device, max(K, 1), 2)
     out_edges = _empty(torch.int32, device, max(K, 1), 2) if want_edges else None
     state = {"pool_words": pool_words, "variant": variant if ref_sizes is not None else 0}

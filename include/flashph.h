@@ -380,7 +380,10 @@ int fph_reduce_replay(int64_t K, const int32_t *col_off, const uint32_t *col_dat
  * seg_pool[0] = -1, capacity ignored): the complete reduction instead --
  * every column stored (bump-allocated in the pool), no truncation; the
  * pairs of fph_reduce, in the reference's serial order (faster than
- * fph_reduce on dense complexes with long columns).  max_col: rows kept by
+ * fph_reduce on dense complexes with long columns).  A bitmap in `pool`
+ * has a window of rows in LDS (the rows from the current pivot on, about
+ * 9.5e5 of them); bit 3 turns the window off, bits 8..15 cap it at that
+ * many units of 32768 rows (tests; 0 = as many as LDS holds).  max_col: rows kept by
  * a column addition in the reference semantics, 0 = the reference's
  * MAX_COL of 1048576 (a smaller value reproduces the truncation branch on
  * small complexes, for tests).  status as for fph_reduce.

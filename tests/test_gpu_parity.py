@@ -227,6 +227,60 @@ def test_serial_global_bitmap():
         _pipeline._BITMAP_LDS_WORDS, _pipeline.REPLAY = old
 
 
+@pytest.mark.parametrize("units", [1, 0, -1], ids=["window32k", "window", "nowindow"])
+def test_serial_window(units):
+    """The serial reducer's global bitmap with a window of rows in LDS: a
+    32,768-row window that moves several times per column on clouds of
+    50K-72K rows, the largest window, and no window.  Reference semantics
+    (with and without store overflow, one cloud and a batch), a lowered
+    MAX_COL (the length bound's recount and the list-form handover) and the
+    complete reduction in serial order, each against the oracle."""
+    import oracle
+    from flash_ph import _pipeline, rips_persistence_batch
+    old = (_pipeline._BITMAP_LDS_WORDS, _pipeline.REPLAY, _pipeline.SERIAL_WINDOW,
+           _pipeline.SERIAL_WINDOW_UNITS, _pipeline.REF_MAX_COL, _pipeline.EXACT_SERIAL_MIN_ROWS)
+    _pipeline._BITMAP_LDS_WORDS = 64
+    _pipeline.REPLAY = False
+    _pipeline.SERIAL_WINDOW = units >= 0
+    _pipeline.SERIAL_WINDOW_UNITS = max(units, 0)
+    try:
+        clouds = [(_store_overflow_clouds()[0], None), (cases.randn(400, 3, 2), 1.0),
+                  (cases.clifford(600, 3, 0.02), 0.6)]
+        refs = []
+        for x, t in clouds:
+            t = oracle.auto_threshold(x) if t is None else t
+            print(f"[gpu] serial window units={units} n={x.shape[0]} d={x.shape[1]}", flush=True)
+            ref = oracle.run(x, t, 1)
+            refs.append((t, ref))
+            dg, cx = _pipeline.compute(torch.from_numpy(x).cuda(), t, 1, semantics="reference")
+            _eq("h1", _cpu(dg[1]), ref["h1"])
+            assert cx.info["h1_stats"]["xors"] == ref["stats"]["xor1"]
+        print(f"[gpu] serial window units={units} batch", flush=True)
+        got = rips_persistence_batch([torch.from_numpy(x).cuda() for x, _ in clouds],
+                                     [t for t, _ in refs], max_dim=1)
+        for (t, ref), g in zip(refs, got):
+            _eq("batch h1", _cpu(g[1]), ref["h1"])
+        x, t, max_col = cases.randn(700, 3, 3), 0.9, 24
+        print(f"[gpu] serial window units={units} max_col={max_col}", flush=True)
+        _pipeline.REF_MAX_COL = max_col
+        ref = oracle.run(x, t, 1, max_col=max_col)
+        assert ref["stats"]["truncations"] > 0
+        dg, _ = _pipeline.compute(torch.from_numpy(x).cuda(), t, 1, semantics="reference")
+        _eq("h1 truncated", _cpu(dg[1]), ref["h1"])
+        _pipeline.REF_MAX_COL = old[4]
+        print(f"[gpu] serial window units={units} complete reduction", flush=True)
+        _pipeline.EXACT_SERIAL_MIN_ROWS = 0
+        x = clouds[0][0]
+        t = refs[0][0]
+        ex = oracle.run(x, t, 1, exact=True)
+        dg, _ = _pipeline.compute(torch.from_numpy(x).cuda(), t, 1, semantics="exact")
+        _eq("h1 exact", _cpu(dg[1]), ex["h1"])
+    finally:
+        (_pipeline._BITMAP_LDS_WORDS, _pipeline.REPLAY, _pipeline.SERIAL_WINDOW,
+         _pipeline.SERIAL_WINDOW_UNITS, _pipeline.REF_MAX_COL,
+         _pipeline.EXACT_SERIAL_MIN_ROWS) = old
+
+
 @pytest.mark.parametrize("variant", [0, 1])
 def test_serial_reducer_variants(variant):
     """Both forms of the working column (bitmap, sorted list) of the
