@@ -11,7 +11,10 @@
 // where class(birth) is the first edge index carrying the same FP32 birth
 // value (edge values are non-decreasing in rank), so one radix sort yields
 // exactly that row order.  With no triangle at all the reference lists the
-// essential columns in ascending rank (reduce_h1_gpu.py:495-502).
+// essential columns in ascending rank (reduce_h1_gpu.py:495-502).  In a
+// batch, class(birth) is a global edge index and each cloud's edges form one
+// range, so one radix sort of every key orders the clouds as well; each
+// row then goes to its cloud's block of the output.
 #include "fph_common.h"
 
 #include <rocprim/rocprim.hpp>
@@ -90,20 +93,58 @@ __global__ void k_diag_keys(long long K, const unsigned char* __restrict__ kind,
     }
 }
 
+// pre[s] = rows of the clouds before s; pre[nseg] = all rows.  One workgroup.
+__global__ void k_seg_prefix(const int* __restrict__ counts, int nseg, int* __restrict__ pre) {
+    __shared__ int wsum[4];
+    const int t = threadIdx.x, lane = fph_lane(), wid = t / FPH_WAVE;
+    int carry = 0;
+    for (int b = 0; b <= nseg; b += 256) {
+        const int i = b + t;
+        const int v = i < nseg ? counts[i] : 0;
+        int incl = v;
+        for (int o = 1; o < FPH_WAVE; o <<= 1) {
+            const int y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        if (lane == FPH_WAVE - 1) wsum[wid] = incl;
+        __syncthreads();
+        int before = carry, all = carry;
+        for (int w = 0; w < 4; w++) {
+            if (w < wid) before += wsum[w];
+            all += wsum[w];
+        }
+        if (i <= nseg) pre[i] = before + incl - v;
+        __syncthreads();
+        carry = all;
+    }
+}
+
+// Row k of the sorted keys.  One cloud: row k of the output.  A batch (pre
+// not null): the cloud s is read off the key's birth class (an edge of
+// cloud s, edges [seg_edges[s], seg_edges[s + 1])), and the row goes to
+// seg_cols[s] + (k - pre[s]).
 __global__ void k_diag_write(const unsigned long long* __restrict__ keys, long long K,
-                             const int* __restrict__ counts, const int* __restrict__ pivot,
+                             const int* __restrict__ counts, const int* __restrict__ pre,
+                             const int* __restrict__ pivot,
                              const int* __restrict__ col_edge, long long row_base,
                              const int* __restrict__ row_edge, const float* __restrict__ edge_filt,
                              int ess_asc_all, const int* __restrict__ seg_cols,
+                             const int* __restrict__ seg_edges,
                              const unsigned char* __restrict__ seg_ess_asc, int nseg,
                              float* __restrict__ out, int* __restrict__ out_edges) {
-    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= K) return;
-    const int s = segment_of(seg_cols, nseg, k);
-    const long long k0 = seg_cols ? seg_cols[s] : 0;
-    if (k - k0 >= counts[s]) return;
+    const long long kk = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (kk >= K) return;
+    const unsigned long long key = keys[kk];
+    int s = 0;
+    long long k = kk;
+    if (pre) {
+        if (kk >= pre[nseg]) return;
+        s = segment_of(seg_edges, nseg, (long long)(key >> 33));
+        k = seg_cols[s] + (kk - pre[s]);
+    } else if (kk >= counts[0]) {
+        return;
+    }
     const int asc = seg_ess_asc ? seg_ess_asc[s] : ess_asc_all;
-    const unsigned long long key = keys[k];
     const int group = (int)((key >> 31) & 3);
     const unsigned ord = (unsigned)(key & 0x7FFFFFFFull);
     const long long c = (group == 0 || (group == 2 && asc)) ? (long long)ord : 0x7FFFFFFFLL - ord;
@@ -136,19 +177,19 @@ extern "C" int fph_diagram(int64_t K, const uint8_t* kind, const int32_t* col_of
     if (K < 0 || !ws_bytes || !counts || nseg < 1) return FPH_ERR_ARG;
     if (nseg > 1 && !seg_cols) return FPH_ERR_ARG;
     if (K >= 0x7FFFFFFFLL) return FPH_ERR_LIMIT;
+    if (nseg > 1 && !seg_edges) return FPH_ERR_ARG;
     const size_t k = (size_t)std::max<int64_t>(K, 1);
+    // keys below E << 33, the unused ones all ones: the low 33 + bits(E)
+    // bits order them
+    int end_bit = 33;
+    for (long long e = E; e > 0; e >>= 1) end_bit++;
+    end_bit = std::min(end_bit, 64);
     size_t tmp = 0;
-    if (seg_cols) {
-        FPH_CHECK_HIP(rocprim::segmented_radix_sort_keys(
-            nullptr, tmp, (const unsigned long long*)nullptr, (unsigned long long*)nullptr,
-            (unsigned)k, (unsigned)nseg, (const int*)nullptr, (const int*)nullptr, 0, 64, s));
-    } else {
-        FPH_CHECK_HIP(rocprim::radix_sort_keys(nullptr, tmp, (const unsigned long long*)nullptr,
-                                               (unsigned long long*)nullptr, (unsigned)k, 0, 64,
-                                               s));
-    }
+    FPH_CHECK_HIP(rocprim::radix_sort_keys(nullptr, tmp, (const unsigned long long*)nullptr,
+                                           (unsigned long long*)nullptr, (unsigned)k, 0, end_bit,
+                                           s));
     tmp = align_up(tmp);
-    const size_t need = tmp + 2 * align_up(8 * k);
+    const size_t need = tmp + 2 * align_up(8 * k) + align_up(4 * ((size_t)nseg + 1));
     if (!ws) { *ws_bytes = need; return FPH_OK; }
     if (*ws_bytes < need) return FPH_ERR_ARG;
     k_zero_counts<<<fph_blocks(nseg, 256), 256, 0, s>>>(counts, (int)nseg);
@@ -163,16 +204,17 @@ extern "C" int fph_diagram(int64_t K, const uint8_t* kind, const int32_t* col_of
                                                   seg_ess_asc, (int)nseg, keys, counts);
     FPH_LAUNCH_CHECK();
     size_t t = tmp;
+    FPH_CHECK_HIP(rocprim::radix_sort_keys(ws, t, keys, sorted, (unsigned)K, 0, end_bit, s));
+    int* pre = nullptr;
     if (seg_cols) {
-        FPH_CHECK_HIP(rocprim::segmented_radix_sort_keys(ws, t, keys, sorted, (unsigned)K,
-                                                         (unsigned)nseg, seg_cols, seg_cols + 1,
-                                                         0, 64, s));
-    } else {
-        FPH_CHECK_HIP(rocprim::radix_sort_keys(ws, t, keys, sorted, (unsigned)K, 0, 64, s));
+        pre = (int*)(w + tmp + 2 * align_up(8 * k));
+        k_seg_prefix<<<1, 256, 0, s>>>(counts, (int)nseg, pre);
+        FPH_LAUNCH_CHECK();
     }
-    k_diag_write<<<fph_blocks(K, 256), 256, 0, s>>>(sorted, K, counts, pivot, col_edge, row_base,
-                                                   row_edge, edge_filt, ess_ascending, seg_cols,
-                                                   seg_ess_asc, (int)nseg, out, out_edges);
+    k_diag_write<<<fph_blocks(K, 256), 256, 0, s>>>(sorted, K, counts, pre, pivot, col_edge,
+                                                   row_base, row_edge, edge_filt, ess_ascending,
+                                                   seg_cols, seg_edges, seg_ess_asc, (int)nseg,
+                                                   out, out_edges);
     FPH_LAUNCH_CHECK();
     return FPH_OK;
 }
