@@ -416,7 +416,24 @@ def main():
     ap.add_argument("--profile-plain", action="store_true",
                     help="only run the n=10K d=10 t=1.6 H0-H2 call: 5 untimed, then 20 "
                          "(default launch parameters; for a kernel trace)")
+    ap.add_argument("--profile-clifford", action="store_true",
+                    help="only run the forward pass of the 64 Clifford clouds (n=2000, d=4, "
+                         "t=0.6, H0+H1) three times (for a kernel trace)")
     args = ap.parse_args()
+    if args.profile_clifford:
+        import torch
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import cases
+        from flash_ph import rips_persistence_batch
+        clouds = [torch.from_numpy(c).cuda() for c in cases.clifford_batch(64, 2000, seed=0)]
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            dg = rips_persistence_batch(clouds, 0.6, max_dim=1)
+            torch.cuda.synchronize()
+            print(f"64 Clifford clouds H0+H1: {(time.perf_counter() - t0) * 1e3:.1f} ms, "
+                  f"H1 rows {sum(d[1].shape[0] for d in dg)}", flush=True)
+        return
     if args.profile_plain:
         import torch
         from flash_ph import rips_persistence
