@@ -2778,12 +2778,14 @@ __global__ __launch_bounds__(kOrdThreads) void k_order_desc(const unsigned char*
 // earlier column (in the reference's order) has settled, and whether it is
 // stored depends on the store's fill after them.  So each wavefront takes
 // the next column of a segment, reduces it as far as the stored columns
-// allow, and commits in order: it waits for the previous position's record
-// (the store's fill after it), re-reads the owner of its pivot, and then
-// continues (a column committed meanwhile stored that pivot) or stops and
-// stores or drops.  A stored pivot is marked pending in own[] before the
-// record is written and the rows follow the record, so the chain of records
-// stays short; a column that meets a pending owner waits for its rows.
+// allow, stages the result in the pool and publishes it as a candidate, and
+// goes on to another column without waiting: candidates are settled in
+// order by settling runs (pipe_hold), which store one by pointing own[p] at
+// its staged rows, drop one that does not fit, or withdraw one whose pivot
+// was stored meanwhile and hand it back to the run's wavefront, which adds
+// on from the staged rows.  A run starts at a position whose predecessor
+// has a record (the store's fill after it), taken by that position's
+// publisher or by the run that reached it, whichever claims it.
 //
 // Coherence: all wavefronts of a segment run on one XCD (an XCD claims a
 // whole segment, and a wavefront only takes columns of its XCD's segment),
@@ -2793,7 +2795,6 @@ __global__ __launch_bounds__(kOrdThreads) void k_order_desc(const unsigned char*
 // is issued.
 
 constexpr int kPipeLdsMax = 20480;                   // LDS words per wavefront at most
-constexpr unsigned long long kPending = 1ull << 63;  // own[]: stored, rows being written
 constexpr unsigned long long kNoRecord = ~0ull;
 constexpr int kPipeSlots = 16;                       // open segments per XCD
 constexpr int kPipeNext = 8 * kPipeSlots;            // ctl: next segment to open
@@ -2813,12 +2814,12 @@ struct PipeArgs {
     unsigned long long* rec;        // per position: store fill after it (used << 32 | words),
                                     // written where a settling run ends
     unsigned long long* fin;        // per position: candidate final pivot and length
-    unsigned long long* verdict;    // per position: stored at / not stored / continue
+    unsigned* stg;                  // per position: pool offset of its staged candidate
     int* claim;                     // per position: 2 once its own wavefront settles from it
     int* ctl;                       // kPipeCtl control words
     unsigned* gbits;                // global bitmaps, wmax words per workgroup (or null)
     long long wmax;
-    long long spin_limit;
+    unsigned long long chunk;       // pool words a wavefront reserves at a time for staging
     int nseg;
     int arena;                      // LDS words per wavefront
     int maxcol;
@@ -2923,31 +2924,31 @@ __device__ __forceinline__ bool pipe_flip_chunk(const unsigned* src, int n, int 
 
 // A column's candidate final pivot, published for the settling run:
 // (p << 32) | len, len = 0 for a column reduced to zero; kNoRecord before.
-// Verdicts: the pool offset of a stored column, or one of these.
-constexpr unsigned long long kNoVerdict = ~0ull;
-constexpr unsigned long long kContinue = ~0ull - 1;  // p was stored meanwhile: add it
-constexpr unsigned long long kUnstored = ~0ull - 2;  // settled, not stored (or zero)
+// Its rows are staged in the pool first ([len, rows...] at stg[position]),
+// so that settling can store it by pointing own[p] there.
 
-// Settles positions from q (claimed by the caller, whose candidate is
+// Settles positions from q (claimed by the caller; q's candidate is
 // published) onwards while they are ready: one round trip reads the next 64
-// candidates, one the owners of their pivots; then, in order, a candidate
-// whose pivot has an owner (stored before the run or by an earlier candidate
-// of it) is told to continue and ends the run, otherwise it is stored if it
-// fits.  Pending owners are drained before the verdicts go out.  The run
-// ends at the first candidate not yet published or at a continued one (the
-// caller's own claim is released first); the record of its last settled
-// position lets that position's wavefront take over.  Positions inside the
-// run need no claim: their predecessors get no record, so their wavefronts
-// only wait for verdicts.  state = used << 32 | words stored.
-__device__ bool pipe_hold(const PipeArgs& P, int s, long long q, unsigned long long state) {
+// candidates, their staging offsets and the owners of their pivots; then,
+// in order, a candidate whose pivot has an owner (stored before the run or
+// by an earlier candidate of it) must continue, which ends the run; any
+// other is stored if it fits (own[p] set to its staged rows) and gets its
+// pivot.  A run that reaches an unpublished position writes the record of
+// its last settled one, then looks at that position again: published in the
+// meantime, it claims it and goes on (its wavefront, having published, may
+// have seen no record yet; one of the two sees the other's write).  Returns
+// -1 when the run ended, or the position of the candidate that must
+// continue, withdrawn and claimed, for the caller to take over (it settles
+// from there itself once it publishes again).  state = used << 32 | words
+// stored.
+__device__ long long pipe_hold(const PipeArgs& P, int s, long long q, unsigned long long state) {
     const RedArgs& a = P.a;
     const int lane = fph_lane();
     const long long* sp = P.seg_pool + kSegPoolWords * s;
-    const long long base = sp[0], cap = sp[1], row_lo = sp[2];
+    const long long cap = sp[1], row_lo = sp[2];
     unsigned long long* own = a.own + (row_lo - a.row_base);
     const long long hi = P.seg_hi[s];
     long long used = (long long)(state >> 32), stored = (long long)(state & 0xFFFFFFFFull);
-    bool first = true;
     long long last = -1;  // last position this run settled
     while (true) {
         const long long pos = q + lane;
@@ -2959,64 +2960,63 @@ __device__ bool pipe_hold(const PipeArgs& P, int s, long long q, unsigned long l
         m2 = min(m2, m);
         const int pl = (int)(f >> 32), ll = (int)(f & 0xFFFFFFFFull);
         unsigned long long o = 0;
-        if (lane < m2 && ll > 0) o = ld_coh64(&own[pl]);
+        unsigned sg = 0;
+        if (lane < m2 && ll > 0) {
+            o = ld_coh64(&own[pl]);
+            sg = ld_coh(&P.stg[pos]);
+        }
         int stop = m2;  // first position not settled
         bool kept = false;
-        unsigned long long vd = kUnstored;
-        long long batch_words = 0;
         for (int l = 0; l < m2; l++) {
             const int p_l = __shfl(pl, l), len_l = __shfl(ll, l);
             const unsigned long long o_l = __shfl(o, l);
-            if (len_l == 0) continue;  // reduced to zero: kUnstored
+            if (len_l == 0) continue;  // reduced to zero
             const bool dup = __ballot(lane < l && kept && pl == p_l) != 0ull;
             if (o_l != kNoOwner || dup) {
                 stop = l;
                 break;
             }
-            bool keep;
-            long long at;
-            if (P.exact) {
-                keep = true;
-                at = batch_words;  // offset in this batch's block, placed below
-                batch_words += ((long long)len_l + 1 + kLine - 1) / kLine * kLine;
-            } else {
+            bool keep = true;
+            if (!P.exact) {
                 keep = used + len_l <= cap;
-                at = base + stored;
                 if (keep) {
                     used += len_l;
                     stored += len_l + 1;
                 }
             }
-            if (lane == l) {
-                kept = keep;
-                vd = keep ? (unsigned long long)at : kUnstored;
-            }
+            if (lane == l) kept = keep;
         }
-        if (P.exact && batch_words > 0) {
-            unsigned long long at0 = 0;
-            if (lane == 0) at0 = atomicAdd(a.pool_ptr, (unsigned long long)batch_words);
-            at0 = __shfl(at0, 0);
-            if (at0 + batch_words > a.pool_cap || at0 + batch_words > 0xFFFFFFFFull)
-                return pipe_fail(P, 1);
-            if (kept) vd += at0;
+        if (lane < stop) {
+            const int c = (int)P.order[pos];
+            if (kept) st_coh64(&own[pl], ((unsigned long long)(c + 1) << 32) | sg);
+            a.pivot[c] = ll > 0 ? (int)(row_lo + pl) : -1;
         }
-        if (lane < stop && kept) st_coh64(&own[pl], kPending);
+        vm_drain();  // the next round's owner reads see these owners
         const bool cont = stop < m2;  // the candidate at `stop` continues
-        if (cont && lane == stop) __hip_atomic_store(&P.claim[pos], 0, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-        vm_drain();
-        if (lane < stop) st_coh64(&P.verdict[pos], vd);
-        if (cont && lane == stop) st_coh64(&P.verdict[pos], kContinue);
+        if (cont && lane == stop) {
+            // withdrawn, and claimed for this wavefront, which takes it over
+            // (its first publisher may still try to claim it: that fails)
+            st_coh64(&P.fin[pos], kNoRecord);
+            __hip_atomic_store(&P.claim[pos], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         if (lane == 0 && stop > 0) atomicAdd(&P.seg_done[s], stop);
         if (stop > 0) last = q + stop - 1;
-        first = false;
         if (stop < m || q + m >= hi) {
             // the run ends: the record of its last settled position
             vm_drain();
             if (lane == 0 && last >= 0)
                 st_coh64(&P.rec[last],
                          ((unsigned long long)used << 32) | (unsigned long long)stored);
-            return true;
+            if (cont) return q + stop;
+            if (stop == m) return -1;  // the segment's end
+            vm_drain();
+            const long long u = q + stop;
+            if (ld_coh64(&P.fin[u]) == kNoRecord) return -1;
+            int won = 0;
+            if (lane == 0) won = atomicCAS(&P.claim[u], 0, 2) == 0;
+            if (!__shfl(won, 0)) return -1;
+            q = u;
+            continue;
         }
         q += m;
     }
@@ -3053,42 +3053,48 @@ __device__ __forceinline__ bool pipe_flip_first(unsigned w, int nb, long long ro
 // serial_bitmap (all zero on entry and on return).  False on error or abort.
 template <bool G>
 __device__ bool pipe_column(const PipeArgs& P, int s, long long r, unsigned* bits,
-                            unsigned* summ, int& nxor, long long& merged, int& longest) {
+                            unsigned* summ, Alloc& al, int& nxor, long long& merged,
+                            int& longest) {
     const RedArgs& a = P.a;
     const int lane = fph_lane();
     const long long* sp = P.seg_pool + kSegPoolWords * s;
     const long long row_lo = sp[2];
     const int rows = (int)sp[3];
     const int S = ((rows + 31) / 32 + 31) / 32;
-    const int c = (int)P.order[r];
-    const int lo_c = a.off[c];
-    const int n0 = min(a.off[c + 1] - lo_c, P.maxcol);
     unsigned long long* own = a.own + (row_lo - a.row_base);  // by local row
-    int len = 0, p = 0, phigh = 0;
-    if (n0 > 0) {
-        int d0 = 0, lm0 = 0;
-        bool bad = false;
-        for (int b0 = 0; b0 < n0; b0 += kPipeUnroll * FPH_WAVE)
-            if (!pipe_flip_chunk<G>(a.data + lo_c, n0, b0, row_lo, rows, bits, summ, d0, lm0))
-                bad = true;
-        if (bad) return pipe_fail(P, 3);
-        len = n0;
-        p = (int)((long long)a.data[lo_c] - row_lo);
-        phigh = (int)((long long)a.data[lo_c + n0 - 1] - row_lo);
-    }
-    Deadline dl;
-    unsigned long long vd = kNoVerdict;
+    // position r's column from the CSR; then, while a settling run hands one
+    // back, a withdrawn candidate from its staged rows
+    long long pos = r;
+    bool adopted = false;
     while (true) {
+        const int c = (int)P.order[pos];
+        const unsigned* src;
+        int n0;
+        if (!adopted) {
+            const int lo_c = a.off[c];
+            src = a.data + lo_c;
+            n0 = min(a.off[c + 1] - lo_c, P.maxcol);
+        } else {
+            const unsigned sg0 = ld_coh(&P.stg[pos]);
+            src = a.pool + sg0 + 1;
+            n0 = (int)ld_coh(a.pool + sg0);
+        }
+        int len = 0, p = 0, phigh = 0;
+        if (n0 > 0) {
+            int d0 = 0, lm0 = 0;
+            bool bad = false;
+            for (int b0 = 0; b0 < n0; b0 += kPipeUnroll * FPH_WAVE)
+                if (!pipe_flip_chunk<G>(src, n0, b0, row_lo, rows, bits, summ, d0, lm0))
+                    bad = true;
+            if (bad) return pipe_fail(P, 3);
+            len = n0;
+            p = (int)((long long)ld_coh(src) - row_lo);
+            phigh = (int)((long long)ld_coh(src + n0 - 1) - row_lo);
+        }
         // add stored columns while the pivot has one
         while (len > 0) {
             unsigned long long v = ld_coh64(&own[p]);
             if (v == kNoOwner) break;
-            while (v & kPending) {  // stored; its rows are being written
-                if (ld_coh_i(&P.ctl[kPipeAbort])) return false;
-                if (dl.expired(P.spin_limit)) return pipe_fail(P, 5);
-                __builtin_amdgcn_s_sleep(1);
-                v = ld_coh64(&own[p]);
-            }
             // [len, rows...]: the length and the first 63 rows in one round trip
             // (a read past the column's end is a pool word, ignored; the pool
             // has 64 words of slack past its capacity)
@@ -3132,71 +3138,61 @@ __device__ bool pipe_column(const PipeArgs& P, int s, long long r, unsigned* bit
             if (!(np > p)) return pipe_fail(P, 3);
             p = np;
         }
-        // a candidate: publish it, then wait for its verdict, or settle it
-        // (and what follows) once every earlier position has settled
-        if (lane == 0)
-            st_coh64(&P.fin[r], ((unsigned long long)(unsigned)p << 32) | (unsigned)len);
-        vm_drain();
-        while (true) {
-            vd = ld_coh64(&P.verdict[r]);
-            if (vd != kNoVerdict) break;
-            const unsigned long long prev = pipe_prev(P, s, r);
-            if (prev != kNoRecord) {
-                int won = 0;
-                if (lane == 0) won = atomicCAS(&P.claim[r], 0, 2) == 0;
-                won = __shfl(won, 0);
-                if (won) {
-                    if (!pipe_hold(P, s, r, prev)) return false;
-                    continue;  // its own verdict is written
+        // the candidate's rows staged in ascending order, the bitmap cleared
+        long long at = 0;
+        if (len > 0) {
+            longest = max(longest, len);
+            at = pool_alloc(a, al, len + 1);
+            if (at < 0) return pipe_fail(P, 1);
+            int written = 0;
+            const int sw_hi = min(phigh >> 10, S - 1);
+            for (int sw0 = p >> 10; sw0 <= sw_hi; sw0 += FPH_WAVE) {
+                const int sw = sw0 + lane;
+                const unsigned val = sw <= sw_hi ? summ[sw] : 0u;
+                int cnt = 0;
+                for (unsigned m = val; m; m &= m - 1)
+                    cnt += __popc(bm_load<G>(&bits[sw * 32 + __builtin_ctz(m)]));
+                int tot;
+                long long o = wave_excl_scan(cnt, &tot) + written;
+                for (unsigned m = val; m; m &= m - 1) {
+                    const int w = sw * 32 + __builtin_ctz(m);
+                    unsigned bw = bm_load<G>(&bits[w]);
+                    for (; bw; bw &= bw - 1)
+                        st_coh(a.pool + at + 1 + o++,
+                               (unsigned)(row_lo + w * 32 + __builtin_ctz(bw)));
+                    bm_zero<G>(&bits[w]);
                 }
+                if (val) summ[sw] = 0u;
+                written += tot;
             }
-            if (ld_coh_i(&P.ctl[kPipeAbort])) return false;
-            if (dl.expired(P.spin_limit)) return pipe_fail(P, 5);
-            __builtin_amdgcn_s_sleep(1);
+            if (written != len) return pipe_fail(P, 3);
+            if (lane == 0) {
+                st_coh(a.pool + at, (unsigned)len);
+                st_coh(&P.stg[pos], (unsigned)at);
+            }
         }
-        if (vd != kContinue) break;
-        // the pivot was stored by an earlier column: withdraw and go on
-        if (lane == 0) {
-            st_coh64(&P.verdict[r], kNoVerdict);
-            st_coh64(&P.fin[r], kNoRecord);
-        }
-        vm_drain();
-    }
-    const long long at = vd == kUnstored ? -1 : (long long)vd;
-    if (lane == 0) a.pivot[c] = len > 0 ? (int)(row_lo + p) : -1;
-    if (len == 0) return true;
-    longest = max(longest, len);
-    // rows out (when stored) in ascending order, and the bitmap cleared
-    int written = 0;
-    const int sw_hi = min(phigh >> 10, S - 1);
-    for (int sw0 = p >> 10; sw0 <= sw_hi; sw0 += FPH_WAVE) {
-        const int sw = sw0 + lane;
-        const unsigned val = sw <= sw_hi ? summ[sw] : 0u;
-        int cnt = 0;
-        for (unsigned m = val; m; m &= m - 1)
-            cnt += __popc(bm_load<G>(&bits[sw * 32 + __builtin_ctz(m)]));
-        int tot;
-        long long o = wave_excl_scan(cnt, &tot) + written;
-        for (unsigned m = val; m; m &= m - 1) {
-            const int w = sw * 32 + __builtin_ctz(m);
-            unsigned bw = bm_load<G>(&bits[w]);
-            if (at >= 0)
-                for (; bw; bw &= bw - 1)
-                    st_coh(a.pool + at + 1 + o++,
-                           (unsigned)(row_lo + w * 32 + __builtin_ctz(bw)));
-            bm_zero<G>(&bits[w]);
-        }
-        if (val) summ[sw] = 0u;
-        written += tot;
-    }
-    if (written != len) return pipe_fail(P, 3);
-    if (at >= 0) {
-        if (lane == 0) st_coh(a.pool + at, (unsigned)len);
+        // publish (rows first), then settle from here if the position before
+        // has settled: this wavefront or the run that reached it, not both
         vm_drain();
         if (lane == 0)
-            st_coh64(&own[p], ((unsigned long long)(c + 1) << 32) | (unsigned long long)at);
+            st_coh64(&P.fin[pos], ((unsigned long long)(unsigned)p << 32) | (unsigned)len);
+        vm_drain();
+        if (ld_coh_i(&P.ctl[kPipeAbort])) return false;
+        const unsigned long long prev = pipe_prev(P, s, pos);
+        if (adopted) {
+            // claimed when it was withdrawn, after the record before it
+            if (prev == kNoRecord) return pipe_fail(P, 5);
+        } else {
+            if (prev == kNoRecord) return true;
+            int won = 0;
+            if (lane == 0) won = atomicCAS(&P.claim[pos], 0, 2) == 0;
+            if (!__shfl(won, 0)) return true;
+        }
+        const long long u = pipe_hold(P, s, pos, prev);
+        if (u < 0) return true;
+        pos = u;  // a withdrawn candidate: add what was stored for its pivot
+        adopted = true;
     }
-    return true;
 }
 
 // One wavefront per workgroup.  Each XCD keeps up to kPipeSlots segments
@@ -3215,6 +3211,7 @@ __global__ __launch_bounds__(FPH_WAVE) void k_reduce_pipe(PipeArgs P) {
     unsigned* gslot = P.gbits ? P.gbits + (long long)blockIdx.x * P.wmax : nullptr;
     int nxor = 0, longest = 0;
     long long merged = 0;
+    Alloc al = make_alloc(P.chunk);
     int* slots = P.ctl + xcc * kPipeSlots;
     while (!ld_coh_i(&P.ctl[kPipeAbort])) {
         const bool all_open = ld_coh_i(&P.ctl[kPipeNext]) >= P.nseg;
@@ -3251,8 +3248,8 @@ __global__ __launch_bounds__(FPH_WAVE) void k_reduce_pipe(PipeArgs P) {
             const int W = (rows + 31) / 32, S = (W + 31) / 32;
             const bool ok =
                 W + S <= P.arena && !P.force_global
-                    ? pipe_column<false>(P, seg, pos, smem + S, smem, nxor, merged, longest)
-                    : pipe_column<true>(P, seg, pos, gslot, smem, nxor, merged, longest);
+                    ? pipe_column<false>(P, seg, pos, smem + S, smem, al, nxor, merged, longest)
+                    : pipe_column<true>(P, seg, pos, gslot, smem, al, nxor, merged, longest);
             if (!ok) break;
             continue;
         }
@@ -3314,6 +3311,13 @@ __global__ void k_pipe_prep(const unsigned* __restrict__ order, const int* __res
 // seg_order: segments by descending residual count (ties by index); up to
 // kPipeRankMax segments, beyond that in index order.
 constexpr int kPipeRankMax = 8192;
+
+// Every position settled (a check of the protocol): status 5 otherwise.
+__global__ void k_pipe_check(const int* __restrict__ seg_lo, const int* __restrict__ seg_hi,
+                             const int* __restrict__ seg_done, int nseg, int* __restrict__ status) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nseg && seg_done[s] != seg_hi[s] - seg_lo[s]) atomicCAS(&status[0], 0, 5);
+}
 
 __global__ void k_pipe_rank(const int* __restrict__ seg_lo, const int* __restrict__ seg_hi,
                             int nseg, int* __restrict__ seg_order) {
@@ -3623,7 +3627,7 @@ namespace {
 struct PipeLayout {
     int grid = 0, arena = 0;
     long long wmax = 0;
-    size_t rec = 0, fin = 0, verdict = 0, claim = 0, seg_lo = 0, seg_hi = 0, seg_next = 0,
+    size_t rec = 0, fin = 0, stg = 0, claim = 0, seg_lo = 0, seg_hi = 0, seg_next = 0,
            seg_done = 0, seg_order = 0, seg_state = 0, ctl = 0, gbits = 0, total = 0;
 };
 
@@ -3653,7 +3657,7 @@ bool pipe_layout(int64_t K, int64_t nseg, int64_t max_rows, int cus, bool force_
     };
     L.rec = take(8 * (size_t)std::max<int64_t>(K, 1));
     L.fin = take(8 * (size_t)std::max<int64_t>(K, 1));
-    L.verdict = take(8 * (size_t)std::max<int64_t>(K, 1));
+    L.stg = take(4 * (size_t)std::max<int64_t>(K, 1));
     L.claim = take(4 * (size_t)std::max<int64_t>(K, 1));
     L.seg_lo = take(4 * (size_t)nseg);
     L.seg_hi = take(4 * (size_t)nseg);
@@ -3683,7 +3687,7 @@ extern "C" int fph_reduce_pipe(int64_t K, const int32_t* col_off, const uint32_t
         return FPH_ERR_ARG;
     if (max_col == 0) max_col = kSerialMaxCol;
     if (max_col < 1 || max_col > 0x7FFFFFFFLL) return FPH_ERR_ARG;
-    int dev = 0, cus = 0, khz = 0;
+    int dev = 0, cus = 0;
     FPH_CHECK_HIP(hipGetDevice(&dev));
     FPH_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     PipeLayout L;
@@ -3733,7 +3737,7 @@ extern "C" int fph_reduce_pipe(int64_t K, const int32_t* col_off, const uint32_t
     P.seg_state = (unsigned long long*)(wb + L.seg_state);
     P.rec = (unsigned long long*)(wb + L.rec);
     P.fin = (unsigned long long*)(wb + L.fin);
-    P.verdict = (unsigned long long*)(wb + L.verdict);
+    P.stg = (unsigned*)(wb + L.stg);
     P.claim = (int*)(wb + L.claim);
     P.ctl = (int*)(wb + L.ctl);
     P.gbits = L.wmax > 0 ? (unsigned*)(wb + L.gbits) : nullptr;
@@ -3744,9 +3748,15 @@ extern "C" int fph_reduce_pipe(int64_t K, const int32_t* col_off, const uint32_t
     P.exact = (variant & 2) ? 1 : 0;
     P.force_global = (variant & 4) ? 1 : 0;
     P.inflight = std::max(8, L.grid / 16);  // half of an XCD's wavefronts
-    FPH_CHECK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-    const long long secs = wait_seconds > 0 ? (long long)wait_seconds : 120LL;
-    P.spin_limit = secs * (long long)std::max(khz, 1) * 1000LL;
+    // staging reservations: a quarter of the free pool spread over the
+    // wavefronts, 256 to 65536 words each
+    {
+        const long long free_words = std::max<long long>(pool_cap - pool_start, 0);
+        long long ch = free_words / (4LL * std::max(L.grid, 1));
+        ch = std::min<long long>(std::max<long long>(ch, 256), 65536);
+        P.chunk = (unsigned long long)(ch / kLine * kLine);
+    }
+    (void)wait_seconds;  // no wavefront waits for another
     k_reduce_init<<<1, 64, 0, s>>>(status, a.pool_ptr, col_off, K, a);
     FPH_LAUNCH_CHECK();
     k_set_u64<<<1, 64, 0, s>>>(a.pool_ptr, (unsigned long long)pool_start);
@@ -3760,7 +3770,6 @@ extern "C" int fph_reduce_pipe(int64_t K, const int32_t* col_off, const uint32_t
         if (orc) return orc;
         k_fill_u64<<<fph_blocks(K, 256), 256, 0, s>>>(P.rec, K, kNoRecord);
         k_fill_u64<<<fph_blocks(K, 256), 256, 0, s>>>(P.fin, K, kNoRecord);
-        k_fill_u64<<<fph_blocks(K, 256), 256, 0, s>>>(P.verdict, K, kNoVerdict);
         k_fill_u32<<<fph_blocks(K, 256), 256, 0, s>>>((unsigned*)P.claim, K, 0u);
         if (P.gbits)
             k_fill_u32<<<fph_blocks((long long)L.grid * L.wmax, 256), 256, 0, s>>>(
@@ -3778,6 +3787,9 @@ extern "C" int fph_reduce_pipe(int64_t K, const int32_t* col_off, const uint32_t
         FPH_CHECK_HIP(hipFuncSetAttribute((const void*)k_reduce_pipe,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         k_reduce_pipe<<<(unsigned)L.grid, FPH_WAVE, lds, s>>>(P);
+        FPH_LAUNCH_CHECK();
+        k_pipe_check<<<fph_blocks(nseg, 256), 256, 0, s>>>(P.seg_lo, P.seg_hi, P.seg_done,
+                                                           (int)nseg, status);
         FPH_LAUNCH_CHECK();
     }
     k_reduce_fini<<<1, 64, 0, s>>>(status, a.pool_ptr);

@@ -561,6 +561,9 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
         caps = ([1 << 62] * len(ref_sizes) if serial_exact else
                 reference_pool_caps([(cof, kc) for cof, kc, _ in ref_sizes]))
         base = _round32(C + K) + 32
+        # fph_reduce_pipe stages every candidate from here on (over the store
+        # regions and bitmaps, which only fph_reduce_serial uses)
+        stage_start = base
         seg_pool, bitmaps = [], []
         row0 = row_base
         for cap, (_, kc, rows) in zip(caps, ref_sizes):
@@ -621,7 +624,7 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
                     and (PIPE_FORCE_GLOBAL or _use_pipe(max_rows, nseg))):
                 call_ws("fph_reduce_pipe",
                         (K, ptr(off), ptr(data), row_base, nrows, ptr(seg_cols), nseg,
-                         ptr(seg_pool_t), max_rows, pool_start, ptr(kind), ptr(own), ptr(pivot),
+                         ptr(seg_pool_t), max_rows, stage_start, ptr(kind), ptr(own), ptr(pivot),
                          ptr(res_list), rescnt_p, ptr(pool), pool_words, ptr(scratch),
                          scratch_bytes),
                         (status_p, state["variant"] | (4 if PIPE_FORCE_GLOBAL else 0),

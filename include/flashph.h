@@ -412,8 +412,10 @@ int fph_reduce_serial(int64_t K, const int32_t *col_off, const uint32_t *col_dat
  * max_rows: the largest seg_pool[3].  Workspace by size query (ws == NULL).
  * variant bit 0 (the sorted-list form) is not taken: status[0] = 4 asks the
  * caller for fph_reduce_serial with bit 0; bit 2 puts every working bitmap
- * in `ws` (for tests of that form on small clouds).  wait_seconds bounds a wait for
- * an earlier column (status 5; <= 0: 120 s).
+ * in `ws` (for tests of that form on small clouds).  Candidates are staged
+ * in `pool` from pool_start on (status 1 when it is too small).
+ * wait_seconds is accepted and unused: no wavefront waits for another.
+ * status[0] = 5 if a position was left unsettled (a protocol check).
  * Replaces _general_cohomology_reduce_v2 (cohomology_general.py:202-371)
  * like fph_reduce_serial.
  */
