@@ -2820,6 +2820,7 @@ struct PipeArgs {
     unsigned* gbits;                // global bitmaps, wmax words per workgroup (or null)
     long long wmax;
     unsigned long long chunk;       // pool words a wavefront reserves at a time for staging
+    long long spin_limit;           // wall-clock ticks a wavefront may find no work
     int nseg;
     int arena;                      // LDS words per wavefront
     int maxcol;
@@ -3212,6 +3213,7 @@ __global__ __launch_bounds__(FPH_WAVE) void k_reduce_pipe(PipeArgs P) {
     int nxor = 0, longest = 0;
     long long merged = 0;
     Alloc al = make_alloc(P.chunk);
+    Deadline idle;  // a wavefront finding no work for spin_limit: status 5 (a scheduling bug)
     int* slots = P.ctl + xcc * kPipeSlots;
     while (!ld_coh_i(&P.ctl[kPipeAbort])) {
         const bool all_open = ld_coh_i(&P.ctl[kPipeNext]) >= P.nseg;
@@ -3244,6 +3246,7 @@ __global__ __launch_bounds__(FPH_WAVE) void k_reduce_pipe(PipeArgs P) {
             }
         }
         if (seg >= 0) {
+            idle.t0 = -1;
             const int rows = (int)P.seg_pool[kSegPoolWords * seg + 3];
             const int W = (rows + 31) / 32, S = (W + 31) / 32;
             const bool ok =
@@ -3255,6 +3258,10 @@ __global__ __launch_bounds__(FPH_WAVE) void k_reduce_pipe(PipeArgs P) {
         }
         if (all_open) {
             if (!left) break;  // nothing left for this XCD
+            if (idle.expired(P.spin_limit)) {
+                pipe_fail(P, 5);
+                break;
+            }
             __builtin_amdgcn_s_sleep(2);
             continue;
         }
@@ -3267,7 +3274,13 @@ __global__ __launch_bounds__(FPH_WAVE) void k_reduce_pipe(PipeArgs P) {
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
-        if (freeable < 0) __builtin_amdgcn_s_sleep(2);
+        if (freeable < 0) {
+            if (idle.expired(P.spin_limit)) {
+                pipe_fail(P, 5);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
         wave_sync();
     }
     if (lane == 0) {
@@ -3756,7 +3769,12 @@ extern "C" int fph_reduce_pipe(int64_t K, const int32_t* col_off, const uint32_t
         ch = std::min<long long>(std::max<long long>(ch, 256), 65536);
         P.chunk = (unsigned long long)(ch / kLine * kLine);
     }
-    (void)wait_seconds;  // no wavefront waits for another
+    {
+        int khz = 0;
+        FPH_CHECK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        const long long secs = wait_seconds > 0 ? (long long)wait_seconds : 120LL;
+        P.spin_limit = secs * (long long)std::max(khz, 1) * 1000LL;
+    }
     k_reduce_init<<<1, 64, 0, s>>>(status, a.pool_ptr, col_off, K, a);
     FPH_LAUNCH_CHECK();
     k_set_u64<<<1, 64, 0, s>>>(a.pool_ptr, (unsigned long long)pool_start);

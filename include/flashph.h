@@ -414,8 +414,9 @@ int fph_reduce_serial(int64_t K, const int32_t *col_off, const uint32_t *col_dat
  * caller for fph_reduce_serial with bit 0; bit 2 puts every working bitmap
  * in `ws` (for tests of that form on small clouds).  Candidates are staged
  * in `pool` from pool_start on (status 1 when it is too small).
- * wait_seconds is accepted and unused: no wavefront waits for another.
- * status[0] = 5 if a position was left unsettled (a protocol check).
+ * No wavefront waits for another; wait_seconds (<= 0: 120 s) bounds how
+ * long one may find no work before the call gives up with status[0] = 5,
+ * which is also set if a position was left unsettled (protocol checks).
  * Replaces _general_cohomology_reduce_v2 (cohomology_general.py:202-371)
  * like fph_reduce_serial.
  */
