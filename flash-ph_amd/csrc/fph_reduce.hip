@@ -2826,8 +2826,7 @@ struct PipeArgs {
     int maxcol;
     int exact;                      // the complete reduction: every column is stored
     int force_global;               // bitmaps in global memory even where LDS holds them
-    int inflight;                   // columns of a segment in flight at most, while
-                                    // unopened segments remain
+    int inflight;                   // columns of a segment handed out and not settled, at most
 };
 
 __device__ __forceinline__ unsigned long long ld_coh64(const unsigned long long* p) {
@@ -2852,14 +2851,17 @@ __device__ bool pipe_fail(const PipeArgs& P, int code) {
     return false;
 }
 
+// Exclusive prefix sum over the wavefront (DPP: within rows of 16 lanes,
+// then across rows); *total = the sum.
 __device__ __forceinline__ int wave_excl_scan(int v, int* total) {
-    const int lane = fph_lane();
     int incl = v;
-    for (int o = 1; o < FPH_WAVE; o <<= 1) {
-        const int y = __shfl_up(incl, o);
-        if (lane >= o) incl += y;
-    }
-    *total = __shfl(incl, FPH_WAVE - 1);
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, false);  // row_shr:1
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xf, 0xf, false);  // row_shr:2
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xf, 0xf, false);  // row_shr:4
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xf, 0xf, false);  // row_shr:8
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    *total = __builtin_amdgcn_readlane(incl, FPH_WAVE - 1);
     return incl - v;
 }
 
@@ -3150,18 +3152,37 @@ __device__ bool pipe_column(const PipeArgs& P, int s, long long r, unsigned* bit
             for (int sw0 = p >> 10; sw0 <= sw_hi; sw0 += FPH_WAVE) {
                 const int sw = sw0 + lane;
                 const unsigned val = sw <= sw_hi ? summ[sw] : 0u;
+                if (__ballot(val != 0u) == 0ull) continue;  // 65,536 rows, none set
+                // a lane's words 8 at a time, their loads in flight together
                 int cnt = 0;
-                for (unsigned m = val; m; m &= m - 1)
-                    cnt += __popc(bm_load<G>(&bits[sw * 32 + __builtin_ctz(m)]));
+                for (unsigned m = val; m;) {
+                    unsigned v8[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        v8[q] = m ? bm_load<G>(&bits[sw * 32 + __builtin_ctz(m)]) : 0u;
+                        m &= m - 1;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; q++) cnt += __popc(v8[q]);
+                }
                 int tot;
                 long long o = wave_excl_scan(cnt, &tot) + written;
-                for (unsigned m = val; m; m &= m - 1) {
-                    const int w = sw * 32 + __builtin_ctz(m);
-                    unsigned bw = bm_load<G>(&bits[w]);
-                    for (; bw; bw &= bw - 1)
-                        st_coh(a.pool + at + 1 + o++,
-                               (unsigned)(row_lo + w * 32 + __builtin_ctz(bw)));
-                    bm_zero<G>(&bits[w]);
+                for (unsigned m = val; m;) {
+                    unsigned v8[8];
+                    int w8[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        w8[q] = m ? sw * 32 + __builtin_ctz(m) : -1;
+                        v8[q] = m ? bm_load<G>(&bits[w8[q]]) : 0u;
+                        m &= m - 1;
+                    }
+                    for (int q = 0; q < 8; q++) {
+                        if (w8[q] < 0) break;
+                        for (unsigned bw = v8[q]; bw; bw &= bw - 1)
+                            st_coh(a.pool + at + 1 + o++,
+                                   (unsigned)(row_lo + w8[q] * 32 + __builtin_ctz(bw)));
+                        bm_zero<G>(&bits[w8[q]]);
+                    }
                 }
                 if (val) summ[sw] = 0u;
                 written += tot;
@@ -3202,7 +3223,10 @@ __device__ bool pipe_column(const PipeArgs& P, int s, long long r, unsigned* bit
 // takes the next column of the first open segment of its XCD with columns
 // left and fewer than `inflight` of them unsettled; when none qualifies it
 // opens the next segment in a free slot (or one whose columns are all
-// handed out).  Once every segment is open the cap is lifted.
+// handed out).  The cap holds throughout: a column handed out far ahead of
+// the settled positions is likely to stop at a pivot stored later and be
+// taken over again (the 256-cloud batch: 92 ms with a cap of 32, 400 ms
+// with none once every segment is open).
 __global__ __launch_bounds__(FPH_WAVE) void k_reduce_pipe(PipeArgs P) {
     extern __shared__ unsigned smem[];
     const int lane = fph_lane();
@@ -3236,7 +3260,7 @@ __global__ __launch_bounds__(FPH_WAVE) void k_reduce_pipe(PipeArgs P) {
                 continue;
             }
             left = true;
-            if (!all_open && nx - ld_coh_i(&P.seg_done[cur]) >= P.inflight) continue;
+            if (nx - ld_coh_i(&P.seg_done[cur]) >= P.inflight) continue;
             int i = 0;
             if (lane == 0) i = atomicAdd(&P.seg_next[cur], 1);
             i = __shfl(i, 0);
