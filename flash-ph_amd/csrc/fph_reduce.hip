@@ -3153,9 +3153,22 @@ __device__ bool pipe_column(const PipeArgs& P, int s, long long r, unsigned* bit
                 const int sw = sw0 + lane;
                 const unsigned val = sw <= sw_hi ? summ[sw] : 0u;
                 if (__ballot(val != 0u) == 0ull) continue;  // 65,536 rows, none set
-                // a lane's words 8 at a time, their loads in flight together
+                // a lane's words 8 at a time, their loads in flight together;
+                // the first 8 are kept for the writing pass
                 int cnt = 0;
-                for (unsigned m = val; m;) {
+                unsigned c8[8];
+                int cw8[8];
+                unsigned m = val;
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    cw8[q] = m ? sw * 32 + __builtin_ctz(m) : -1;
+                    c8[q] = m ? bm_load<G>(&bits[cw8[q]]) : 0u;
+                    m &= m - 1;
+                }
+                const unsigned rest = m;  // words past the first 8
+#pragma unroll
+                for (int q = 0; q < 8; q++) cnt += __popc(c8[q]);
+                while (m) {
                     unsigned v8[8];
 #pragma unroll
                     for (int q = 0; q < 8; q++) {
@@ -3167,7 +3180,14 @@ __device__ bool pipe_column(const PipeArgs& P, int s, long long r, unsigned* bit
                 }
                 int tot;
                 long long o = wave_excl_scan(cnt, &tot) + written;
-                for (unsigned m = val; m;) {
+                for (int q = 0; q < 8; q++) {
+                    if (cw8[q] < 0) break;
+                    for (unsigned bw = c8[q]; bw; bw &= bw - 1)
+                        st_coh(a.pool + at + 1 + o++,
+                               (unsigned)(row_lo + cw8[q] * 32 + __builtin_ctz(bw)));
+                    bm_zero<G>(&bits[cw8[q]]);
+                }
+                for (m = rest; m;) {
                     unsigned v8[8];
                     int w8[8];
 #pragma unroll
