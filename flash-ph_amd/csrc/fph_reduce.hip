@@ -2794,7 +2794,7 @@ __global__ __launch_bounds__(kOrdThreads) void k_order_desc(const unsigned char*
 // one publishes is drained (s_waitcnt vmcnt(0)) before the publishing store
 // is issued.
 
-constexpr int kPipeLdsMax = 20480;                   // LDS words per wavefront at most
+constexpr int kPipeLdsMax = 20224;                   // LDS words per wavefront at most (2 per CU)
 constexpr unsigned long long kNoRecord = ~0ull;
 constexpr int kPipeSlots = 16;                       // open segments per XCD
 constexpr int kPipeNext = 8 * kPipeSlots;            // ctl: next segment to open
@@ -2821,6 +2821,8 @@ struct PipeArgs {
     long long wmax;
     unsigned long long chunk;       // pool words a wavefront reserves at a time for staging
     long long spin_limit;           // wall-clock ticks a wavefront may find no work
+    int window;                     // global bitmaps get an LDS window (pipe_column_win)
+    int win_cap;                    // at most this many 1024-word window units (0: no cap)
     int nseg;
     int arena;                      // LDS words per wavefront
     int maxcol;
@@ -3050,6 +3052,334 @@ __device__ __forceinline__ bool pipe_flip_first(unsigned w, int nb, long long ro
     delta += 2 * __popcll(__ballot(ok && !(old & m))) - __popcll(mok);
     if (mok) lmax = max(lmax, __builtin_amdgcn_readlane(x, 63 - __clzll((long long)mok)));
     return !__ballot(need && !ok);
+}
+
+// Window words of the pipelined reducer's window form: the arena beside tw
+// mark words, a multiple of 1024 (whole summary words), at most cap units
+// of 1024 words (0: no cap).
+__host__ __device__ __forceinline__ int pipe_window_words(int arena, int tw, int cap) {
+    int ww = ((arena - tw) * 32 / 33) & ~1023;
+    if (cap > 0 && ww > cap * 1024) ww = cap * 1024;
+    return ww;
+}
+
+// ---- the pipelined reducer's window form ------------------------------
+//
+// For segments whose bitmap does not fit a wavefront's LDS arena: as the
+// serial reducer's window (Win, above), by one wavefront: the rows of ww
+// words from the column's first pivot on live in LDS with their summary,
+// rows past them are flipped in the wavefront's global slot by atomics not
+// waited for, their words marked in LDS; the window moves to the next
+// marked nonzero word when it empties; the length is an upper bound until
+// the column ends.  A wavefront's LDS operations take effect in issue
+// order, so no barrier is needed; its global flips are drained
+// (s_waitcnt vmcnt(0)) before the slot is read.
+
+// Flips rows src[b0 + u * 64 + lane] (u < kPipeUnroll, < n), all loads
+// first; hi = max(hi, rows flipped); false when a row lies outside
+// [32 wlo, rows).
+__device__ __forceinline__ bool pw_flip_chunk(const Win& W, const unsigned* src, int n, int b0,
+                                              long long row_lo, int rows, int& hi) {
+    const int lane = fph_lane();
+    int x[kPipeUnroll];
+    bool ok[kPipeUnroll];
+    bool bad = false;
+#pragma unroll
+    for (int u = 0; u < kPipeUnroll; u++) {
+        const int e = b0 + u * FPH_WAVE + lane;
+        ok[u] = false;
+        x[u] = 0;
+        if (e < n) {
+            const long long xx = (long long)ld_coh(src + e) - row_lo;
+            ok[u] = xx >= 32LL * W.wlo && xx < rows;
+            bad |= !ok[u];
+            x[u] = ok[u] ? (int)xx : 0;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kPipeUnroll; u++)
+        if (ok[u]) {
+            win_flip(W, x[u]);
+            hi = max(hi, x[u]);
+        }
+    return !__ballot(bad);
+}
+
+// Lowest set row of the window at or above window word j0 (-1 none, -2 the
+// summary and the bitmap disagree): win_search by one wavefront.
+
+// The first nonzero marked global word at or past g0 and at most gmax, or
+// -1; marks of zero words passed are cleared.  After vm_drain().
+__device__ int pw_next_word(Win& W, int g0, int gmax) {
+    const int lane = fph_lane();
+    const int te = min(W.tw - 1, gmax >> 5);
+    for (int tb = g0 >> 5; tb <= te; tb += FPH_WAVE) {
+        const int tw = tb + lane;
+        unsigned bits = tw <= te ? W.ts[tw] : 0u;
+        if (tw == (g0 >> 5)) bits &= ~0u << (g0 & 31);
+        int first = 0x7FFFFFFF;
+        unsigned zero = 0u;
+        for (unsigned m = bits; m; m &= m - 1) {
+            const int g = tw * 32 + __builtin_ctz(m);
+            if (ld_agent(&W.gbits[g])) {
+                first = g;
+                break;
+            }
+            zero |= m & (~m + 1u);
+        }
+        if (zero) W.ts[tw] &= ~zero;
+        for (int o = FPH_WAVE / 2; o; o >>= 1) first = min(first, __shfl_xor(first, o));
+        if (first != 0x7FFFFFFF) return first;
+    }
+    return -1;
+}
+
+// Moves the (empty) window to global word gn: the marked words of [gn, gn +
+// ww) copied in, zeroed and unmarked.
+__device__ void pw_move(Win& W, int gn) {
+    const int lane = fph_lane();
+    W.wlo = gn;
+    const int ge = min(gn + W.ww, W.gw);
+    for (int g = gn + lane; g < ge; g += FPH_WAVE) {
+        if ((W.ts[g >> 5] >> (g & 31)) & 1u) {
+            const unsigned v = ld_agent(&W.gbits[g]);
+            if (v) {
+                const int j = g - gn;
+                W.wb[j] = v;
+                atomicOr(&W.wsum[j >> 5], 1u << (j & 31));
+                __hip_atomic_store(&W.gbits[g], 0u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+    }
+    for (int tw = (gn >> 5) + lane; tw <= (ge - 1) >> 5; tw += FPH_WAVE) {
+        const int lo = max(gn - tw * 32, 0), hi = min(ge - tw * 32, 32);  // bits [lo, hi)
+        const unsigned m = (hi == 32 ? ~0u : (1u << hi) - 1u) & (~0u << lo);
+        atomicAnd(&W.ts[tw], ~m);
+    }
+    vm_drain();
+}
+
+// Set rows of window summary words [s0, s1] and marked global words of mark
+// words [m0, m1].  After vm_drain().
+__device__ int pw_count(const Win& W, int s0, int s1, int m0, int m1) {
+    const int lane = fph_lane();
+    int cnt = 0;
+    for (int sw = s0 + lane; sw <= s1; sw += FPH_WAVE)
+        for (unsigned m = W.wsum[sw]; m; m &= m - 1) cnt += __popc(W.wb[sw * 32 + __builtin_ctz(m)]);
+    for (int tw = m0 + lane; tw <= m1; tw += FPH_WAVE) {
+        unsigned m = W.ts[tw];
+        while (m) {
+            int g[8];
+            unsigned v[8];
+            const int k = win_load8(W, tw, m, g, v);
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (q < k) cnt += __popc(v[q]);
+        }
+    }
+    int tot;
+    (void)wave_excl_scan(cnt, &tot);
+    return tot;
+}
+
+// Writes the column's rows ascending to dst (unless null) and clears it:
+// window summary words [s0, s1], marked global words of mark words [m0, m1]
+// (zeroed, unmarked).  Returns the rows found.  After vm_drain().
+__device__ int pw_drain(Win& W, int s0, int s1, int m0, int m1, unsigned* dst,
+                        long long row_lo) {
+    const int lane = fph_lane();
+    int written = 0;
+    for (int sb = s0; sb <= s1; sb += FPH_WAVE) {
+        const int sw = sb + lane;
+        const unsigned val = sw <= s1 ? W.wsum[sw] : 0u;
+        if (__ballot(val != 0u) == 0ull) continue;
+        int cnt = 0;
+        for (unsigned m = val; m; m &= m - 1) cnt += __popc(W.wb[sw * 32 + __builtin_ctz(m)]);
+        int tot;
+        int at = written + wave_excl_scan(cnt, &tot);
+        for (unsigned m = val; m; m &= m - 1) {
+            const int j = sw * 32 + __builtin_ctz(m);
+            unsigned bw = W.wb[j];
+            if (dst)
+                for (; bw; bw &= bw - 1)
+                    st_coh(dst + at++,
+                           (unsigned)(row_lo + (long long)(W.wlo + j) * 32 + __builtin_ctz(bw)));
+            W.wb[j] = 0u;
+        }
+        if (val) W.wsum[sw] = 0u;
+        written += tot;
+    }
+    for (int tb = m0; tb <= m1; tb += FPH_WAVE) {
+        const int tw = tb + lane;
+        const unsigned bits = tw <= m1 ? W.ts[tw] : 0u;
+        if (__ballot(bits != 0u) == 0ull) continue;
+        int cnt = 0;
+        for (unsigned m = bits; m;) {
+            int g[8];
+            unsigned v[8];
+            const int k = win_load8(W, tw, m, g, v);
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (q < k) cnt += __popc(v[q]);
+        }
+        int tot;
+        int at = written + wave_excl_scan(cnt, &tot);
+        for (unsigned m = bits; m;) {
+            int g[8];
+            unsigned v[8];
+            const int k = win_load8(W, tw, m, g, v);
+            for (int q = 0; q < k; q++) {
+                if (dst)
+                    for (unsigned bw = v[q]; bw; bw &= bw - 1)
+                        st_coh(dst + at++, (unsigned)(row_lo + (long long)g[q] * 32 + __builtin_ctz(bw)));
+                __hip_atomic_store(&W.gbits[g[q]], 0u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+        if (bits) W.ts[tw] = 0u;
+        written += tot;
+    }
+    vm_drain();
+    return written;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = FPH_WAVE / 2; o; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+// pipe_column's window form: column order[r] of segment s (then any
+// candidate a settling run hands back), the window and marks in smem (all
+// zero on entry and on return), the slot gslot.
+__device__ bool pipe_column_win(const PipeArgs& P, int s, long long r, unsigned* smem,
+                                unsigned* gslot, Alloc& al, int& nxor, long long& merged,
+                                int& longest) {
+    const RedArgs& a = P.a;
+    const int lane = fph_lane();
+    const long long* sp = P.seg_pool + kSegPoolWords * s;
+    const long long row_lo = sp[2];
+    const int rows = (int)sp[3];
+    unsigned long long* own = a.own + (row_lo - a.row_base);  // by local row
+    Win W;
+    W.gw = (rows + 31) / 32;
+    W.tw = (W.gw + 31) / 32;
+    W.ww = pipe_window_words(P.arena, W.tw, P.win_cap);
+    W.ts = smem;
+    W.wsum = smem + W.tw;
+    W.wb = smem + W.tw + (W.ww >> 5);
+    W.gbits = gslot;
+    long long pos = r;
+    bool adopted = false;
+    while (true) {
+        const int c = (int)P.order[pos];
+        const unsigned* src;
+        int n0;
+        if (!adopted) {
+            const int lo_c = a.off[c];
+            src = a.data + lo_c;
+            n0 = min(a.off[c + 1] - lo_c, P.maxcol);
+        } else {
+            const unsigned sg0 = ld_coh(&P.stg[pos]);
+            src = a.pool + sg0 + 1;
+            n0 = (int)ld_coh(a.pool + sg0);
+        }
+        int len = 0, p = 0;
+        if (n0 > 0) {
+            const long long p0 = (long long)ld_coh(src) - row_lo;
+            if (p0 < 0 || p0 >= rows) return pipe_fail(P, 3);
+            p = (int)p0;
+            W.wlo = p >> 5;
+            int hi = p;
+            bool bad = false;
+            for (int b0 = 0; b0 < n0; b0 += kPipeUnroll * FPH_WAVE)
+                if (!pw_flip_chunk(W, src, n0, b0, row_lo, rows, hi)) bad = true;
+            long long ub = n0;  // >= the column's length
+            bool zero = false;
+            while (true) {
+                const unsigned long long v = ld_coh64(&own[p]);
+                if (v == kNoOwner) break;
+                // [len, rows...]: the length and the first 63 rows in one round trip
+                const unsigned* ob = a.pool + (v & 0xFFFFFFFFull);
+                const unsigned w0 = ld_coh(ob + lane);
+                const int nb = __builtin_amdgcn_readlane((int)w0, 0);
+                {
+                    const bool need = lane >= 1 && lane <= nb;
+                    const long long xx = (long long)w0 - row_lo;
+                    const bool ok = need && xx >= 32LL * W.wlo && xx < rows;
+                    if (need && !ok) bad = true;
+                    if (ok) {
+                        win_flip(W, (int)xx);
+                        hi = max(hi, (int)xx);
+                    }
+                }
+                for (int b0 = FPH_WAVE - 1; b0 < nb; b0 += kPipeUnroll * FPH_WAVE)
+                    if (!pw_flip_chunk(W, ob + 1, nb, b0, row_lo, rows, hi)) bad = true;
+                merged += nb;
+                nxor++;
+                ub += nb;
+                if (ub > P.maxcol) {  // count the rows
+                    vm_drain();
+                    const int hm = wave_max(hi);
+                    const int n = pw_count(W, 0, (W.ww >> 5) - 1, (W.wlo + W.ww) >> 5,
+                                           min(W.tw - 1, hm >> 10));
+                    if (n > P.maxcol) return pipe_fail(P, 4);  // the reference truncates
+                    ub = n;
+                }
+                int np = win_search(W, (p >> 5) - W.wlo);
+                if (np == -1) {  // the window is empty: move it to the next set row
+                    vm_drain();
+                    const int gn = pw_next_word(W, W.wlo + W.ww, wave_max(hi) >> 5);
+                    if (gn < 0) {
+                        zero = true;
+                        break;
+                    }
+                    pw_move(W, gn);
+                    const unsigned w0n = W.wb[0];
+                    np = w0n ? gn * 32 + __builtin_ctz(w0n) : -1;
+                }
+                if (!(np > p)) return pipe_fail(P, 3);
+                p = np;
+            }
+            if (__ballot(bad)) return pipe_fail(P, 3);
+            vm_drain();
+            if (!zero) {
+                const int hm = wave_max(hi);
+                const int s0 = ((p >> 5) - W.wlo) >> 5;
+                const int s1 = min((W.ww >> 5) - 1, ((hm >> 5) - W.wlo) >> 5);
+                const int m0 = (W.wlo + W.ww) >> 5, m1 = min(W.tw - 1, hm >> 10);
+                len = pw_count(W, s0, s1, m0, m1);
+                longest = max(longest, len);
+                const long long at = pool_alloc(a, al, len + 1);
+                if (at < 0) return pipe_fail(P, 1);
+                const int written = pw_drain(W, s0, s1, m0, m1, a.pool + at + 1, row_lo);
+                if (written != len) return pipe_fail(P, 3);
+                if (lane == 0) {
+                    st_coh(a.pool + at, (unsigned)len);
+                    st_coh(&P.stg[pos], (unsigned)at);
+                }
+            }
+        }
+        // publish (rows first), then settle from here as pipe_column does
+        vm_drain();
+        if (lane == 0)
+            st_coh64(&P.fin[pos], ((unsigned long long)(unsigned)p << 32) | (unsigned)len);
+        vm_drain();
+        if (ld_coh_i(&P.ctl[kPipeAbort])) return false;
+        const unsigned long long prev = pipe_prev(P, s, pos);
+        if (adopted) {
+            if (prev == kNoRecord) return pipe_fail(P, 5);
+        } else {
+            if (prev == kNoRecord) return true;
+            int won = 0;
+            if (lane == 0) won = atomicCAS(&P.claim[pos], 0, 2) == 0;
+            if (!__shfl(won, 0)) return true;
+        }
+        const long long u = pipe_hold(P, s, pos, prev);
+        if (u < 0) return true;
+        pos = u;
+        adopted = true;
+    }
 }
 
 // Column order[r] of segment s, by one wavefront; bits/summ as in
@@ -3293,10 +3623,13 @@ __global__ __launch_bounds__(FPH_WAVE) void k_reduce_pipe(PipeArgs P) {
             idle.t0 = -1;
             const int rows = (int)P.seg_pool[kSegPoolWords * seg + 3];
             const int W = (rows + 31) / 32, S = (W + 31) / 32;
-            const bool ok =
-                W + S <= P.arena && !P.force_global
-                    ? pipe_column<false>(P, seg, pos, smem + S, smem, al, nxor, merged, longest)
-                    : pipe_column<true>(P, seg, pos, gslot, smem, al, nxor, merged, longest);
+            bool ok;
+            if (W + S <= P.arena && !P.force_global)
+                ok = pipe_column<false>(P, seg, pos, smem + S, smem, al, nxor, merged, longest);
+            else if (P.window && pipe_window_words(P.arena, S, P.win_cap) >= kWinMinWords)
+                ok = pipe_column_win(P, seg, pos, smem, gslot, al, nxor, merged, longest);
+            else
+                ok = pipe_column<true>(P, seg, pos, gslot, smem, al, nxor, merged, longest);
             if (!ok) break;
             continue;
         }
@@ -3804,6 +4137,8 @@ extern "C" int fph_reduce_pipe(int64_t K, const int32_t* col_off, const uint32_t
     P.maxcol = (variant & 2) ? 0x7FFFFFFF : (int)max_col;
     P.exact = (variant & 2) ? 1 : 0;
     P.force_global = (variant & 4) ? 1 : 0;
+    P.window = (variant & 8) ? 0 : 1;
+    P.win_cap = (variant >> 8) & 0xFF;
     P.inflight = std::max(8, L.grid / 16);  // half of an XCD's wavefronts
     // staging reservations: a quarter of the free pool spread over the
     // wavefronts, 256 to 65536 words each

@@ -85,17 +85,36 @@ PIPE = True
 # LDS holds them.
 PIPE_FORCE_GLOBAL = False
 # Rows whose bitmap and summary fit the pipelined reducer's LDS arena
-# (20480 words).  Larger segments keep their bitmaps in global memory; a few
-# of them still gain from many columns in flight (n=1000, d=10 at its auto
-# threshold: 2.76M rows, 8.9 vs 32.6 ms), many at once reduce faster one
-# workgroup each (64 Clifford clouds of 2.8M rows: 132 vs 362 ms).
-_PIPE_LDS_WORDS = 20480
+# (20224 words).  Larger segments keep a window of their bitmap in LDS over
+# a global slot (PIPE_WINDOW; False: the whole bitmap in the slot, which
+# loses to the one-workgroup reducer when many segments share the call: 64
+# Clifford clouds of 2.8M rows, 362 vs 132 ms).  Tests: PIPE_WINDOW_UNITS
+# caps the window (1024-word units) so that small clouds move it.
+_PIPE_LDS_WORDS = 20224
 _PIPE_GLOBAL_MAX_SEGS = 8
+PIPE_WINDOW = True
+PIPE_WINDOW_UNITS = 0
 
 
 def _use_pipe(max_rows: int, nseg: int) -> bool:
     w = (max_rows + 31) // 32
-    return w + (w + 31) // 32 <= _PIPE_LDS_WORDS or nseg <= _PIPE_GLOBAL_MAX_SEGS
+    s = (w + 31) // 32
+    if w + s <= _PIPE_LDS_WORDS:
+        return True
+    if PIPE_WINDOW and (((_PIPE_LDS_WORDS - s) * 32 // 33) & ~1023) >= 1024:
+        return True
+    return nseg <= _PIPE_GLOBAL_MAX_SEGS
+
+
+def _pipe_variant(variant: int) -> int:
+    """The serial reducer's variant bits with the pipelined reducer's window
+    settings in place of the serial ones (bit 3: no window, bits 8..15: the
+    window cap)."""
+    v = variant & ~(8 | (0xFF << 8))
+    if not PIPE_WINDOW:
+        v |= 8
+    v |= min(max(int(PIPE_WINDOW_UNITS), 0), 255) << 8
+    return v | (4 if PIPE_FORCE_GLOBAL else 0)
 # When a list, every reduction attempt appends (pool words, status words).
 REDUCE_LOG = None
 
@@ -627,7 +646,7 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
                          ptr(seg_pool_t), max_rows, stage_start, ptr(kind), ptr(own), ptr(pivot),
                          ptr(res_list), rescnt_p, ptr(pool), pool_words, ptr(scratch),
                          scratch_bytes),
-                        (status_p, state["variant"] | (4 if PIPE_FORCE_GLOBAL else 0),
+                        (status_p, _pipe_variant(state["variant"]),
                          REF_MAX_COL, _WAIT_SECONDS, s), device)
             else:
                 if bm_hi > bm_lo:

@@ -68,6 +68,62 @@ def test_pipe_matches_oracle(name, make, t, force_global):
     assert cx.info["h1_stats"]["xors"] == ref["stats"]["xor1"]
 
 
+@pytest.mark.parametrize("mode", ["window32k", "nowindow"])
+@pytest.mark.parametrize("name,make,t", CLOUDS, ids=[c[0] for c in CLOUDS])
+def test_pipe_window(name, make, t, mode):
+    """Bitmaps past the LDS arena (forced here): a window of rows in LDS over
+    the global slot, capped at 32,768 rows so that it moves on these clouds
+    (50K-2.8M rows), and the whole bitmap in the slot."""
+    import oracle
+    from flash_ph import _pipeline
+    x = make()
+    t = oracle.auto_threshold(x) if t is None else t
+    ref = oracle.run(x, t, 1)
+    old = _pipeline.PIPE_WINDOW, _pipeline.PIPE_WINDOW_UNITS
+    _pipeline.PIPE, _pipeline.PIPE_FORCE_GLOBAL = True, True
+    _pipeline.PIPE_WINDOW = mode != "nowindow"
+    _pipeline.PIPE_WINDOW_UNITS = 1
+    try:
+        print(f"[gpu] pipe window {name} {mode}", flush=True)
+        dg, cx = _pipeline.compute(torch.from_numpy(x).cuda(), t, 1, semantics="reference")
+        _eq("h0", dg[0].cpu().numpy(), ref["h0"])
+        _eq("h1", dg[1].cpu().numpy(), ref["h1"])
+        assert cx.info["h1_stats"]["xors"] == ref["stats"]["xor1"]
+    finally:
+        _pipeline.PIPE_WINDOW, _pipeline.PIPE_WINDOW_UNITS = old
+
+
+def test_pipe_window_batch_and_limits():
+    """The window form on a batch (pipe = one-workgroup reducer = oracle) and
+    with a lowered MAX_COL (the length bound's recount, the list-form
+    handover)."""
+    import oracle
+    from flash_ph import _pipeline, rips_persistence_batch
+    old = (_pipeline.PIPE_WINDOW, _pipeline.PIPE_WINDOW_UNITS, _pipeline.REF_MAX_COL)
+    _pipeline.PIPE_FORCE_GLOBAL = True
+    _pipeline.PIPE_WINDOW, _pipeline.PIPE_WINDOW_UNITS = True, 1
+    try:
+        allc = cases.batch_uneven(256, seed=0)
+        clouds = [allc[i] for i in (140, 225, 3, 17, 40, 51)]
+        ts = [float(oracle.auto_threshold(c)) for c in clouds]
+        xs = [torch.from_numpy(c).cuda() for c in clouds]
+        print("[gpu] pipe window batch", flush=True)
+        _pipeline.PIPE = True
+        got = [[d.cpu().numpy() for d in dg] for dg in rips_persistence_batch(xs, ts, 1)]
+        for k in range(len(clouds)):
+            ref = oracle.run(clouds[k], ts[k], 1)
+            _eq(f"window batch cloud {k} h1", got[k][1], ref["h1"])
+        x, t, max_col = cases.randn(700, 3, 3), 0.9, 24
+        print("[gpu] pipe window max_col", flush=True)
+        _pipeline.REF_MAX_COL = max_col
+        ref = oracle.run(x, t, 1, max_col=max_col)
+        assert ref["stats"]["truncations"] > 0
+        dg, _ = _pipeline.compute(torch.from_numpy(x).cuda(), t, 1, semantics="reference")
+        _eq("h1 truncated", dg[1].cpu().numpy(), ref["h1"])
+    finally:
+        _pipeline.PIPE_WINDOW, _pipeline.PIPE_WINDOW_UNITS, _pipeline.REF_MAX_COL = old
+
+
 def test_pipe_heaviest_batch_cloud():
     """The uneven batch's heaviest cloud: the oracle's diagram and additions,
     and the one-workgroup reducer's diagram; both times printed."""
