@@ -609,6 +609,11 @@ def _reduce_and_diagram(K, col_base, off, data, skip, row_cmf, row_base, nrows, 
         if serial_exact:
             pool_words += 64 * K + 4 * C
         variant = _serial_variant(serial_exact)
+        if PIPE and (PIPE_FORCE_GLOBAL or _use_pipe(max_rows, nseg)):
+            # fph_reduce_pipe stages every candidate, stored or not, in
+            # 32-word lines from per-wavefront reservations: twice the
+            # store regions (the uneven batch's heaviest cloud needs 1.5-2x)
+            pool_words = min(stage_start + 2 * (pool_words - stage_start), 0xFFFFFFFF - 64)
     out = _empty(torch.float32, device, max(K, 1), 2)
     out_edges = _empty(torch.int32, device, max(K, 1), 2) if want_edges else None
     state = {"pool_words": pool_words, "variant": variant if ref_sizes is not None else 0}
