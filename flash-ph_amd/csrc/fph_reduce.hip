@@ -2829,6 +2829,7 @@ struct PipeArgs {
     int exact;                      // the complete reduction: every column is stored
     int force_global;               // bitmaps in global memory even where LDS holds them
     int inflight;                   // columns of a segment handed out and not settled, at most
+    int inflight_lds;               // the same for segments whose bitmaps fit the LDS arena
 };
 
 __device__ __forceinline__ unsigned long long ld_coh64(const unsigned long long* p) {
@@ -3610,7 +3611,10 @@ __global__ __launch_bounds__(FPH_WAVE) void k_reduce_pipe(PipeArgs P) {
                 continue;
             }
             left = true;
-            if (nx - ld_coh_i(&P.seg_done[cur]) >= P.inflight) continue;
+            const int rw = (int)((P.seg_pool[kSegPoolWords * cur + 3] + 31) / 32);
+            const int cap = rw + (rw + 31) / 32 <= P.arena && !P.force_global ? P.inflight_lds
+                                                                              : P.inflight;
+            if (nx - ld_coh_i(&P.seg_done[cur]) >= cap) continue;
             int i = 0;
             if (lane == 0) i = atomicAdd(&P.seg_next[cur], 1);
             i = __shfl(i, 0);
@@ -4139,7 +4143,16 @@ extern "C" int fph_reduce_pipe(int64_t K, const int32_t* col_off, const uint32_t
     P.force_global = (variant & 4) ? 1 : 0;
     P.window = (variant & 8) ? 0 : 1;
     P.win_cap = (variant >> 8) & 0xFF;
-    P.inflight = std::max(8, L.grid / 16);  // half of an XCD's wavefronts
+    // Columns of one segment in flight: more of them fill the XCD sooner but
+    // stop more often at pivots whose owners settle later, to be taken over
+    // at the settling front.  A quarter of an XCD's wavefronts for bitmaps in
+    // LDS, an eighth where staging reads a global slot (a takeover costs more
+    // there): at 512 wavefronts 48 and 16 (256 uneven clouds 92.5 -> 88 ms
+    // against 32 for both; n=1000 dense cloud 14.2 -> 12.7 ms).  Bits 16..23
+    // of variant: that many for both.
+    P.inflight = std::max(8, L.grid / 32);
+    P.inflight_lds = std::max(8, 3 * L.grid / 32);
+    if ((variant >> 16) & 0xFF) P.inflight = P.inflight_lds = (variant >> 16) & 0xFF;
     // staging reservations: a quarter of the free pool spread over the
     // wavefronts, 256 to 65536 words each
     {

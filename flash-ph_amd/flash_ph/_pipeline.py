@@ -94,6 +94,9 @@ _PIPE_LDS_WORDS = 20224
 _PIPE_GLOBAL_MAX_SEGS = 8
 PIPE_WINDOW = True
 PIPE_WINDOW_UNITS = 0
+# At most this many unsettled columns per segment (0: a quarter of an XCD's
+# wavefronts for bitmaps in LDS, an eighth else; 1..255 for tuning).
+PIPE_INFLIGHT = 0
 
 
 def _use_pipe(max_rows: int, nseg: int) -> bool:
@@ -109,11 +112,12 @@ def _use_pipe(max_rows: int, nseg: int) -> bool:
 def _pipe_variant(variant: int) -> int:
     """The serial reducer's variant bits with the pipelined reducer's window
     settings in place of the serial ones (bit 3: no window, bits 8..15: the
-    window cap)."""
-    v = variant & ~(8 | (0xFF << 8))
+    window cap, bits 16..23: the in-flight cap)."""
+    v = variant & ~(8 | (0xFF << 8) | (0xFF << 16))
     if not PIPE_WINDOW:
         v |= 8
     v |= min(max(int(PIPE_WINDOW_UNITS), 0), 255) << 8
+    v |= min(max(int(PIPE_INFLIGHT), 0), 255) << 16
     return v | (4 if PIPE_FORCE_GLOBAL else 0)
 # When a list, every reduction attempt appends (pool words, status words).
 REDUCE_LOG = None

@@ -410,7 +410,9 @@ int fph_reduce_serial(int64_t K, const int32_t *col_off, const uint32_t *col_dat
  * / 32) + ceil(max_rows / 1024) <= 20224 words, else a window of rows in
  * LDS over a slot in `ws` (bit 3 of variant: the whole bitmap in the slot,
  * summaries in LDS; bits 8..15: at most that many 1024-word window units,
- * for tests; FPH_ERR_LIMIT past 20224 summary words, about 2.1e7 rows).
+ * for tests; FPH_ERR_LIMIT past 20224 summary words, about 2.1e7 rows;
+ * bits 16..23: at most that many unsettled columns per segment, default
+ * a quarter of an XCD's wavefronts for bitmaps in LDS, an eighth else).
  * max_rows: the largest seg_pool[3].  Workspace by size query (ws == NULL).
  * variant bit 0 (the sorted-list form) is not taken: status[0] = 4 asks the
  * caller for fph_reduce_serial with bit 0; bit 2 puts every working bitmap
