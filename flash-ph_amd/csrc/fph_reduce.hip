@@ -612,6 +612,52 @@ __global__ void k_publish_apparent(RedArgs a, long long K, int set_own) {
     }
 }
 
+// k_publish_apparent without owners or lines, for many columns: a wavefront
+// takes 64 consecutive columns (kinds and offsets in one coalesced round
+// trip), writes their headers, and copies their entries -- contiguous in
+// `data` -- 64 x kPubUnroll at a time, each entry's column found by a
+// binary search over the lanes' offsets.  A wavefront per column waited
+// for a kind, then offsets, then entries for every column.
+constexpr int kPubUnroll = 4;
+__global__ void k_publish_apparent_flat(RedArgs a, long long K) {
+    const int lane = fph_lane();
+    const long long nw = (long long)gridDim.x * (blockDim.x / FPH_WAVE);
+    for (long long c0 = ((long long)blockIdx.x * (blockDim.x / FPH_WAVE) + threadIdx.x / FPH_WAVE) *
+                        FPH_WAVE;
+         c0 < K; c0 += nw * FPH_WAVE) {
+        const long long c = c0 + lane;
+        const bool in = c < K;
+        const int lo = in ? a.off[c] : 0, hi = in ? a.off[c + 1] : 0;
+        const int app = in && a.kind[c] == 2;
+        if (app) a.pool[(long long)lo + c] = (unsigned)(hi - lo);
+        if (!__ballot(app)) continue;
+        const int nin = (int)min<long long>(FPH_WAVE, K - c0);
+        const int b0 = __shfl(lo, 0), b1 = __shfl(hi, nin - 1);
+        for (int base = b0; base < b1; base += FPH_WAVE * kPubUnroll) {
+            unsigned v[kPubUnroll];
+            long long dst[kPubUnroll];
+#pragma unroll
+            for (int u = 0; u < kPubUnroll; u++) {
+                const int i = base + u * FPH_WAVE + lane;
+                int j = 0;  // the last column of the batch starting at or before i
+#pragma unroll
+                for (int step = FPH_WAVE / 2; step; step >>= 1) {
+                    const int t = j + step;
+                    const int lt = __shfl(lo, t < nin ? t : 0);
+                    if (t < nin && lt <= i) j = t;
+                }
+                const int app_j = __shfl(app, j);  // every lane: a cross-lane read
+                const bool take = i < b1 && app_j;
+                dst[u] = take ? (long long)i + c0 + j + 1 : -1;
+                v[u] = take ? a.data[i] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < kPubUnroll; u++)
+                if (dst[u] >= 0) a.pool[dst[u]] = v[u];
+        }
+    }
+}
+
 // Phase A: one wavefront per residual column, chaining apparent owners.
 __global__ __launch_bounds__(kPhaseAWaves * FPH_WAVE) void k_reduce_chain(
     RedArgs a, const int* __restrict__ res_list, const int* __restrict__ res_count) {
@@ -3820,9 +3866,8 @@ extern "C" int fph_reduce_serial(int64_t K, const int32_t* col_off, const uint32
     k_set_u64<<<1, 64, 0, s>>>(a.pool_ptr, (unsigned long long)pool_start);
     FPH_LAUNCH_CHECK();
     if (K > 0) {
-        const unsigned grid_w =
-            (unsigned)std::min<int64_t>(1024, (K + kPhaseAWaves - 1) / kPhaseAWaves);
-        k_publish_apparent<<<grid_w, kPhaseAWaves * FPH_WAVE, 0, s>>>(a, K, 0);
+        k_publish_apparent_flat<<<(unsigned)std::min<int64_t>(16384, (K + 255) / 256), 256, 0,
+                                  s>>>(a, K);
         FPH_LAUNCH_CHECK();
         const int orc = launch_order(kind, res_list, res_count, K, keys, order, sort_tmp, sort_sz, s);
         if (orc) return orc;
@@ -4172,9 +4217,8 @@ extern "C" int fph_reduce_pipe(int64_t K, const int32_t* col_off, const uint32_t
     k_set_u64<<<1, 64, 0, s>>>(a.pool_ptr, (unsigned long long)pool_start);
     FPH_LAUNCH_CHECK();
     if (K > 0) {
-        const unsigned grid_w =
-            (unsigned)std::min<int64_t>(1024, (K + kPhaseAWaves - 1) / kPhaseAWaves);
-        k_publish_apparent<<<grid_w, kPhaseAWaves * FPH_WAVE, 0, s>>>(a, K, 0);
+        k_publish_apparent_flat<<<(unsigned)std::min<int64_t>(16384, (K + 255) / 256), 256, 0,
+                                  s>>>(a, K);
         FPH_LAUNCH_CHECK();
         const int orc = launch_order(kind, res_list, res_count, K, keys, order, sort_tmp, sort_sz, s);
         if (orc) return orc;
