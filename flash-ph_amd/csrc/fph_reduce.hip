@@ -2515,8 +2515,45 @@ __device__ long long serial_prefill(const RedArgs& a, SerShared& sh, int c0, int
                                     long long cap) {
     const int t = threadIdx.x;
     long long used = 0;
-    for (int cb = c0; cb < c1; cb += kSerThreads) {
+    bool fast = true;  // until the store first fills up
+    for (int cb = c0; cb < c1;) {
+        if (fast) {
+            // four chunks at once (kinds and offsets in one round trip, first
+            // rows in another) when all of them fit
+            bool app4[4];
+            int cn4[4], lo4[4], sum = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int c = cb + k * kSerThreads + t;
+                const bool in = c < c1;
+                const unsigned char kd = in ? a.kind[c] : 0;
+                const int lo = in ? a.off[c] : 0, hi = in ? a.off[c + 1] : 0;
+                app4[k] = in && kd == 2;
+                cn4[k] = app4[k] ? hi - lo : 0;
+                lo4[k] = lo;
+                sum += cn4[k];
+            }
+            int total;
+            (void)block_excl_scan(sum, sh, &total);
+            if (used + total <= cap) {
+                unsigned first[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) first[k] = cn4[k] > 0 ? a.data[lo4[k]] : 0u;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (cn4[k] > 0) {
+                        const int c = cb + k * kSerThreads + t;
+                        a.own[(long long)first[k] - a.row_base] =
+                            ((unsigned long long)(c + 1) << 32) | (unsigned long long)(lo4[k] + c);
+                    }
+                used += total;
+                cb += 4 * kSerThreads;
+                continue;
+            }
+            fast = false;
+        }
         const int c = cb + t;
+        cb += kSerThreads;
         const bool app = c < c1 && a.kind[c] == 2;
         const int cn = app ? a.off[c + 1] - a.off[c] : 0;
         int chunk;
